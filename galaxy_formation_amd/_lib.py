@@ -1,0 +1,81 @@
+"""Loads libgf.so (built in-tree by csrc/Makefile) and declares its C ABI
+(include/gf.h).  There is no fallback: if the library is missing or has no
+GPU to run on, every call raises."""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgf.so")
+
+_dbl = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_i32 = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64 = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+_ci, _cd, _cf, _cll = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_longlong
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    # reference engine ABI (cpp/nbody.cpp:415-505; engine.py:26-35)
+    "nbody_num_threads": (_ci, []),
+    "nbody_accel_active": (_ci, [_ci, _dbl, _dbl, _dbl, _cd, _cd, _vp, _ci]),
+    "nbody_accel_direct": (_ci, [_ci, _dbl, _dbl, _dbl, _cd]),
+    "nbody_render": (_ci, [_ci, _dbl, _u8, _ci, _dbl, _dbl, _cd, _ci, _ci, _f32]),
+    "nbody_tonemap": (_ci, [_ci, _ci, _ci, _f32, _f32, _ci, _ci, _f32, _cf, _cf, _cf, _u8]),
+    # library
+    "gf_last_error": (ctypes.c_char_p, []),
+    "gf_version": (_ci, []),
+    "gf_device_info": (_ci, [ctypes.c_char_p, _ci]),
+    "gf_set_precision": (_ci, [_ci]),
+    "gf_get_precision": (_ci, []),
+    "gf_last_interactions": (_cll, []),
+    "gf_accel_direct_subset": (_ci, [_ci, _dbl, _dbl, _ci, _vp, _dbl, _cd]),
+    "gf_tree_info": (_ci, [_vp, _vp, _vp]),
+    "gf_tree_export": (_ci, [_ci] + [_vp] * 10),
+    "gf_gravity_dev": (_ci, [_ci, _vp, _vp, _vp, _cd, _cd, _vp, _ci, _ci, _vp]),
+    "gf_direct_dev": (_ci, [_ci, _vp, _vp, _ci, _vp, _vp, _cd, _vp]),
+    "gf_interactions_dev": (_ci, [_vp]),
+    "gf_sync": (_ci, [_vp]),
+}
+
+_lib = None
+
+
+class GFError(RuntimeError):
+    """A libgf.so entry point failed (message from gf_last_error)."""
+
+
+def load():
+    """The library, with argtypes declared.  Raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GFError(f"{LIB_PATH} is missing: build it first (python -c "
+                          "'import __graft_entry__ as g; g.build()' or make -C "
+                          "galaxy_formation_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    """Raises GFError with the library's message for a negative status."""
+    if rc < 0:
+        msg = load().gf_last_error()
+        raise GFError(msg.decode() if msg else f"libgf call failed ({rc})")
+    return rc
+
+
+def ptr(a):
+    """ctypes pointer of a numpy array (None stays None)."""
+    return None if a is None else a.ctypes.data_as(_vp)

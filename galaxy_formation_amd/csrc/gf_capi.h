@@ -1,0 +1,33 @@
+// Library context shared by the C ABI translation units.
+#pragma once
+#include "../../include/gf.h"
+#include "gf_gravity.h"
+
+#define GF_ABI_VERSION 1
+
+struct GfCtx {
+    int inited = 0;
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    int precision = GF_FP32;
+    unsigned long long last_inter = 0;
+    GfTree tree;
+    GfWalkScratch walk;
+    // staging buffers of the host-pointer entry points (grown, never shrunk)
+    double* d_pos = nullptr; size_t d_pos_cap = 0;
+    double* d_mass = nullptr; size_t d_mass_cap = 0;
+    double* d_acc = nullptr; size_t d_acc_cap = 0;
+    uint8_t* d_act = nullptr; size_t d_act_cap = 0;
+    int64_t* d_tgt = nullptr; size_t d_tgt_cap = 0;
+    double* d_partial = nullptr; size_t d_partial_cap = 0;
+    void* d_img = nullptr; size_t d_img_cap = 0;
+    void* d_glow = nullptr; size_t d_glow_cap = 0;
+    void* d_out8 = nullptr; size_t d_out8_cap = 0;
+    void* d_small = nullptr; size_t d_small_cap = 0;
+};
+
+extern GfCtx g_ctx;
+int gf_ctx_init();
+hipStream_t gf_stream(void* s);
+int gf_stage_reserve(void** p, size_t* cap, size_t bytes);

@@ -1,0 +1,306 @@
+// C ABI of libgf.so: the five symbols of the reference engine
+// (cpp/nbody.cpp:415-505, bound by galaxy_formation/engine.py:26-35) with the
+// same arguments, plus gf_* entry points for the device-resident path, the
+// SPH kernels, the learned smoothing length and parity checks.
+//
+// State kept between calls, as the reference keeps its global tree
+// (cpp/nbody.cpp:307): the last gravity tree (for rebuild = 0 refits) and
+// staging buffers.  Entry points are not re-entrant.  Host-pointer entry
+// points run on the library's stream and return when their outputs are in
+// host memory; *_dev entry points take device pointers and a stream (null =
+// the library's stream) and return without synchronising.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "gf_capi.h"
+#include "gf_gravity.h"
+#include "gf_render.h"
+
+static char g_err[1024] = "";
+
+void gf_set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+GfCtx g_ctx;
+
+int gf_ctx_init() {
+    if (g_ctx.inited) return 0;
+    int count = 0;
+    const hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        gf_set_error("no HIP device available (%s)", hipGetErrorString(e));
+        return -1;
+    }
+    GF_HIP(hipGetDevice(&g_ctx.device));
+    GF_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    GF_HIP(hipGetDeviceProperties(&prop, g_ctx.device));
+    g_ctx.n_cu = prop.multiProcessorCount;
+    g_ctx.inited = 1;
+    return 0;
+}
+
+hipStream_t gf_stream(void* s) {
+    return s ? reinterpret_cast<hipStream_t>(s) : g_ctx.stream;
+}
+
+int gf_stage_reserve(void** p, size_t* cap, size_t bytes) {
+    if (bytes <= *cap && *p) return 0;
+    if (*p) GF_HIP(hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    const size_t b = bytes ? bytes : 16;
+    GF_HIP(hipMalloc(p, b));
+    *cap = b;
+    return 0;
+}
+
+#define STAGE(ptr, bytes) gf_stage_reserve(reinterpret_cast<void**>(&(ptr)), &ptr##_cap, (bytes))
+
+static int check_walk_errors() {
+    int oc[2] = {0, 0};
+    GF_HIP(hipMemcpy(oc, g_ctx.walk.ovf_count, sizeof(oc), hipMemcpyDeviceToHost));
+    if (oc[1] != 0) {
+        gf_set_error("tree walk: %d groups overflowed the global stack", oc[1]);
+        return -1;
+    }
+    return 0;
+}
+
+static int gravity_impl(int n, const double* pos, const double* mass, double* acc, double theta,
+                        double eps, const uint8_t* active, int rebuild, int precision,
+                        hipStream_t s) {
+    GfTree& T = g_ctx.tree;
+    const float th = float(theta);
+    if (rebuild || !T.built || T.n != n) {
+        if (gf_tree_build(T, n, pos, mass, th, s)) return -1;
+    } else {
+        if (gf_tree_refit(T, pos, mass, s)) return -1;
+    }
+    if (n == 0) return 0;
+    if (gf_walk_reserve(g_ctx.walk, T.cap_nodes)) return -1;
+    GF_HIP(hipMemsetAsync(g_ctx.walk.inter, 0, sizeof(unsigned long long), s));
+    return gf_walk(T, g_ctx.walk, active, acc, eps, precision, s);
+}
+
+extern "C" {
+
+const char* gf_last_error(void) { return g_err; }
+
+int gf_version(void) { return GF_ABI_VERSION; }
+
+int gf_device_info(char* buf, int len) {
+    if (gf_ctx_init()) return -1;
+    hipDeviceProp_t prop;
+    GF_HIP(hipGetDeviceProperties(&prop, g_ctx.device));
+    snprintf(buf, size_t(len), "%s %s CUs=%d LDS/block=%zu clock_kHz=%d", prop.name,
+             prop.gcnArchName, prop.multiProcessorCount, prop.sharedMemPerBlock, prop.clockRate);
+    return 0;
+}
+
+int nbody_num_threads(void) {
+    if (gf_ctx_init()) return -1;
+    return g_ctx.n_cu;
+}
+
+int gf_set_precision(int p) {
+    if (p != GF_FP32 && p != GF_FP64) {
+        gf_set_error("precision must be 0 (FP32 interactions) or 1 (FP64)");
+        return -1;
+    }
+    g_ctx.precision = p;
+    return 0;
+}
+
+int gf_get_precision(void) { return g_ctx.precision; }
+
+int nbody_accel_active(int n, const double* pos, const double* mass, double* acc, double theta,
+                       double eps, const uint8_t* active, int rebuild) {
+    if (gf_ctx_init()) return -1;
+    if (n < 0) {
+        gf_set_error("n must be >= 0");
+        return -1;
+    }
+    hipStream_t s = g_ctx.stream;
+    const size_t b3 = sizeof(double) * 3 * size_t(n), b1 = sizeof(double) * size_t(n);
+    if (STAGE(g_ctx.d_pos, b3) || STAGE(g_ctx.d_mass, b1) || STAGE(g_ctx.d_acc, b3) ||
+        STAGE(g_ctx.d_act, size_t(n)))
+        return -1;
+    if (n > 0) {
+        GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
+        GF_HIP(hipMemcpyAsync(g_ctx.d_mass, mass, b1, hipMemcpyHostToDevice, s));
+        GF_HIP(hipMemcpyAsync(g_ctx.d_acc, acc, b3, hipMemcpyHostToDevice, s));
+        if (active) GF_HIP(hipMemcpyAsync(g_ctx.d_act, active, size_t(n), hipMemcpyHostToDevice, s));
+    }
+    if (gravity_impl(n, g_ctx.d_pos, g_ctx.d_mass, g_ctx.d_acc, theta, eps,
+                     active ? g_ctx.d_act : nullptr, rebuild, g_ctx.precision, s))
+        return -1;
+    if (n > 0) GF_HIP(hipMemcpyAsync(acc, g_ctx.d_acc, b3, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    if (n > 0) {
+        if (check_walk_errors()) return -1;
+        GF_HIP(hipMemcpy(&g_ctx.last_inter, g_ctx.walk.inter, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int nbody_accel_direct(int n, const double* pos, const double* mass, double* acc, double eps) {
+    return gf_accel_direct_subset(n, pos, mass, n, nullptr, acc, eps);
+}
+
+int gf_accel_direct_subset(int n, const double* pos, const double* mass, int nt,
+                           const int64_t* tgt, double* acc, double eps) {
+    if (gf_ctx_init()) return -1;
+    if (n <= 0 || nt <= 0) return 0;
+    hipStream_t s = g_ctx.stream;
+    const size_t b3 = sizeof(double) * 3 * size_t(n), b1 = sizeof(double) * size_t(n);
+    const size_t bt = sizeof(double) * 3 * size_t(nt);
+    if (STAGE(g_ctx.d_pos, b3) || STAGE(g_ctx.d_mass, b1) || STAGE(g_ctx.d_acc, bt) ||
+        STAGE(g_ctx.d_tgt, sizeof(int64_t) * size_t(nt)) ||
+        STAGE(g_ctx.d_partial, sizeof(double) * gf_direct_partial_doubles(n, nt)))
+        return -1;
+    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(g_ctx.d_mass, mass, b1, hipMemcpyHostToDevice, s));
+    if (tgt) GF_HIP(hipMemcpyAsync(g_ctx.d_tgt, tgt, sizeof(int64_t) * nt, hipMemcpyHostToDevice, s));
+    if (gf_direct(n, g_ctx.d_pos, g_ctx.d_mass, nt, tgt ? g_ctx.d_tgt : nullptr, g_ctx.d_acc, eps,
+                  g_ctx.d_partial, s))
+        return -1;
+    GF_HIP(hipMemcpyAsync(acc, g_ctx.d_acc, bt, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+long long gf_last_interactions(void) { return (long long)g_ctx.last_inter; }
+
+int gf_tree_info(int* nodes, int* groups, int* deep_cells) {
+    if (gf_ctx_init()) return -1;
+    int c[4] = {0, 0, 0, 0};
+    if (g_ctx.tree.counters) GF_HIP(hipMemcpy(c, g_ctx.tree.counters, sizeof(c), hipMemcpyDeviceToHost));
+    if (nodes) *nodes = c[0];
+    if (groups) *groups = c[1];
+    if (deep_cells) *deep_cells = c[2];
+    return 0;
+}
+
+int gf_tree_export(int cap, int* start, int* count, int* next, int* leaf, int* depth,
+                   double* com4, double* open_r, double* box4, int* idx, int* groups) {
+    if (gf_ctx_init()) return -1;
+    const GfTree& T = g_ctx.tree;
+    int nn = 0, ng = 0;
+    if (gf_tree_info(&nn, &ng, nullptr)) return -1;
+    if (nn > cap) {
+        gf_set_error("tree export: %d nodes do not fit in %d", nn, cap);
+        return -1;
+    }
+    const size_t ni = sizeof(int) * size_t(nn);
+    if (start) GF_HIP(hipMemcpy(start, T.start, ni, hipMemcpyDeviceToHost));
+    if (count) GF_HIP(hipMemcpy(count, T.count, ni, hipMemcpyDeviceToHost));
+    if (next) GF_HIP(hipMemcpy(next, T.next, ni, hipMemcpyDeviceToHost));
+    if (leaf || depth) {
+        int8_t* tmp = new int8_t[size_t(nn) + 1];
+        if (leaf) {
+            GF_HIP(hipMemcpy(tmp, T.leaf, size_t(nn), hipMemcpyDeviceToHost));
+            for (int k = 0; k < nn; ++k) leaf[k] = tmp[k];
+        }
+        if (depth) {
+            GF_HIP(hipMemcpy(tmp, T.depth, size_t(nn), hipMemcpyDeviceToHost));
+            for (int k = 0; k < nn; ++k) depth[k] = tmp[k];
+        }
+        delete[] tmp;
+    }
+    if (com4) GF_HIP(hipMemcpy(com4, T.com, sizeof(double4) * size_t(nn), hipMemcpyDeviceToHost));
+    if (open_r) GF_HIP(hipMemcpy(open_r, T.open_r, sizeof(double) * size_t(nn), hipMemcpyDeviceToHost));
+    if (box4) GF_HIP(hipMemcpy(box4, T.box, sizeof(double4) * size_t(nn), hipMemcpyDeviceToHost));
+    if (idx) GF_HIP(hipMemcpy(idx, T.idx, sizeof(int) * size_t(T.n), hipMemcpyDeviceToHost));
+    if (groups) GF_HIP(hipMemcpy(groups, T.groups, sizeof(int) * size_t(ng), hipMemcpyDeviceToHost));
+    return nn;
+}
+
+int gf_gravity_dev(int n, const double* pos, const double* mass, double* acc, double theta,
+                   double eps, const uint8_t* active, int rebuild, int precision, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (n < 0 || (precision != GF_FP32 && precision != GF_FP64)) {
+        gf_set_error("gf_gravity_dev: bad n or precision");
+        return -1;
+    }
+    return gravity_impl(n, pos, mass, acc, theta, eps, active, rebuild, precision,
+                        gf_stream(stream));
+}
+
+int gf_direct_dev(int n, const double* pos, const double* mass, int nt, const int64_t* tgt,
+                  double* acc, double eps, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (STAGE(g_ctx.d_partial, sizeof(double) * gf_direct_partial_doubles(n, nt))) return -1;
+    return gf_direct(n, pos, mass, nt, tgt, acc, eps, g_ctx.d_partial, gf_stream(stream));
+}
+
+int gf_interactions_dev(long long* out) {
+    if (gf_ctx_init()) return -1;
+    unsigned long long v = 0;
+    if (g_ctx.walk.inter) GF_HIP(hipMemcpy(&v, g_ctx.walk.inter, sizeof(v), hipMemcpyDeviceToHost));
+    *out = (long long)v;
+    return 0;
+}
+
+int gf_sync(void* stream) {
+    if (gf_ctx_init()) return -1;
+    GF_HIP(hipStreamSynchronize(gf_stream(stream)));
+    if (g_ctx.walk.ovf_count && check_walk_errors()) return -1;
+    return 0;
+}
+
+int nbody_render(int n, const double* pos, const uint8_t* group, int ngroups, const double* rot,
+                 const double* center, double scale, int W, int H, float* img) {
+    if (gf_ctx_init()) return -1;
+    hipStream_t s = g_ctx.stream;
+    const size_t bimg = sizeof(float) * size_t(ngroups) * W * H;
+    if (STAGE(g_ctx.d_pos, sizeof(double) * 3 * size_t(n)) || STAGE(g_ctx.d_act, size_t(n)) ||
+        STAGE(g_ctx.d_img, bimg) || STAGE(g_ctx.d_small, sizeof(double) * 12))
+        return -1;
+    if (n > 0) {
+        GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, sizeof(double) * 3 * size_t(n), hipMemcpyHostToDevice, s));
+        GF_HIP(hipMemcpyAsync(g_ctx.d_act, group, size_t(n), hipMemcpyHostToDevice, s));
+    }
+    double small[12];
+    memcpy(small, rot, sizeof(double) * 9);
+    memcpy(small + 9, center, sizeof(double) * 3);
+    GF_HIP(hipMemcpyAsync(g_ctx.d_small, small, sizeof(small), hipMemcpyHostToDevice, s));
+    double* dsm = reinterpret_cast<double*>(g_ctx.d_small);
+    if (gf_render_deposit(n, g_ctx.d_pos, g_ctx.d_act, ngroups, dsm, dsm + 9, scale, W, H,
+                          reinterpret_cast<float*>(g_ctx.d_img), s))
+        return -1;
+    GF_HIP(hipMemcpyAsync(img, g_ctx.d_img, bimg, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int nbody_tonemap(int W, int H, int ngroups, const float* img, const float* glow, int gw, int gh,
+                  const float* colors, float inv_ref, float K, float exposure, uint8_t* out) {
+    if (gf_ctx_init()) return -1;
+    hipStream_t s = g_ctx.stream;
+    const size_t bimg = sizeof(float) * size_t(ngroups) * W * H;
+    const size_t bglow = sizeof(float) * size_t(ngroups) * gw * gh;
+    const size_t bout = 3 * size_t(W) * H;
+    if (STAGE(g_ctx.d_img, bimg) || STAGE(g_ctx.d_glow, bglow) || STAGE(g_ctx.d_out8, bout) ||
+        STAGE(g_ctx.d_small, sizeof(float) * 3 * size_t(ngroups) + 96))
+        return -1;
+    GF_HIP(hipMemcpyAsync(g_ctx.d_img, img, bimg, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(g_ctx.d_glow, glow, bglow, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(g_ctx.d_small, colors, sizeof(float) * 3 * size_t(ngroups), hipMemcpyHostToDevice, s));
+    if (gf_render_tonemap(W, H, ngroups, reinterpret_cast<const float*>(g_ctx.d_img),
+                          reinterpret_cast<const float*>(g_ctx.d_glow), gw, gh,
+                          reinterpret_cast<const float*>(g_ctx.d_small), inv_ref, K, exposure,
+                          reinterpret_cast<uint8_t*>(g_ctx.d_out8), s))
+        return -1;
+    GF_HIP(hipMemcpyAsync(out, g_ctx.d_out8, bout, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// Shared definitions of the gfx950 force-path kernels (wave64, HIP).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Tree constants of the reference octree (cpp/nbody.cpp:33-35).
+#define GF_LEAF_SIZE 8
+#define GF_GROUP_SIZE 64
+#define GF_MAX_DEPTH 48
+// Octant digits held by one 63-bit Morton key (3 bits per level).
+#define GF_KEY_LEVELS 21
+
+#define GF_WAVE 64
+
+// ------------------------------------------------------------ error state
+// Host-side error reporting: every entry point returns 0 on success and a
+// negative code on failure, with the message kept for gf_last_error().
+void gf_set_error(const char* fmt, ...);
+
+#define GF_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            gf_set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #call,               \
+                         hipGetErrorString(e_));                                          \
+            return -1;                                                                    \
+        }                                                                                 \
+    } while (0)
+
+#define GF_LAUNCH_CHECK() GF_HIP(hipGetLastError())
+
+// ------------------------------------------------------------- device helpers
+__device__ __forceinline__ int gf_lane() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t gf_lanemask_lt() {
+    const int l = __lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ int gf_wave_incl_scan(int v) {
+    const int l = __lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (l >= o) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int gf_wave_sum_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double gf_wave_min_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ double gf_wave_max_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ double gf_wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ float gf_wave_sum_f(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Octant digit of a key at depth d (1..GF_KEY_LEVELS); digit 1 is the root split.
+__device__ __forceinline__ int gf_key_digit(uint64_t key, int d) {
+    return int((key >> (3 * (GF_KEY_LEVELS - d))) & 7u);
+}
+
+// Key prefix made of the first d digits (d = 0 gives 0 for every key).
+__device__ __forceinline__ uint64_t gf_key_prefix(uint64_t key, int d) {
+    return d == 0 ? 0ull : (key >> (3 * (GF_KEY_LEVELS - d)));
+}
+
+static inline int gf_div_up(long long a, long long b) { return int((a + b - 1) / b); }

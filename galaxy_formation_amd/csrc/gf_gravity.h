@@ -1,0 +1,31 @@
+// Barnes-Hut group walk + list evaluation, and the FP64 direct sum (gf_gravity.hip).
+#pragma once
+#include "gf_tree.h"
+
+enum GfPrecision { GF_FP32 = 0, GF_FP64 = 1 };
+
+struct GfWalkScratch {
+    int* work = nullptr;                   // 1 + 8 ints: group cursor(s)
+    unsigned long long* inter = nullptr;   // interaction counter
+    int* ovf = nullptr;                    // groups whose LDS stack overflowed
+    int* ovf_count = nullptr;
+    int* gstack = nullptr;                 // global stacks for those groups
+    int cap_groups = 0;
+};
+
+int gf_walk_reserve(GfWalkScratch& W, int max_groups);
+void gf_walk_free(GfWalkScratch& W);
+
+// Accelerations of the active particles (active: input order, may be null =
+// all) from the tree T, written to acc (n, 3) in input order; inactive rows
+// and rows of massless groups are left untouched (cpp/nbody.cpp:344-365,
+// :404-407).  Adds the interaction count to W.inter.
+int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* acc, double eps,
+            int precision, hipStream_t s);
+
+// FP64 direct sum (cpp/nbody.cpp:428-441) for nt targets (tgt: indices, or
+// null for all n with nt == n).  partial: 3 * nt * nsplit doubles of scratch
+// (gf_direct_partial_doubles).
+size_t gf_direct_partial_doubles(int n, int nt);
+int gf_direct(int n, const double* pos, const double* mass, int nt, const int64_t* tgt,
+              double* acc, double eps, double* partial, hipStream_t s);

@@ -1,0 +1,481 @@
+// Barnes-Hut group walk and list evaluation (cpp/nbody.cpp:323-411), and the
+// FP64 direct sum (cpp/nbody.cpp:428-441).
+//
+// One wave64 walks the tree for one group (the outermost nodes holding <= 64
+// particles, nbody.cpp:234-243) and evaluates the resulting interaction list
+// for every particle of the group.
+//   * The walk is a LIFO of node indices in LDS processed 64 nodes at a time
+//     (one node per lane).  The multipole acceptance test is the reference's,
+//     |com - b|^2 > (open_r + br)^2 with (b, br) the sphere around the
+//     group's massive members (nbody.cpp:350-375), evaluated in FP64 on FP64
+//     centres of mass, so both evaluation precisions use one interaction set.
+//     Accepted nodes append their monopole to the list; opened leaves append
+//     their particles; opened internal nodes push their children.
+//   * The list lives in LDS and is evaluated whenever it fills.  Small groups
+//     split the list over several lanes per particle ("slices"), so a group
+//     of P particles keeps 64 / P lanes busy per particle.
+//   * FP32 evaluation works on coordinates relative to the group centre b,
+//     rounded to FP32 once per list entry and group; each flush's FP32 partial
+//     sum is added to a compensated (two-sum) FP32 accumulator.
+//   * FP64 evaluation uses v_rsq_f64 refined by one Newton step.
+// A group whose LDS stack would overflow is queued and walked again by a
+// second launch whose stacks live in global memory (bounded by 512 entries
+// per tree level: the LIFO keeps entries sorted by depth, and all entries of
+// one depth were pushed by one batch of 64 nodes with at most 8 children each).
+#include <type_traits>
+
+#include "gf_gravity.h"
+
+namespace {
+
+constexpr int SCAP = 1024;                          // LDS stack entries
+constexpr int LCAP = 512;                           // LDS list entries
+constexpr int GSCAP = 512 * (GF_KEY_LEVELS + 2);    // global stack entries per block
+constexpr int WALK_BLOCKS = 8192;
+constexpr int GST_BLOCKS = 256;
+
+struct WalkArgs {
+    const int* start;
+    const int* count;
+    const int8_t* leaf;
+    const int* child8;
+    const double4* com;
+    const double* open_r;
+    const int* groups;
+    const int* counters;
+    const double* tx;
+    const double* ty;
+    const double* tz;
+    const double* tm;
+    const uint32_t* idx;
+    const uint8_t* active;
+    double* acc;
+    double eps2;
+    int* work;                 // [0] LDS-stack launch, [1] global-stack launch
+    unsigned long long* inter;
+    int* ovf;
+    int* ovf_count;            // [0] queued groups, [1] global-stack overflows (error)
+    int* gstack;
+};
+
+__device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v) {
+    const float s = hi + v;
+    const float bp = s - hi;
+    const float err = (hi - (s - bp)) + (v - bp);
+    hi = s;
+    lo += err;
+}
+
+template <bool F64>
+struct Acc;
+
+template <>
+struct Acc<false> {
+    float hx = 0, hy = 0, hz = 0, lx = 0, ly = 0, lz = 0;
+    float px, py, pz;
+    __device__ void flush(const float4* L, int ln, int sl, int S, float eps2) {
+        float ax = 0, ay = 0, az = 0;
+        int j = sl;
+        for (; j + 3 * S < ln; j += 4 * S) {
+            float4 e[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) e[u] = L[j + u * S];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float dx = e[u].x - px, dy = e[u].y - py, dz = e[u].z - pz;
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float f = e[u].w * (inv * inv * inv);
+                ax = fmaf(f, dx, ax);
+                ay = fmaf(f, dy, ay);
+                az = fmaf(f, dz, az);
+            }
+        }
+        for (; j < ln; j += S) {
+            const float4 e = L[j];
+            const float dx = e.x - px, dy = e.y - py, dz = e.z - pz;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            const float f = e.w * (inv * inv * inv);
+            ax = fmaf(f, dx, ax);
+            ay = fmaf(f, dy, ay);
+            az = fmaf(f, dz, az);
+        }
+        two_sum_add(hx, lx, ax);
+        two_sum_add(hy, ly, ay);
+        two_sum_add(hz, lz, az);
+    }
+    __device__ double3 total() const {
+        return make_double3(double(hx) + double(lx), double(hy) + double(ly),
+                            double(hz) + double(lz));
+    }
+};
+
+template <>
+struct Acc<true> {
+    double ax = 0, ay = 0, az = 0;
+    double px, py, pz;
+    __device__ void flush(const double4* L, int ln, int sl, int S, double eps2) {
+        for (int j = sl; j < ln; j += S) {
+            const double4 e = L[j];
+            const double dx = e.x - px, dy = e.y - py, dz = e.z - pz;
+            const double r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, eps2)));
+            double y = __builtin_amdgcn_rsq(r2);
+            y = y * fma(-0.5 * r2, y * y, 1.5);
+            const double f = e.w * (y * y * y);
+            ax = fma(f, dx, ax);
+            ay = fma(f, dy, ay);
+            az = fma(f, dz, az);
+        }
+    }
+    __device__ double3 total() const { return make_double3(ax, ay, az); }
+};
+
+template <bool F64, bool GST>
+__global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
+    // group sphere and acceptance test: the operations as written (no FMA
+    // contraction), so the interaction set is the one the FP64 oracle forms
+#pragma clang fp contract(off)
+    using Ent = typename std::conditional<F64, double4, float4>::type;
+    __shared__ int s_stack[GST ? 1 : SCAP];
+    __shared__ Ent s_list[LCAP];
+    __shared__ double s_red[3][64];
+    const int lane = __lane_id();
+    int* stack = GST ? a.gstack + size_t(blockIdx.x) * GSCAP : s_stack;
+    const int scap = GST ? GSCAP : SCAP;
+    const int ngroups = GST ? a.ovf_count[0] : a.counters[1];
+    int* cursor = a.work + (GST ? 1 : 0);
+    using Real = typename std::conditional<F64, double, float>::type;
+    const Real eps2r = Real(a.eps2);     // FP32: float(eps * eps), as nbody.cpp:330
+
+    while (true) {
+        int gi = 0;
+        if (lane == 0) gi = atomicAdd(cursor, 1);
+        gi = __shfl(gi, 0, 64);
+        if (gi >= ngroups) break;
+        const int G = a.groups[GST ? a.ovf[gi] : gi];
+        const int b = a.start[G], np = a.count[G];
+
+        // activity and the sphere around the massive members (FP64)
+        bool any_act = false;
+        double x0 = 1e300, y0 = 1e300, z0 = 1e300, x1 = -1e300, y1 = -1e300, z1 = -1e300;
+        for (int c0 = 0; c0 < np; c0 += 64) {
+            const int i = b + c0 + lane;
+            const bool in = c0 + lane < np;
+            bool act = false;
+            if (in) {
+                act = a.active ? a.active[a.idx[i]] != 0 : true;
+                if (a.tm[i] > 0) {
+                    const double x = a.tx[i], y = a.ty[i], z = a.tz[i];
+                    x0 = fmin(x0, x); x1 = fmax(x1, x);
+                    y0 = fmin(y0, y); y1 = fmax(y1, y);
+                    z0 = fmin(z0, z); z1 = fmax(z1, z);
+                }
+            }
+            any_act |= __ballot(act) != 0;
+        }
+        if (!any_act) continue;
+        x0 = gf_wave_min_d(x0); y0 = gf_wave_min_d(y0); z0 = gf_wave_min_d(z0);
+        x1 = gf_wave_max_d(x1); y1 = gf_wave_max_d(y1); z1 = gf_wave_max_d(z1);
+        if (x0 > x1) continue;                                   // massless group
+        const double bx = 0.5 * (x0 + x1), by = 0.5 * (y0 + y1), bz = 0.5 * (z0 + z1);
+        double br2 = 0;
+        for (int c0 = 0; c0 < np; c0 += 64) {
+            const int i = b + c0 + lane;
+            if (c0 + lane < np && a.tm[i] > 0) {
+                const double dx = a.tx[i] - bx, dy = a.ty[i] - by, dz = a.tz[i] - bz;
+                br2 = fmax(br2, dx * dx + dy * dy + dz * dz);
+            }
+        }
+        const double br = sqrt(gf_wave_max_d(br2));
+
+        for (int c0 = 0; c0 < np; c0 += 64) {
+            const int P = min(64, np - c0);
+            const int S = 64 / P;
+            const int p = lane % P, sl = lane / P;
+            const bool evl = sl < S;
+            const int ip = b + c0 + p;
+            Acc<F64> A;
+            if constexpr (F64) {
+                A.px = a.tx[ip]; A.py = a.ty[ip]; A.pz = a.tz[ip];
+            } else {
+                A.px = float(a.tx[ip] - bx); A.py = float(a.ty[ip] - by); A.pz = float(a.tz[ip] - bz);
+            }
+            bool chunk_act = false;
+            if (lane < P) chunk_act = a.active ? a.active[a.idx[ip]] != 0 : true;
+            const int n_act = __popcll(__ballot(chunk_act));
+            if (n_act == 0) continue;
+
+            int sn = 1, ln = 0;
+            long long total_len = 0;
+            bool overflow = false;
+            if (lane == 0) stack[0] = 0;
+            __syncthreads();
+            while (sn > 0) {
+                const int take = min(64, sn);
+                sn -= take;
+                const int node = lane < take ? stack[sn + lane] : -1;
+                __syncthreads();
+                int push = 0, nent = 0, first = 0;
+                bool mono = false;
+                int4 c_lo = make_int4(-1, -1, -1, -1), c_hi = c_lo;
+                double4 cm = make_double4(0, 0, 0, 0);
+                if (node >= 0) {
+                    cm = a.com[node];
+                    const double dx = cm.x - bx, dy = cm.y - by, dz = cm.z - bz;
+                    const double t = a.open_r[node] + br;
+                    if (dx * dx + dy * dy + dz * dz > t * t) {
+                        mono = true;
+                        nent = 1;
+                    } else if (a.leaf[node]) {
+                        first = a.start[node];
+                        nent = a.count[node];
+                    } else {
+                        const int4* cs = reinterpret_cast<const int4*>(a.child8 + 8 * size_t(node));
+                        c_lo = cs[0];
+                        c_hi = cs[1];
+                        push = (c_lo.x >= 0) + (c_lo.y >= 0) + (c_lo.z >= 0) + (c_lo.w >= 0) +
+                               (c_hi.x >= 0) + (c_hi.y >= 0) + (c_hi.z >= 0) + (c_hi.w >= 0);
+                    }
+                }
+                // push children (lane order keeps the stack sorted by depth)
+                const int pinc = gf_wave_incl_scan(push);
+                const int ptot = __shfl(pinc, 63, 64);
+                if (sn + ptot > scap) {
+                    overflow = true;
+                    break;
+                }
+                {
+                    int o = sn + pinc - push;
+                    const int cc[8] = {c_lo.x, c_lo.y, c_lo.z, c_lo.w, c_hi.x, c_hi.y, c_hi.z, c_hi.w};
+#pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                        if (cc[k] >= 0) stack[o++] = cc[k];
+                }
+                sn += ptot;
+                // append list entries, evaluating the list whenever it is full
+                int rem = nent, src = 0;
+                while (true) {
+                    const int einc = gf_wave_incl_scan(rem);
+                    const int etot = __shfl(einc, 63, 64);
+                    if (etot == 0) break;
+                    if (ln == LCAP) {
+                        __syncthreads();
+                        if (evl) A.flush(s_list, ln, sl, S, eps2r);
+                        total_len += ln;
+                        ln = 0;
+                        __syncthreads();
+                    }
+                    const int space = LCAP - ln;
+                    const int my0 = einc - rem;
+                    const int canw = max(0, min(rem, space - my0));
+                    for (int k = 0; k < canw; ++k) {
+                        const int slot = ln + my0 + k;
+                        if (mono) {
+                            if constexpr (F64) {
+                                s_list[slot] = cm;
+                            } else {
+                                s_list[slot] = make_float4(float(cm.x - bx), float(cm.y - by),
+                                                           float(cm.z - bz), float(cm.w));
+                            }
+                        } else {
+                            const int j = first + src + k;
+                            if constexpr (F64) {
+                                s_list[slot] = make_double4(a.tx[j], a.ty[j], a.tz[j], a.tm[j]);
+                            } else {
+                                s_list[slot] = make_float4(float(a.tx[j] - bx), float(a.ty[j] - by),
+                                                           float(a.tz[j] - bz), float(a.tm[j]));
+                            }
+                        }
+                    }
+                    rem -= canw;
+                    src += canw;
+                    ln += min(etot, space);
+                }
+                __syncthreads();
+            }
+            if (overflow) {
+                if (!GST) {
+                    if (lane == 0) a.ovf[atomicAdd(&a.ovf_count[0], 1)] = gi;
+                } else if (lane == 0) {
+                    atomicAdd(&a.ovf_count[1], 1);
+                }
+                __syncthreads();
+                break;                    // the whole group is redone by the queued walk
+            }
+            if (ln > 0) {
+                __syncthreads();
+                if (evl) A.flush(s_list, ln, sl, S, eps2r);
+                total_len += ln;
+                ln = 0;
+            }
+            // reduce the slices of each particle in FP64
+            double3 v = make_double3(0, 0, 0);
+            if (evl) v = A.total();
+            __syncthreads();
+            s_red[0][lane] = v.x;
+            s_red[1][lane] = v.y;
+            s_red[2][lane] = v.z;
+            __syncthreads();
+            if (lane < P && chunk_act) {
+                double sx = 0, sy = 0, sz = 0;
+                for (int q = 0; q < S; ++q) {
+                    sx += s_red[0][q * P + lane];
+                    sy += s_red[1][q * P + lane];
+                    sz += s_red[2][q * P + lane];
+                }
+                const size_t o = 3 * size_t(a.idx[ip]);
+                a.acc[o] = sx;
+                a.acc[o + 1] = sy;
+                a.acc[o + 2] = sz;
+            }
+            __syncthreads();
+            if (lane == 0)
+                atomicAdd(a.inter, (unsigned long long)total_len * (unsigned long long)n_act);
+        }
+    }
+}
+
+__global__ void k_walk_reset(int* work, int* ovf_count) {
+    if (threadIdx.x == 0) {
+        work[0] = 0;
+        work[1] = 0;
+        ovf_count[0] = 0;
+        ovf_count[1] = 0;
+    }
+}
+
+// ------------------------------------------------------------ direct sum
+constexpr int DT = 256;
+
+__global__ __launch_bounds__(DT) void k_direct(int n, const double* __restrict__ pos,
+                                               const double* __restrict__ mass, int nt,
+                                               const int64_t* __restrict__ tgt, double eps2,
+                                               int per_split, double* __restrict__ partial) {
+    __shared__ double4 src[DT];
+    const int t = blockIdx.x * DT + threadIdx.x;
+    const int split = blockIdx.y;
+    const bool ok = t < nt;
+    const long long i = ok ? (tgt ? tgt[t] : t) : 0;
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    double ax = 0, ay = 0, az = 0;
+    const int j0 = split * per_split, j1 = min(n, j0 + per_split);
+    for (int jb = j0; jb < j1; jb += DT) {
+        const int j = jb + threadIdx.x;
+        __syncthreads();
+        src[threadIdx.x] = j < j1 ? make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], mass[j])
+                                  : make_double4(0, 0, 0, 0);
+        __syncthreads();
+        const int m = min(DT, j1 - jb);
+        for (int k = 0; k < m; ++k) {
+            const double4 s = src[k];
+            const double dx = s.x - x, dy = s.y - y, dz = s.z - z;
+            const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+            const double f = s.w / (r2 * sqrt(r2));
+            ax += f * dx;
+            ay += f * dy;
+            az += f * dz;
+        }
+    }
+    if (ok) {
+        const size_t o = 3 * (size_t(split) * nt + t);
+        partial[o] = ax;
+        partial[o + 1] = ay;
+        partial[o + 2] = az;
+    }
+}
+
+__global__ __launch_bounds__(DT) void k_direct_reduce(int nt, int nsplit,
+                                                      const double* __restrict__ partial,
+                                                      double* __restrict__ acc) {
+    const int t = blockIdx.x * DT + threadIdx.x;
+    if (t >= nt) return;
+    double ax = 0, ay = 0, az = 0;
+    for (int s = 0; s < nsplit; ++s) {
+        const size_t o = 3 * (size_t(s) * nt + t);
+        ax += partial[o];
+        ay += partial[o + 1];
+        az += partial[o + 2];
+    }
+    acc[3 * size_t(t)] = ax;
+    acc[3 * size_t(t) + 1] = ay;
+    acc[3 * size_t(t) + 2] = az;
+}
+
+int direct_splits(int n, int nt) {
+    const int tb = gf_div_up(nt, DT);
+    const int want = std::max(1, 2048 / std::max(1, tb));
+    return std::max(1, std::min(want, gf_div_up(n, DT)));
+}
+
+}  // namespace
+
+int gf_walk_reserve(GfWalkScratch& W, int max_groups) {
+    if (!W.work) {
+        GF_HIP(hipMalloc(&W.work, 16 * sizeof(int)));
+        GF_HIP(hipMalloc(&W.inter, sizeof(unsigned long long)));
+        GF_HIP(hipMalloc(&W.ovf_count, 4 * sizeof(int)));
+        GF_HIP(hipMalloc(&W.gstack, sizeof(int) * size_t(GST_BLOCKS) * GSCAP));
+        GF_HIP(hipMemset(W.inter, 0, sizeof(unsigned long long)));
+    }
+    if (max_groups > W.cap_groups) {
+        if (W.ovf) GF_HIP(hipFree(W.ovf));
+        GF_HIP(hipMalloc(&W.ovf, sizeof(int) * size_t(std::max(1, max_groups))));
+        W.cap_groups = max_groups;
+    }
+    return 0;
+}
+
+void gf_walk_free(GfWalkScratch& W) {
+    if (W.work) (void)hipFree(W.work);
+    if (W.inter) (void)hipFree(W.inter);
+    if (W.ovf_count) (void)hipFree(W.ovf_count);
+    if (W.gstack) (void)hipFree(W.gstack);
+    if (W.ovf) (void)hipFree(W.ovf);
+    W = GfWalkScratch();
+}
+
+int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* acc, double eps,
+            int precision, hipStream_t s) {
+    if (T.n == 0) return 0;
+    if (!T.built) {
+        gf_set_error("gravity walk without a built tree");
+        return -1;
+    }
+    if (gf_walk_reserve(W, T.cap_nodes)) return -1;
+    WalkArgs a;
+    a.start = T.start; a.count = T.count; a.leaf = T.leaf; a.child8 = T.child8;
+    a.com = T.com; a.open_r = T.open_r; a.groups = T.groups; a.counters = T.counters;
+    a.tx = T.tx; a.ty = T.ty; a.tz = T.tz; a.tm = T.tm; a.idx = T.idx;
+    a.active = active; a.acc = acc; a.eps2 = eps * eps;
+    a.work = W.work; a.inter = W.inter; a.ovf = W.ovf; a.ovf_count = W.ovf_count;
+    a.gstack = W.gstack;
+    hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, W.work, W.ovf_count);
+    if (precision == GF_FP64) {
+        hipLaunchKernelGGL((k_walk<true, false>), dim3(WALK_BLOCKS), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_walk<true, true>), dim3(GST_BLOCKS), dim3(64), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_walk<false, false>), dim3(WALK_BLOCKS), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_walk<false, true>), dim3(GST_BLOCKS), dim3(64), 0, s, a);
+    }
+    GF_LAUNCH_CHECK();
+    return 0;
+}
+
+size_t gf_direct_partial_doubles(int n, int nt) {
+    return 3 * size_t(std::max(nt, 1)) * size_t(direct_splits(n, nt));
+}
+
+int gf_direct(int n, const double* pos, const double* mass, int nt, const int64_t* tgt,
+              double* acc, double eps, double* partial, hipStream_t s) {
+    if (nt <= 0 || n <= 0) return 0;
+    const int ns = direct_splits(n, nt);
+    const int per = gf_div_up(gf_div_up(n, ns), DT) * DT;
+    const int nsplit = gf_div_up(n, per);
+    hipLaunchKernelGGL(k_direct, dim3(gf_div_up(nt, DT), nsplit), dim3(DT), 0, s, n, pos, mass,
+                       nt, tgt, eps * eps, per, partial);
+    hipLaunchKernelGGL(k_direct_reduce, dim3(gf_div_up(nt, DT)), dim3(DT), 0, s, nt, nsplit,
+                       partial, acc);
+    GF_LAUNCH_CHECK();
+    return 0;
+}

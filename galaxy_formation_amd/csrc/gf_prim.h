@@ -1,0 +1,17 @@
+// Device-wide scan and radix sort (gf_prim.hip).
+#pragma once
+#include "gf_common.h"
+
+// Ints of temporary storage gf_scan_excl needs for n inputs.
+size_t gf_scan_temp_ints(int n);
+// out[i] = in[0] + ... + in[i-1] for i in [0, n]; out has n + 1 entries
+// (out[n] is the total).  in == out is allowed.
+int gf_scan_excl(const int* in, int* out, int n, int* temp, hipStream_t s);
+
+// Ints of temporary storage gf_radix_sort_pairs needs for n pairs.
+size_t gf_sort_temp_ints(int n);
+// Stable LSD radix sort of (key, val) on key bits [begin_bit, end_bit), 8 bits
+// per pass.  The sorted pairs end in (key_alt, val_alt) when *in_alt is set on
+// return, else in (key, val).
+int gf_radix_sort_pairs(uint64_t* key, uint32_t* val, uint64_t* key_alt, uint32_t* val_alt, int n,
+                        int begin_bit, int end_bit, int* temp, hipStream_t s, int* in_alt);

@@ -1,0 +1,237 @@
+// Device-wide primitives: exclusive scan of int32 and a stable LSD radix sort
+// of (uint64 key, uint32 value) pairs.  Both are written for wave64: ranks
+// inside a wave come from ballots, block-level offsets from LDS.
+#include "gf_prim.h"
+
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+// Exclusive scan of the 256 per-thread values of a block; returns the
+// thread's prefix and writes the block total to *total.
+__device__ int block_excl_scan_256(int v, int* lds /*4 ints*/, int* total) {
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    int inc = gf_wave_incl_scan(v);
+    if (lane == 63) lds[w] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int t = lds[i];
+        if (i < w) off += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int* __restrict__ in, int n,
+                                                              int* __restrict__ partial) {
+    __shared__ int lds[4];
+    const long long base = (long long)blockIdx.x * SCAN_TILE;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        long long i = base + (long long)k * SCAN_THREADS + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    int tot;
+    block_excl_scan_256(s, lds, &tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// One block of 1024 threads scans nb partials in place (exclusive) and stores
+// the grand total at partial[nb].
+__global__ __launch_bounds__(1024) void k_scan_partials(int* partial, int nb) {
+    __shared__ int lds[16];
+    const int t = threadIdx.x, lane = __lane_id(), w = t >> 6;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = t * per;
+    int s = 0;
+    for (int i = 0; i < per; ++i)
+        if (b0 + i < nb) s += partial[b0 + i];
+    int inc = gf_wave_incl_scan(s);
+    if (lane == 63) lds[w] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int i = 0; i < 16; ++i) {
+        if (i < w) off += lds[i];
+        tot += lds[i];
+    }
+    int run = off + inc - s;
+    for (int i = 0; i < per; ++i) {
+        if (b0 + i < nb) {
+            int v = partial[b0 + i];
+            partial[b0 + i] = run;
+            run += v;
+        }
+    }
+    if (t == 0) partial[nb] = tot;
+}
+
+// in and out may be the same array: every thread reads its items before the
+// block barrier and writes only those items after it.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int* in, int n,
+                                                            const int* __restrict__ partial,
+                                                            int nb, int* out) {
+    __shared__ int lds[4];
+    const long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        long long i = base + k;
+        v[k] = i < n ? in[i] : 0;
+        s += v[k];
+    }
+    int tot;
+    int run = block_excl_scan_256(s, lds, &tot) + partial[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        long long i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partial[nb];
+}
+
+// ------------------------------------------------------------------ radix sort
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;                       // per lane
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;     // 4096 keys per block
+constexpr int RS_WAVE_SPAN = 64 * RS_ITEMS;        // 1024 contiguous keys per wave
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ key, int n,
+                                                        int shift, int nblocks,
+                                                        int* __restrict__ hist) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int k = 0; k < RS_ITEMS; ++k) {
+        long long i = base + (long long)k * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(key[i] >> shift) & 0xFF], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: wave w of block b owns keys [b*TILE + w*1024, +1024), read
+// as 16 chunks of 64.  Inside a chunk, lanes holding the same digit are
+// ranked by lane order (8 ballots); chunks are ranked in order through a
+// per-wave digit counter; waves are ranked in order through an LDS prefix.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
+    const uint64_t* __restrict__ key, const uint32_t* __restrict__ val, int n, int shift,
+    int nblocks, const int* __restrict__ offs, uint64_t* __restrict__ key_out,
+    uint32_t* __restrict__ val_out) {
+    __shared__ int cnt[4][256];
+    __shared__ int wbase[4][256];
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < 4 * 256; i += RS_THREADS) (&cnt[0][0])[i] = 0;
+    __syncthreads();
+
+    const long long base = (long long)blockIdx.x * RS_TILE + (long long)w * RS_WAVE_SPAN;
+    uint64_t k_[RS_ITEMS];
+    uint32_t v_[RS_ITEMS];
+    int r_[RS_ITEMS];
+    const uint64_t lt = gf_lanemask_lt();
+#pragma unroll
+    for (int c = 0; c < RS_ITEMS; ++c) {
+        const long long i = base + (long long)c * 64 + lane;
+        const bool ok = i < n;
+        k_[c] = ok ? key[i] : 0ull;
+        v_[c] = ok ? val[i] : 0u;
+        const int d = int((k_[c] >> shift) & 0xFF);
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        int pre = 0;
+        if (ok) pre = cnt[w][d];
+        const int rank = __popcll(peers & lt);
+        const bool leader = ok && (peers & lt) == 0;
+        // every lane has read the counter before any leader writes it: the
+        // wave issues its LDS operations in program order
+        __builtin_amdgcn_wave_barrier();
+        if (leader) cnt[w][d] = pre + __popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        r_[c] = ok ? pre + rank : -1;
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;   // one digit per thread
+        int run = offs[d * nblocks + blockIdx.x];
+#pragma unroll
+        for (int ww = 0; ww < 4; ++ww) {
+            wbase[ww][d] = run;
+            run += cnt[ww][d];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < RS_ITEMS; ++c) {
+        if (r_[c] >= 0) {
+            const int d = int((k_[c] >> shift) & 0xFF);
+            const int dst = wbase[w][d] + r_[c];
+            key_out[dst] = k_[c];
+            val_out[dst] = v_[c];
+        }
+    }
+}
+
+}  // namespace
+
+size_t gf_scan_temp_ints(int n) { return size_t(gf_div_up(n, SCAN_TILE) + 1); }
+
+int gf_scan_excl(const int* in, int* out, int n, int* partial, hipStream_t s) {
+    if (n <= 0) {
+        GF_HIP(hipMemsetAsync(out, 0, sizeof(int), s));
+        return 0;
+    }
+    const int nb = gf_div_up(n, SCAN_TILE);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, partial);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, partial, nb);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, partial, nb, out);
+    GF_LAUNCH_CHECK();
+    return 0;
+}
+
+size_t gf_sort_temp_ints(int n) {
+    const size_t nh = size_t(256) * gf_div_up(n, RS_TILE);
+    return nh + 1 + gf_scan_temp_ints(int(nh));
+}
+
+int gf_radix_sort_pairs(uint64_t* key, uint32_t* val, uint64_t* key_alt, uint32_t* val_alt, int n,
+                        int begin_bit, int end_bit, int* temp, hipStream_t s, int* in_alt) {
+    *in_alt = 0;
+    if (n <= 1) return 0;
+    const int nb = gf_div_up(n, RS_TILE);
+    const int nh = 256 * nb;
+    int* hist = temp;
+    int* partial = temp + nh + 1;   // hist[nh] receives the total; partials follow
+    uint64_t *ka = key, *kb = key_alt;
+    uint32_t *va = val, *vb = val_alt;
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, s, ka, n, shift, nb, hist);
+        // exclusive scan of the bin-major histogram, in place
+        const int nbs = gf_div_up(nh, SCAN_TILE);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nbs), dim3(SCAN_THREADS), 0, s, hist, nh, partial);
+        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, partial, nbs);
+        hipLaunchKernelGGL(k_scan_down, dim3(nbs), dim3(SCAN_THREADS), 0, s, hist, nh, partial,
+                           nbs, hist);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_THREADS), 0, s, ka, va, n, shift, nb,
+                           hist, kb, vb);
+        GF_LAUNCH_CHECK();
+        uint64_t* tk = ka; ka = kb; kb = tk;
+        uint32_t* tv = va; va = vb; vb = tv;
+        *in_alt ^= 1;
+    }
+    return 0;
+}

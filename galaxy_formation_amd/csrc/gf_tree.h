@@ -1,0 +1,56 @@
+// Device octree of the Barnes-Hut gravity (gf_tree.hip).
+#pragma once
+#include "gf_common.h"
+
+// All pointers are device memory.  Particle arrays are indexed by tree
+// position (Morton order) unless noted; node arrays by DFS index, the layout
+// of cpp/nbody.cpp:176-200 (preorder, children in octant order 0..7).
+struct GfTree {
+    int n = 0;          // particles the held tree was built for
+    int cap_n = 0;      // particle capacity of the buffers
+    int cap_nodes = 0;  // node capacity of the buffers
+    int built = 0;      // a tree is held (refit allowed)
+    float theta = 0.7f; // opening angle of the held tree (cast to float, nbody.cpp:327)
+
+    // per particle
+    uint64_t *key = nullptr, *key_alt = nullptr;
+    uint32_t *idx = nullptr, *idx_alt = nullptr;  // tree position -> input index
+    int8_t *lcp = nullptr, *d0 = nullptr;
+    int *chain = nullptr;                          // n + 1, then node base per position
+    double *tx = nullptr, *ty = nullptr, *tz = nullptr, *tm = nullptr;
+    double *tdisp = nullptr;                       // displacement since the build
+    double *xb = nullptr;                          // positions at the build (input order, AoS)
+
+    // per node
+    int *start = nullptr, *count = nullptr, *next = nullptr, *parent = nullptr;
+    int8_t *leaf = nullptr, *depth = nullptr;
+    int *child8 = nullptr;                         // 8 slots per node, octant order, -1 = empty
+    int *arrive = nullptr;
+    double4 *com = nullptr;                        // centre of mass and mass (FP64)
+    double *open_r = nullptr;                      // opening radius (FP64)
+    double4 *box = nullptr;                        // geometric centre and half-width (FP64)
+    double *ndisp = nullptr;                       // largest displacement below the node
+    int *gflag = nullptr;                          // nodes + 1, then group offsets
+    int *groups = nullptr;                         // group nodes, DFS order
+
+    // scalars on the device: [0] nodes, [1] groups, [2] cells cut at the key
+    // depth with more than GF_LEAF_SIZE particles, [3] spare
+    int *counters = nullptr;
+    double *root = nullptr;                        // root centre (3) and half-width
+    double *bbox_part = nullptr;
+    int *scan_tmp = nullptr, *sort_tmp = nullptr;
+};
+
+// Node capacity that always suffices for n particles: at most n leaves and,
+// per depth, at most n / (GF_LEAF_SIZE + 1) internal nodes.
+int gf_tree_node_bound(int n);
+
+int gf_tree_reserve(GfTree& T, int n);
+void gf_tree_free(GfTree& T);
+
+// Builds the tree of pos (n, 3) / mass (n) (device, input order).
+int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float theta,
+                  hipStream_t s);
+// Refit (cpp/nbody.cpp:262-305): same cells, new masses, centres of mass and
+// opening radii enlarged by the largest displacement since the build.
+int gf_tree_refit(GfTree& T, const double* pos, const double* mass, hipStream_t s);

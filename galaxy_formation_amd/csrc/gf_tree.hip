@@ -1,0 +1,522 @@
+// Octree build and refit on the device.
+//
+// The tree is the one cpp/nbody.cpp:202-256 builds: root box from the bounding
+// box (half-width x 1.0001 + 1e-12), octant o = [x > cx] | [y > cy] << 1 |
+// [z > cz] << 2 with child centres c +- h from repeated FP64 halving, a cell
+// is a leaf when it holds <= 8 particles, nodes in DFS preorder with a `next`
+// index, particles in a leaf in ascending input index.
+//
+// Instead of recursive partitioning, every particle gets a 63-bit key whose
+// 21 octant digits come from that same FP64 comparison chain, so the key
+// order is the reference's partition order.  A stable radix sort of
+// (key, index) gives the tree order.  With L_j the number of leading digits
+// keys j-1 and j share, the cell of depth d that starts at position j exists
+// iff L_j < d, and it holds more than 8 particles iff the next 8 keys share
+// its d digits.  Both facts only need a window of 8 positions, so the number
+// of nodes that start at each position, their DFS indices (an exclusive
+// scan: preorder is the order of (start, depth)) and `next` (the DFS index
+// of the first node starting at the cell's end) are all computed in parallel.
+// Centres of mass are summed bottom-up in FP64 by "last child to arrive
+// finishes the parent", children in octant order.
+#include "gf_prim.h"
+#include "gf_tree.h"
+
+namespace {
+
+constexpr int TB = 256;
+
+__device__ __forceinline__ int8_t lcp_levels(uint64_t a, uint64_t b) {
+    const uint64_t x = a ^ b;
+    if (x == 0) return GF_KEY_LEVELS;
+    return int8_t((__clzll(x) - 1) / 3);
+}
+
+__global__ __launch_bounds__(TB) void k_bbox_partial(const double* __restrict__ pos, int n,
+                                                     double* __restrict__ part,
+                                                     uint32_t* __restrict__ idx) {
+    __shared__ double lds[4][6];
+    double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+    for (int i = blockIdx.x * TB + threadIdx.x; i < n; i += gridDim.x * TB) {
+        idx[i] = uint32_t(i);
+        const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        v[0] = fmin(v[0], x); v[1] = fmin(v[1], y); v[2] = fmin(v[2], z);
+        v[3] = fmax(v[3], x); v[4] = fmax(v[4], y); v[5] = fmax(v[5], z);
+    }
+    const int w = threadIdx.x >> 6;
+    for (int k = 0; k < 3; ++k) v[k] = gf_wave_min_d(v[k]);
+    for (int k = 3; k < 6; ++k) v[k] = gf_wave_max_d(v[k]);
+    if (__lane_id() == 0)
+        for (int k = 0; k < 6; ++k) lds[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        double r = lds[0][k];
+        for (int ww = 1; ww < 4; ++ww) r = k < 3 ? fmin(r, lds[ww][k]) : fmax(r, lds[ww][k]);
+        part[6 * blockIdx.x + k] = r;
+    }
+}
+
+// Root box exactly as nbody.cpp:218-219.
+__global__ void k_bbox_final(const double* __restrict__ part, int nb, double* __restrict__ root) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0) return;
+    double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+    for (int b = 0; b < nb; ++b) {
+        for (int k = 0; k < 3; ++k) v[k] = fmin(v[k], part[6 * b + k]);
+        for (int k = 3; k < 6; ++k) v[k] = fmax(v[k], part[6 * b + k]);
+    }
+    root[0] = 0.5 * (v[0] + v[3]);
+    root[1] = 0.5 * (v[1] + v[4]);
+    root[2] = 0.5 * (v[2] + v[5]);
+    const double ext = fmax(fmax(v[3] - v[0], v[4] - v[1]), v[5] - v[2]);
+    root[3] = 0.5 * ext * 1.0001 + 1e-12;
+}
+
+// Key digits from the FP64 comparison chain of nbody.cpp:80 and :92-96.
+__global__ __launch_bounds__(TB) void k_keys(const double* __restrict__ pos, int n,
+                                             const double* __restrict__ root,
+                                             uint64_t* __restrict__ key) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    double cx = root[0], cy = root[1], cz = root[2], h = root[3];
+    uint64_t k = 0;
+#pragma unroll
+    for (int t = 0; t < GF_KEY_LEVELS; ++t) {
+        const int o = int(x > cx) | (int(y > cy) << 1) | (int(z > cz) << 2);
+        k = (k << 3) | uint64_t(o);
+        h = 0.5 * h;
+        cx = cx + ((o & 1) ? h : -h);
+        cy = cy + ((o & 2) ? h : -h);
+        cz = cz + ((o & 4) ? h : -h);
+    }
+    key[i] = k;
+}
+
+__global__ __launch_bounds__(TB) void k_lcp(const uint64_t* __restrict__ key, int n,
+                                            int8_t* __restrict__ lcp) {
+    const int j = blockIdx.x * TB + threadIdx.x;
+    if (j >= n) return;
+    lcp[j] = j == 0 ? int8_t(-1) : lcp_levels(key[j - 1], key[j]);
+}
+
+// Number of nodes that start at sorted position j (a chain of nested cells of
+// depths d0 .. d_last, the last one a leaf), and d0.
+__global__ __launch_bounds__(TB) void k_chain(const int8_t* __restrict__ lcp, int n,
+                                              int* __restrict__ chain, int8_t* __restrict__ d0o) {
+    const int j = blockIdx.x * TB + threadIdx.x;
+    if (j >= n) return;
+    const int L = lcp[j];
+    const int d0 = L + 1;
+    int c = 0;
+    bool node = (j == 0);
+    if (j > 0 && L < GF_KEY_LEVELS) {
+        // the parent cell (depth L) starts at ip, the last position before j
+        // with lcp < L; it is internal iff it holds more than 8 particles
+        int ip = -1;
+        for (int t = 1; t <= GF_LEAF_SIZE; ++t) {
+            const int i = j - t;
+            if (i < 0) break;
+            if (lcp[i] < L) { ip = i; break; }
+        }
+        if (ip < 0) {
+            node = true;                          // parent spans >= 10 positions
+        } else {
+            node = ip + GF_LEAF_SIZE <= n - 1;
+            for (int k = j + 1; node && k <= ip + GF_LEAF_SIZE; ++k) node = lcp[k] >= L;
+        }
+    }
+    if (node) {
+        int m8 = -1;                              // cells (d, j) with d <= m8 hold > 8
+        if (j + GF_LEAF_SIZE <= n - 1) {
+            m8 = GF_KEY_LEVELS;
+            for (int k = j + 1; k <= j + GF_LEAF_SIZE; ++k) m8 = min(m8, int(lcp[k]));
+        }
+        int dl = m8 >= d0 ? m8 + 1 : d0;
+        dl = min(dl, GF_KEY_LEVELS);
+        c = dl - d0 + 1;
+    }
+    chain[j] = c;
+    d0o[j] = int8_t(d0);
+}
+
+__global__ void k_copy_count(const int* __restrict__ base, int n, int* __restrict__ counters) {
+    if (threadIdx.x == 0) {
+        counters[0] = base[n];
+        counters[1] = 0;
+        counters[2] = 0;
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_node_init(const int* __restrict__ counters,
+                                                  int* __restrict__ child8,
+                                                  int* __restrict__ arrive) {
+    const int nn = counters[0];
+    for (int i = blockIdx.x * TB + threadIdx.x; i < nn; i += gridDim.x * TB) {
+        int4* c = reinterpret_cast<int4*>(child8 + 8 * size_t(i));
+        c[0] = make_int4(-1, -1, -1, -1);
+        c[1] = make_int4(-1, -1, -1, -1);
+        arrive[i] = 0;
+    }
+}
+
+// Writes the nodes that start at position j: range, depth, leaf flag, next,
+// parent, geometric box, and registers each in its parent's octant slot.
+__global__ __launch_bounds__(TB) void k_emit(
+    const uint64_t* __restrict__ key, const int* __restrict__ base, const int8_t* __restrict__ d0a,
+    int n, const double* __restrict__ root, int* __restrict__ start, int* __restrict__ count,
+    int* __restrict__ next, int* __restrict__ parent, int8_t* __restrict__ leaf,
+    int8_t* __restrict__ depth, double4* __restrict__ box, int* __restrict__ child8,
+    int* __restrict__ counters) {
+#pragma clang fp contract(off)
+    const int j = blockIdx.x * TB + threadIdx.x;
+    if (j >= n) return;
+    const int b = base[j];
+    const int c = base[j + 1] - b;
+    if (c == 0) return;
+    const int d0 = d0a[j];
+    const uint64_t kj = key[j];
+    double cx = root[0], cy = root[1], cz = root[2], h = root[3];
+    for (int t = 1; t < d0; ++t) {
+        const int o = gf_key_digit(kj, t);
+        h = 0.5 * h;
+        cx = cx + ((o & 1) ? h : -h);
+        cy = cy + ((o & 2) ? h : -h);
+        cz = cz + ((o & 4) ? h : -h);
+    }
+    int hi_lim = n;                               // cells of one chain are nested
+    for (int t = 0; t < c; ++t) {
+        const int d = d0 + t;
+        if (d >= 1) {
+            const int o = gf_key_digit(kj, d);
+            h = 0.5 * h;
+            cx = cx + ((o & 1) ? h : -h);
+            cy = cy + ((o & 2) ? h : -h);
+            cz = cz + ((o & 4) ? h : -h);
+        }
+        const int id = b + t;
+        const uint64_t pj = gf_key_prefix(kj, d);
+        int lo = j + 1, hi = hi_lim;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (gf_key_prefix(key[mid], d) == pj) lo = mid + 1; else hi = mid;
+        }
+        const int end = lo;
+        hi_lim = end;
+        const bool is_leaf = (t == c - 1);
+        start[id] = j;
+        count[id] = end - j;
+        next[id] = base[end];
+        depth[id] = int8_t(d);
+        leaf[id] = int8_t(is_leaf);
+        box[id] = make_double4(cx, cy, cz, h);
+        int p;
+        if (t > 0) {
+            p = id - 1;
+        } else if (d == 0) {
+            p = -1;
+        } else {
+            // parent cell: depth d - 1, starts at the first key sharing d - 1 digits
+            const uint64_t pp = gf_key_prefix(kj, d - 1);
+            int l2 = 0, h2 = j;
+            while (l2 < h2) {
+                const int mid = (l2 + h2) >> 1;
+                if (gf_key_prefix(key[mid], d - 1) < pp) l2 = mid + 1; else h2 = mid;
+            }
+            p = base[l2] + (d - 1 - d0a[l2]);
+        }
+        parent[id] = p;
+        if (p >= 0) child8[8 * p + gf_key_digit(kj, d)] = id;
+        if (is_leaf && end - j > GF_LEAF_SIZE && d < GF_MAX_DEPTH) atomicAdd(&counters[2], 1);
+    }
+}
+
+// Inside a leaf the reference keeps ascending input index (its partition is
+// a stable counting sort that stops at the leaf, nbody.cpp:76-90, :122), while
+// the key sort ordered the leaf by deeper digits: sort each leaf by index.
+__global__ __launch_bounds__(TB) void k_leaf_sort(const int* __restrict__ counters,
+                                                  const int* __restrict__ start,
+                                                  const int* __restrict__ count,
+                                                  const int8_t* __restrict__ leaf,
+                                                  uint32_t* __restrict__ idx,
+                                                  uint64_t* __restrict__ key) {
+    const int nn = counters[0];
+    for (int id = blockIdx.x * TB + threadIdx.x; id < nn; id += gridDim.x * TB) {
+        if (!leaf[id]) continue;
+        const int b = start[id], e = b + count[id];
+        for (int i = b + 1; i < e; ++i) {
+            const uint32_t v = idx[i];
+            const uint64_t k = key[i];
+            int p = i;
+            while (p > b && idx[p - 1] > v) {
+                idx[p] = idx[p - 1];
+                key[p] = key[p - 1];
+                --p;
+            }
+            idx[p] = v;
+            key[p] = k;
+        }
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_gather(const double* __restrict__ pos,
+                                               const double* __restrict__ mass,
+                                               const uint32_t* __restrict__ idx, int n,
+                                               const double* __restrict__ xb, double* __restrict__ tx,
+                                               double* __restrict__ ty, double* __restrict__ tz,
+                                               double* __restrict__ tm,
+                                               double* __restrict__ tdisp) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    const int j = int(idx[i]);
+    const double x = pos[3 * j], y = pos[3 * j + 1], z = pos[3 * j + 2], m = mass[j];
+    tx[i] = x; ty[i] = y; tz[i] = z; tm[i] = m;
+    double d = 0.0;
+    if (xb) {
+        const double dx = x - xb[3 * j], dy = y - xb[3 * j + 1], dz = z - xb[3 * j + 2];
+        d = m > 0 ? sqrt(dx * dx + dy * dy + dz * dz) : 0.0;
+    }
+    tdisp[i] = d;
+}
+
+// nbody.cpp:98-115 (build) and :297-303 (refit) in FP64: open_r =
+// 2 (half + dm) / theta + |com - centre| (dm = 0 right after a build).
+__device__ __forceinline__ void finalize_node(int id, double M, double sx, double sy, double sz,
+                                              double dm, double theta_d,
+                                              const double4* __restrict__ box, double4* com,
+                                              double* open_r, double* ndisp) {
+#pragma clang fp contract(off)
+    const double inv = M > 0 ? 1.0 / M : 0.0;
+    const double cx = sx * inv, cy = sy * inv, cz = sz * inv;
+    const double4 bx = box[id];
+    const double dx = cx - bx.x, dy = cy - bx.y, dz = cz - bx.z;
+    com[id] = make_double4(cx, cy, cz, M);
+    open_r[id] = 2 * (bx.w + dm) / theta_d + sqrt(dx * dx + dy * dy + dz * dz);
+    ndisp[id] = dm;
+}
+
+__global__ __launch_bounds__(TB) void k_com_climb(
+    const int* __restrict__ counters, const int* __restrict__ start, const int* __restrict__ count,
+    const int* __restrict__ parent, const int8_t* __restrict__ leaf,
+    const int* __restrict__ child8, int* __restrict__ arrive, const double4* __restrict__ box,
+    const double* __restrict__ tx, const double* __restrict__ ty, const double* __restrict__ tz,
+    const double* __restrict__ tm, const double* __restrict__ tdisp, double4* com,
+    double* open_r, double* ndisp, double theta_d) {
+#pragma clang fp contract(off)
+    const int nn = counters[0];
+    for (int id = blockIdx.x * TB + threadIdx.x; id < nn; id += gridDim.x * TB) {
+        if (!leaf[id]) continue;
+        double M = 0, sx = 0, sy = 0, sz = 0, dm = 0;
+        const int b = start[id], e = b + count[id];
+        for (int i = b; i < e; ++i) {
+            const double m = tm[i];
+            M += m;
+            sx += m * tx[i];
+            sy += m * ty[i];
+            sz += m * tz[i];
+            dm = fmax(dm, tdisp[i]);
+        }
+        finalize_node(id, M, sx, sy, sz, dm, theta_d, box, com, open_r, ndisp);
+        int cur = id;
+        while (true) {
+            const int p = parent[cur];
+            if (p < 0) break;
+            const int* slots = child8 + 8 * size_t(p);
+            int nch = 0;
+            for (int o = 0; o < 8; ++o) nch += slots[o] >= 0;
+            __threadfence();
+            const int old = atomicAdd(&arrive[p], 1);
+            if (old != nch - 1) break;
+            __threadfence();
+            const volatile double* vc = reinterpret_cast<const volatile double*>(com);
+            const volatile double* vd = ndisp;
+            M = 0; sx = 0; sy = 0; sz = 0; dm = 0;
+            for (int o = 0; o < 8; ++o) {
+                const int c = slots[o];
+                if (c < 0) continue;
+                const double cm = vc[4 * size_t(c) + 3];
+                M += cm;
+                sx += cm * vc[4 * size_t(c)];
+                sy += cm * vc[4 * size_t(c) + 1];
+                sz += cm * vc[4 * size_t(c) + 2];
+                dm = fmax(dm, vd[c]);
+            }
+            finalize_node(p, M, sx, sy, sz, dm, theta_d, box, com, open_r, ndisp);
+            cur = p;
+        }
+    }
+}
+
+// Groups (nbody.cpp:234-243): the outermost nodes holding <= 64 particles, or
+// leaves, i.e. those whose parent holds more than 64.
+// flag has `len` entries (>= node count); entries past the node count are 0.
+__global__ __launch_bounds__(TB) void k_group_flag(const int* __restrict__ counters,
+                                                   const int* __restrict__ count,
+                                                   const int* __restrict__ parent,
+                                                   const int8_t* __restrict__ leaf, int len,
+                                                   int* __restrict__ flag) {
+    const int nn = counters[0];
+    for (int id = blockIdx.x * TB + threadIdx.x; id < len; id += gridDim.x * TB) {
+        bool g = false;
+        if (id < nn) {
+            const int p = parent[id];
+            g = (count[id] <= GF_GROUP_SIZE || leaf[id]) && (p < 0 || count[p] > GF_GROUP_SIZE);
+        }
+        flag[id] = g;
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_group_compact(int* __restrict__ counters,
+                                                      const int* __restrict__ count,
+                                                      const int* __restrict__ parent,
+                                                      const int8_t* __restrict__ leaf,
+                                                      const int* __restrict__ off,
+                                                      int* __restrict__ groups) {
+    const int nn = counters[0];
+    for (int id = blockIdx.x * TB + threadIdx.x; id < nn; id += gridDim.x * TB) {
+        if (off[id + 1] != off[id]) groups[off[id]] = id;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[1] = off[nn];
+}
+
+template <typename T>
+int dalloc(T*& p, size_t count) {
+    if (p) GF_HIP(hipFree(p));
+    p = nullptr;
+    GF_HIP(hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (count ? count : 1)));
+    return 0;
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int node_grid(int cap_nodes) { return max(1, min(2048, gf_div_up(cap_nodes, TB))); }
+
+}  // namespace
+
+int gf_tree_node_bound(int n) {
+    return int(std::min<long long>(0x7fffffffLL,
+                                   (long long)n + (long long)(GF_KEY_LEVELS + 1) * (n / (GF_LEAF_SIZE + 1)) + 2));
+}
+
+int gf_tree_reserve(GfTree& T, int n) {
+    if (!T.counters) {
+        if (dalloc(T.counters, 4) || dalloc(T.root, 4) || dalloc(T.bbox_part, 6 * 1024)) return -1;
+    }
+    if (n > T.cap_n) {
+        const int cn = std::max(n, 1024);
+        if (dalloc(T.key, cn) || dalloc(T.key_alt, cn) || dalloc(T.idx, cn) || dalloc(T.idx_alt, cn) ||
+            dalloc(T.lcp, cn) || dalloc(T.d0, cn) || dalloc(T.chain, size_t(cn) + 1) ||
+            dalloc(T.tx, cn) || dalloc(T.ty, cn) || dalloc(T.tz, cn) || dalloc(T.tm, cn) ||
+            dalloc(T.tdisp, cn) || dalloc(T.xb, 3 * size_t(cn)) ||
+            dalloc(T.scan_tmp, gf_scan_temp_ints(std::max(cn, gf_tree_node_bound(cn)) + 1)) ||
+            dalloc(T.sort_tmp, gf_sort_temp_ints(cn)))
+            return -1;
+        T.cap_n = cn;
+        T.built = 0;
+    }
+    const int nb = gf_tree_node_bound(T.cap_n);
+    if (nb > T.cap_nodes) {
+        if (dalloc(T.start, nb) || dalloc(T.count, nb) || dalloc(T.next, nb) ||
+            dalloc(T.parent, nb) || dalloc(T.leaf, nb) || dalloc(T.depth, nb) ||
+            dalloc(T.child8, 8 * size_t(nb)) || dalloc(T.arrive, nb) || dalloc(T.com, nb) ||
+            dalloc(T.open_r, nb) || dalloc(T.box, nb) || dalloc(T.ndisp, nb) ||
+            dalloc(T.gflag, size_t(nb) + 1) || dalloc(T.groups, nb))
+            return -1;
+        T.cap_nodes = nb;
+        T.built = 0;
+    }
+    return 0;
+}
+
+void gf_tree_free(GfTree& T) {
+    dfree(T.key); dfree(T.key_alt); dfree(T.idx); dfree(T.idx_alt); dfree(T.lcp); dfree(T.d0);
+    dfree(T.chain); dfree(T.tx); dfree(T.ty); dfree(T.tz); dfree(T.tm); dfree(T.tdisp);
+    dfree(T.xb); dfree(T.start); dfree(T.count); dfree(T.next); dfree(T.parent); dfree(T.leaf);
+    dfree(T.depth); dfree(T.child8); dfree(T.arrive); dfree(T.com); dfree(T.open_r);
+    dfree(T.box); dfree(T.ndisp); dfree(T.gflag); dfree(T.groups); dfree(T.counters);
+    dfree(T.root); dfree(T.bbox_part); dfree(T.scan_tmp); dfree(T.sort_tmp);
+    T = GfTree();
+}
+
+__global__ __launch_bounds__(TB) void k_zero_arrive(const int* __restrict__ counters,
+                                                    int* __restrict__ arrive) {
+    const int nn = counters[0];
+    for (int i = blockIdx.x * TB + threadIdx.x; i < nn; i += gridDim.x * TB) arrive[i] = 0;
+}
+
+int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float theta,
+                  hipStream_t s) {
+    if (gf_tree_reserve(T, n)) return -1;
+    T.n = n;
+    T.theta = theta;
+    T.built = 0;
+    if (n == 0) return 0;
+    const int g = gf_div_up(n, TB);
+    const int nbb = std::min(1024, g);
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(TB), 0, s, pos, n, T.bbox_part, T.idx);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, T.bbox_part, nbb, T.root);
+    hipLaunchKernelGGL(k_keys, dim3(g), dim3(TB), 0, s, pos, n, T.root, T.key);
+    GF_LAUNCH_CHECK();
+    int in_alt = 0;
+    if (gf_radix_sort_pairs(T.key, T.idx, T.key_alt, T.idx_alt, n, 0, 3 * GF_KEY_LEVELS,
+                            T.sort_tmp, s, &in_alt))
+        return -1;
+    if (in_alt) {
+        std::swap(T.key, T.key_alt);
+        std::swap(T.idx, T.idx_alt);
+    }
+    hipLaunchKernelGGL(k_lcp, dim3(g), dim3(TB), 0, s, T.key, n, T.lcp);
+    hipLaunchKernelGGL(k_chain, dim3(g), dim3(TB), 0, s, T.lcp, n, T.chain, T.d0);
+    GF_LAUNCH_CHECK();
+    if (gf_scan_excl(T.chain, T.chain, n, T.scan_tmp, s)) return -1;
+    const int ng = node_grid(T.cap_nodes);
+    hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, s, T.chain, n, T.counters);
+    hipLaunchKernelGGL(k_node_init, dim3(ng), dim3(TB), 0, s, T.counters, T.child8, T.arrive);
+    hipLaunchKernelGGL(k_emit, dim3(g), dim3(TB), 0, s, T.key, T.chain, T.d0, n, T.root, T.start,
+                       T.count, T.next, T.parent, T.leaf, T.depth, T.box, T.child8, T.counters);
+    hipLaunchKernelGGL(k_leaf_sort, dim3(ng), dim3(TB), 0, s, T.counters, T.start, T.count,
+                       T.leaf, T.idx, T.key);
+    GF_HIP(hipMemcpyAsync(T.xb, pos, sizeof(double) * 3 * size_t(n), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_gather, dim3(g), dim3(TB), 0, s, pos, mass, T.idx, n, nullptr, T.tx,
+                       T.ty, T.tz, T.tm, T.tdisp);
+    hipLaunchKernelGGL(k_com_climb, dim3(ng), dim3(TB), 0, s, T.counters, T.start, T.count,
+                       T.parent, T.leaf, T.child8, T.arrive, T.box, T.tx, T.ty, T.tz, T.tm,
+                       T.tdisp, T.com, T.open_r, T.ndisp, double(theta));
+    // group offsets: the node count lives on the device, so the scan runs over
+    // the node bound of this n, with zero flags past the node count
+    const int len = gf_tree_node_bound(n);
+    hipLaunchKernelGGL(k_group_flag, dim3(ng), dim3(TB), 0, s, T.counters, T.count, T.parent,
+                       T.leaf, len, T.gflag);
+    GF_LAUNCH_CHECK();
+    if (gf_scan_excl(T.gflag, T.gflag, len, T.scan_tmp, s)) return -1;
+    hipLaunchKernelGGL(k_group_compact, dim3(ng), dim3(TB), 0, s, T.counters, T.count, T.parent,
+                       T.leaf, T.gflag, T.groups);
+    GF_LAUNCH_CHECK();
+    T.built = 1;
+    return 0;
+}
+
+int gf_tree_refit(GfTree& T, const double* pos, const double* mass, hipStream_t s) {
+    const int n = T.n;
+    if (!T.built) {
+        gf_set_error("refit without a built tree");
+        return -1;
+    }
+    if (n == 0) return 0;
+    const int g = gf_div_up(n, TB);
+    const int ng = node_grid(T.cap_nodes);
+    hipLaunchKernelGGL(k_gather, dim3(g), dim3(TB), 0, s, pos, mass, T.idx, n, T.xb, T.tx, T.ty,
+                       T.tz, T.tm, T.tdisp);
+    hipLaunchKernelGGL(k_zero_arrive, dim3(ng), dim3(TB), 0, s, T.counters, T.arrive);
+    hipLaunchKernelGGL(k_com_climb, dim3(ng), dim3(TB), 0, s, T.counters, T.start, T.count,
+                       T.parent, T.leaf, T.child8, T.arrive, T.box, T.tx, T.ty, T.tz, T.tm,
+                       T.tdisp, T.com, T.open_r, T.ndisp, double(T.theta));
+    GF_LAUNCH_CHECK();
+    return 0;
+}

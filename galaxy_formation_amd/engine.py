@@ -1,0 +1,108 @@
+"""Typed interface to the gravity / rendering engine, with the reference's
+API (galaxy_formation/engine.py:39-72): Engine.gravity, gravity_direct,
+project and tonemap take and return the same numpy arrays.  The work runs in
+libgf.so on the GPU; nothing here computes physics on the host.
+"""
+
+import numpy as np
+
+from . import _lib
+
+PRECISIONS = {"fp32": 0, "fp64": 1}
+
+
+class Engine:
+    """Thin, typed interface to libgf.so (engine.py:39-72 of the reference).
+
+    precision: "fp32" (FP32 pair arithmetic on group-relative coordinates,
+    compensated accumulation; the default) or "fp64".  Tree, centres of mass
+    and opening tests are FP64 in both."""
+
+    def __init__(self, precision="fp32"):
+        self.lib = _lib.load()
+        self.precision = precision
+
+    @property
+    def precision(self):
+        return self._precision
+
+    @precision.setter
+    def precision(self, p):
+        if p not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        self._precision = p
+
+    def gravity(self, pos, mass, acc, theta, eps, active=None, rebuild=True):
+        """Barnes-Hut accelerations (G = 1) written into acc for the active
+        particles.  rebuild=False refits the previous tree instead of building
+        a new one (engine.py:45-50)."""
+        n = len(mass)
+        act = np.ones(n, np.uint8) if active is None else np.ascontiguousarray(active, dtype=np.uint8)
+        _lib.check(self.lib.gf_set_precision(PRECISIONS[self._precision]))
+        _lib.check(self.lib.nbody_accel_active(n, pos, mass, acc, theta, eps, _lib.ptr(act),
+                                               int(rebuild)))
+
+    def gravity_direct(self, pos, mass, eps):
+        """Exact O(N^2) accelerations (engine.py:52-56)."""
+        acc = np.zeros_like(pos)
+        _lib.check(self.lib.nbody_accel_direct(len(mass), pos, mass, acc, eps))
+        return acc
+
+    def gravity_direct_subset(self, pos, mass, eps, targets):
+        """Exact FP64 accelerations of the particles `targets` only."""
+        tg = np.ascontiguousarray(targets, np.int64)
+        acc = np.zeros((len(tg), 3))
+        _lib.check(self.lib.gf_accel_direct_subset(len(mass), pos, mass, len(tg), _lib.ptr(tg),
+                                                   acc, eps))
+        return acc
+
+    def last_interactions(self):
+        """Interactions of the last gravity call (sum of list lengths over the
+        evaluated particles)."""
+        return int(self.lib.gf_last_interactions())
+
+    def project(self, pos, group, n_groups, rot, scale, size):
+        """Particle counts per pixel and per colour group, shape
+        (n_groups, size, size) (engine.py:58-63)."""
+        img = np.empty((n_groups, size, size), np.float32)
+        _lib.check(self.lib.nbody_render(len(group), np.ascontiguousarray(pos), group, n_groups,
+                                         np.ascontiguousarray(rot), np.zeros(3), scale, size, size,
+                                         img))
+        return img
+
+    def tonemap(self, img, glow, colors, inv_ref, K, exposure):
+        """8-bit RGB image from the per-group count maps and their (4x
+        smaller) glow maps (engine.py:65-72)."""
+        ng, H, W = img.shape
+        out = np.empty((H, W, 3), np.uint8)
+        _lib.check(self.lib.nbody_tonemap(W, H, ng, img, np.ascontiguousarray(glow), glow.shape[2],
+                                          glow.shape[1],
+                                          np.ascontiguousarray(colors, dtype=np.float32).ravel(),
+                                          inv_ref, K, exposure, out))
+        return out
+
+    # -- parity helpers ----------------------------------------------------
+
+    def tree_info(self):
+        import ctypes
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(self.lib.gf_tree_info(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def tree_export(self):
+        """The tree of the last gravity call, DFS order (cpp/nbody.cpp layout)."""
+        nn, ng, _ = self.tree_info()
+        out = dict(start=np.empty(nn, np.int32), count=np.empty(nn, np.int32),
+                   next=np.empty(nn, np.int32), leaf=np.empty(nn, np.int32),
+                   depth=np.empty(nn, np.int32), com4=np.empty((nn, 4)), open_r=np.empty(nn),
+                   box4=np.empty((nn, 4)), groups=np.empty(ng, np.int32))
+        _lib.check(self.lib.gf_tree_export(nn, *[_lib.ptr(out[k]) for k in
+                                                 ("start", "count", "next", "leaf", "depth",
+                                                  "com4", "open_r", "box4")], None,
+                                           _lib.ptr(out["groups"])))
+        n = int(out["count"][0]) if nn else 0
+        idx = np.empty(n, np.int32)
+        _lib.check(self.lib.gf_tree_export(nn, None, None, None, None, None, None, None, None,
+                                           _lib.ptr(idx), None))
+        out["idx"] = idx
+        return out

@@ -1,0 +1,102 @@
+/*
+ * libgf.so: C ABI of the MI355X (gfx950) force path.
+ *
+ * Conventions: every function returns 0 (or a documented non-negative value)
+ * on success and a negative value on failure; gf_last_error() then holds the
+ * message.  Arrays are C-contiguous: positions, velocities and accelerations
+ * are (n, 3) row-major float64; masses, smoothing lengths, energies (n,)
+ * float64; neighbour tables (n, k) int64 of local indices.  Host-pointer
+ * entry points copy their inputs to the GPU, run on the library's stream and
+ * return when the outputs are back in host memory.  *_dev entry points take
+ * device pointers and a hipStream_t (NULL = the library's stream) and return
+ * without synchronising.  The library keeps state between calls (the last
+ * gravity tree, staging buffers), like the reference's global tree
+ * (cpp/nbody.cpp:307), and is not re-entrant.
+ *
+ * File:line citations are in the reference HugoGW/Galaxy_formation.
+ */
+#ifndef GF_H
+#define GF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------ the reference's engine ABI
+ * Same names and arguments as cpp/nbody.cpp:415-505, bound with the argtypes
+ * of galaxy_formation/engine.py:26-35.  The reference returns void; these
+ * return int status, which ctypes' default restype (c_int) reads unchanged. */
+
+/* Replaces nbody_num_threads (cpp/nbody.cpp:417): the GPU's compute units. */
+int nbody_num_threads(void);
+
+/* Replaces nbody_accel_active (cpp/nbody.cpp:422-425): Barnes-Hut
+ * accelerations (G = 1, Plummer eps) of the active particles (active[i] != 0;
+ * NULL = all), written to acc; other rows are left untouched.  rebuild = 0
+ * refits the tree of the previous call (a change of n forces a build).  The
+ * interaction precision is the one gf_set_precision selected. */
+int nbody_accel_active(int n, const double* pos, const double* mass, double* acc, double theta,
+                       double eps, const uint8_t* active, int rebuild);
+
+/* Replaces nbody_accel_direct (cpp/nbody.cpp:428-441): FP64 O(n^2) sum. */
+int nbody_accel_direct(int n, const double* pos, const double* mass, double* acc, double eps);
+
+/* Replaces nbody_render (cpp/nbody.cpp:445-469): orthographic projection and
+ * bilinear deposit into img (ngroups, H, W) float32. */
+int nbody_render(int n, const double* pos, const uint8_t* group, int ngroups, const double* rot,
+                 const double* center, double scale, int W, int H, float* img);
+
+/* Replaces nbody_tonemap (cpp/nbody.cpp:474-503): RGB8 image (H, W, 3). */
+int nbody_tonemap(int W, int H, int ngroups, const float* img, const float* glow, int gw, int gh,
+                  const float* colors, float inv_ref, float K, float exposure, uint8_t* out);
+
+/* --------------------------------------------------------------- library */
+
+/* Message of the last failure. */
+const char* gf_last_error(void);
+/* ABI version of this header. */
+int gf_version(void);
+/* Device name, architecture and compute-unit count as text. */
+int gf_device_info(char* buf, int len);
+/* Interaction precision of nbody_accel_active: 0 = FP32 pair arithmetic on
+ * group-relative coordinates with compensated accumulation (default), 1 =
+ * FP64.  Tree, centres of mass and acceptance tests are FP64 in both. */
+int gf_set_precision(int precision);
+int gf_get_precision(void);
+/* Interactions (sum over evaluated particles of their list length) of the
+ * last nbody_accel_active call (the count nbody.cpp:397 loops over). */
+long long gf_last_interactions(void);
+
+/* FP64 direct sum (cpp/nbody.cpp:428-441) for the nt targets tgt (indices
+ * into pos; NULL = all n with nt == n); acc is (nt, 3). */
+int gf_accel_direct_subset(int n, const double* pos, const double* mass, int nt,
+                           const int64_t* tgt, double* acc, double eps);
+
+/* Node, group and key-depth-cut cell counts of the held tree. */
+int gf_tree_info(int* nodes, int* groups, int* deep_cells);
+/* The held tree in the layout of cpp/nbody.cpp:38-45 (DFS order): returns the
+ * node count (cap = room in the node arrays); any pointer may be NULL.
+ * com4 = (cx, cy, cz, m) FP64, box4 = (gx, gy, gz, half) FP64, idx = input
+ * index of each tree position (n), groups = group node indices. */
+int gf_tree_export(int cap, int* start, int* count, int* next, int* leaf, int* depth,
+                   double* com4, double* open_r, double* box4, int* idx, int* groups);
+
+/* Device-pointer gravity (same semantics as nbody_accel_active), with the
+ * precision per call. */
+int gf_gravity_dev(int n, const double* pos, const double* mass, double* acc, double theta,
+                   double eps, const uint8_t* active, int rebuild, int precision, void* stream);
+/* Device-pointer FP64 direct sum for nt targets. */
+int gf_direct_dev(int n, const double* pos, const double* mass, int nt, const int64_t* tgt,
+                  double* acc, double eps, void* stream);
+/* Interaction count of the last gravity call (synchronises). */
+int gf_interactions_dev(long long* out);
+/* Waits for the stream and reports kernel-side errors. */
+int gf_sync(void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GF_H */

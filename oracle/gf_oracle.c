@@ -1,0 +1,664 @@
+/*
+ * CPU oracle for the force hot path.  TEST INFRASTRUCTURE ONLY: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this file's
+ * library, and only as the checker or the timed CPU baseline.  The product
+ * (galaxy_formation_amd) never calls into it.
+ *
+ * Restates, in plain C with -ffp-contract=off (every operation is the one
+ * written here), the algorithms of:
+ *   - cpp/nbody.cpp:60-256   octree build (Tree::build, partition, child_center,
+ *                            finalize, build_rec, groups, tree-order copy)
+ *   - cpp/nbody.cpp:262-305  refit
+ *   - cpp/nbody.cpp:323-411  group walk and list evaluation (compute_accel)
+ *   - cpp/nbody.cpp:428-441  FP64 O(N^2) direct sum
+ *   - galaxy_formation/sph.py:31-54   kernel, kernel_grad
+ *   - galaxy_formation/sph.py:57-106  sph_forces
+ *   - galaxy_formation/sph.py:109-135 knn_from_candidates
+ *   - galaxy_formation/sph.py:165-166 exact kNN (cKDTree query), restated as
+ *                                     a brute-force scan ordered by (d^2, index)
+ *   - galaxy_formation/simulation.py:84 h = max(d_k / 2, eps / 2)
+ *
+ * Two precision modes share one tree:
+ *   mode 0 ("ref"):  the reference arithmetic: FP32 node payload, FP32 MAC and
+ *                    FP32 list evaluation in absolute coordinates.
+ *   mode 1 ("fp64"): FP64 centres of mass, FP64 MAC and FP64 evaluation.  This
+ *                    is the interaction set the GPU walk uses for both of its
+ *                    evaluation precisions.
+ *
+ * Pinning (tests/): mode 0 against the reference library built from source
+ * (oracle/_ref) and its golden fixtures; both modes against the FP64 direct sum
+ * (theta -> 0 gives the direct sum exactly), closed forms and invariants.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define LEAF_SIZE 8
+#define GROUP_SIZE 64
+#define MAX_DEPTH 48
+
+typedef struct {
+    /* reference payload (nbody.cpp:38-45) */
+    float cx, cy, cz, m, open_r;
+    int start, count, next, leaf, depth;
+    float gx, gy, gz, half;
+    /* FP64 geometric box and FP64 mode payload */
+    double bcx, bcy, bcz, bhalf;
+    double dcx, dcy, dcz, dm, dopen_r;
+} ONode;
+
+typedef struct {
+    int n, nn, cap, ng;
+    ONode *nodes;
+    int *idx, *tmp, *groups;
+    unsigned char *oct;
+    float *px, *py, *pz, *pm;
+    double *dpx, *dpy, *dpz, *dpm;
+    double *xb;
+    float *disp, *node_disp;
+    double *ddisp, *dnode_disp;
+    float theta;
+    const double *pos, *mass;
+} OTree;
+
+static OTree T;
+
+static void *xrealloc(void *p, size_t s) {
+    void *q = realloc(p, s ? s : 1);
+    if (!q) abort();
+    return q;
+}
+
+static int new_node(void) {
+    if (T.nn == T.cap) {
+        T.cap = T.cap ? 2 * T.cap : 1024;
+        T.nodes = (ONode *)xrealloc(T.nodes, sizeof(ONode) * (size_t)T.cap);
+    }
+    memset(&T.nodes[T.nn], 0, sizeof(ONode));
+    return T.nn++;
+}
+
+/* nbody.cpp:76-90: stable counting sort of idx[b, e) into octants around c. */
+static void partition(int b, int e, const double c[3], int off[9]) {
+    int cnt[8] = {0};
+    const double *pos = T.pos;
+    for (int i = b; i < e; ++i) {
+        int j = T.idx[i];
+        unsigned char o = (unsigned char)((pos[3 * j] > c[0]) | ((pos[3 * j + 1] > c[1]) << 1) |
+                                          ((pos[3 * j + 2] > c[2]) << 2));
+        T.oct[i] = o;
+        ++cnt[o];
+    }
+    off[0] = b;
+    for (int o = 0; o < 8; ++o) off[o + 1] = off[o] + cnt[o];
+    int fill[8];
+    for (int o = 0; o < 8; ++o) fill[o] = off[o];
+    for (int i = b; i < e; ++i) T.tmp[fill[T.oct[i]]++] = T.idx[i];
+    memcpy(T.idx + b, T.tmp + b, sizeof(int) * (size_t)(e - b));
+}
+
+/* nbody.cpp:98-115 (finalize) plus the FP64-mode fields. */
+static void finalize(int id, double M, double sx, double sy, double sz, double dM, double dsx,
+                     double dsy, double dsz, const double c[3], double half, int b, int e,
+                     int leaf, int depth, int next) {
+    ONode *nd = &T.nodes[id];
+    double inv = M > 0 ? 1.0 / M : 0.0;
+    nd->cx = (float)(sx * inv);
+    nd->cy = (float)(sy * inv);
+    nd->cz = (float)(sz * inv);
+    nd->m = (float)M;
+    double dx = sx * inv - c[0], dy = sy * inv - c[1], dz = sz * inv - c[2];
+    nd->open_r = (float)(2 * half / (double)T.theta + sqrt(dx * dx + dy * dy + dz * dz));
+    nd->gx = (float)c[0];
+    nd->gy = (float)c[1];
+    nd->gz = (float)c[2];
+    nd->half = (float)half;
+    nd->bcx = c[0];
+    nd->bcy = c[1];
+    nd->bcz = c[2];
+    nd->bhalf = half;
+    nd->start = b;
+    nd->count = e - b;
+    nd->leaf = leaf;
+    nd->depth = depth;
+    nd->next = next;
+    double dinv = dM > 0 ? 1.0 / dM : 0.0;
+    nd->dcx = dsx * dinv;
+    nd->dcy = dsy * dinv;
+    nd->dcz = dsz * dinv;
+    nd->dm = dM;
+    double ex = nd->dcx - c[0], ey = nd->dcy - c[1], ez = nd->dcz - c[2];
+    nd->dopen_r = 2 * half / (double)T.theta + sqrt(ex * ex + ey * ey + ez * ez);
+}
+
+/* nbody.cpp:118-150 (build_rec); the top-level task split of :153-200 does not
+ * change the tree, so the whole tree is built by this one recursion. */
+static int build_rec(int b, int e, const double c[3], double half, int depth) {
+    int id = new_node();
+    double M = 0, sx = 0, sy = 0, sz = 0;
+    double dM = 0, dsx = 0, dsy = 0, dsz = 0;
+    int leaf = (e - b <= LEAF_SIZE) || depth >= MAX_DEPTH;
+    if (leaf) {
+        for (int i = b; i < e; ++i) {
+            int j = T.idx[i];
+            double m = T.mass[j];
+            M += m;
+            sx += m * T.pos[3 * j];
+            sy += m * T.pos[3 * j + 1];
+            sz += m * T.pos[3 * j + 2];
+        }
+        dM = M; dsx = sx; dsy = sy; dsz = sz;
+    } else {
+        int off[9];
+        partition(b, e, c, off);
+        double h = 0.5 * half;
+        for (int o = 0; o < 8; ++o) {
+            if (off[o] == off[o + 1]) continue;
+            double cc[3];
+            cc[0] = c[0] + ((o & 1) ? h : -h);
+            cc[1] = c[1] + ((o & 2) ? h : -h);
+            cc[2] = c[2] + ((o & 4) ? h : -h);
+            int ch = build_rec(off[o], off[o + 1], cc, h, depth + 1);
+            const ONode *cn = &T.nodes[ch];
+            M += cn->m;
+            sx += (double)cn->m * cn->cx;
+            sy += (double)cn->m * cn->cy;
+            sz += (double)cn->m * cn->cz;
+            dM += cn->dm;
+            dsx += cn->dm * cn->dcx;
+            dsy += cn->dm * cn->dcy;
+            dsz += cn->dm * cn->dcz;
+        }
+    }
+    finalize(id, M, sx, sy, sz, dM, dsx, dsy, dsz, c, half, b, e, leaf, depth, T.nn);
+    return id;
+}
+
+/* nbody.cpp:202-256 (Tree::build). */
+static void tree_build(int n, const double *pos, const double *mass, float theta) {
+    T.pos = pos;
+    T.mass = mass;
+    T.theta = theta;
+    T.n = n;
+    T.idx = (int *)xrealloc(T.idx, sizeof(int) * (size_t)n);
+    T.tmp = (int *)xrealloc(T.tmp, sizeof(int) * (size_t)n);
+    T.oct = (unsigned char *)xrealloc(T.oct, (size_t)n);
+    double x0 = 1e300, y0 = 1e300, z0 = 1e300, x1 = -1e300, y1 = -1e300, z1 = -1e300;
+    for (int i = 0; i < n; ++i) {
+        T.idx[i] = i;
+        x0 = fmin(x0, pos[3 * i]); x1 = fmax(x1, pos[3 * i]);
+        y0 = fmin(y0, pos[3 * i + 1]); y1 = fmax(y1, pos[3 * i + 1]);
+        z0 = fmin(z0, pos[3 * i + 2]); z1 = fmax(z1, pos[3 * i + 2]);
+    }
+    double c[3] = {0.5 * (x0 + x1), 0.5 * (y0 + y1), 0.5 * (z0 + z1)};
+    double ext = fmax(fmax(x1 - x0, y1 - y0), z1 - z0);
+    double half = 0.5 * ext * 1.0001 + 1e-12;
+    T.nn = 0;
+    build_rec(0, n, c, half, 0);
+
+    T.groups = (int *)xrealloc(T.groups, sizeof(int) * (size_t)(T.nn + 1));
+    T.ng = 0;
+    for (int i = 0; i < T.nn;) {
+        if (T.nodes[i].count <= GROUP_SIZE || T.nodes[i].leaf) {
+            T.groups[T.ng++] = i;
+            i = T.nodes[i].next;
+        } else {
+            ++i;
+        }
+    }
+    size_t fs = sizeof(float) * (size_t)n, ds = sizeof(double) * (size_t)n;
+    T.px = (float *)xrealloc(T.px, fs); T.py = (float *)xrealloc(T.py, fs);
+    T.pz = (float *)xrealloc(T.pz, fs); T.pm = (float *)xrealloc(T.pm, fs);
+    T.dpx = (double *)xrealloc(T.dpx, ds); T.dpy = (double *)xrealloc(T.dpy, ds);
+    T.dpz = (double *)xrealloc(T.dpz, ds); T.dpm = (double *)xrealloc(T.dpm, ds);
+    T.xb = (double *)xrealloc(T.xb, 3 * ds);
+    memcpy(T.xb, pos, 3 * ds);
+    for (int i = 0; i < n; ++i) {
+        int j = T.idx[i];
+        T.px[i] = (float)pos[3 * j]; T.py[i] = (float)pos[3 * j + 1];
+        T.pz[i] = (float)pos[3 * j + 2]; T.pm[i] = (float)mass[j];
+        T.dpx[i] = pos[3 * j]; T.dpy[i] = pos[3 * j + 1];
+        T.dpz[i] = pos[3 * j + 2]; T.dpm[i] = mass[j];
+    }
+}
+
+/* nbody.cpp:262-305 (refit), with the FP64-mode analogue beside it. */
+static void tree_refit(int n, const double *pos, const double *mass) {
+    T.pos = pos;
+    T.mass = mass;
+    T.disp = (float *)xrealloc(T.disp, sizeof(float) * (size_t)n);
+    T.ddisp = (double *)xrealloc(T.ddisp, sizeof(double) * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        int j = T.idx[i];
+        T.px[i] = (float)pos[3 * j]; T.py[i] = (float)pos[3 * j + 1];
+        T.pz[i] = (float)pos[3 * j + 2]; T.pm[i] = (float)mass[j];
+        T.dpx[i] = pos[3 * j]; T.dpy[i] = pos[3 * j + 1];
+        T.dpz[i] = pos[3 * j + 2]; T.dpm[i] = mass[j];
+        double dx = pos[3 * j] - T.xb[3 * j], dy = pos[3 * j + 1] - T.xb[3 * j + 1],
+               dz = pos[3 * j + 2] - T.xb[3 * j + 2];
+        double d = sqrt(dx * dx + dy * dy + dz * dz);
+        T.disp[i] = mass[j] > 0 ? (float)d : 0.0f;
+        T.ddisp[i] = mass[j] > 0 ? d : 0.0;
+    }
+    const int nn = T.nn;
+    T.node_disp = (float *)xrealloc(T.node_disp, sizeof(float) * (size_t)nn);
+    T.dnode_disp = (double *)xrealloc(T.dnode_disp, sizeof(double) * (size_t)nn);
+    for (int k = nn - 1; k >= 0; --k) {
+        ONode *q = &T.nodes[k];
+        double M = 0, sx = 0, sy = 0, sz = 0;
+        double dM = 0, dsx = 0, dsy = 0, dsz = 0;
+        float dm = 0;
+        double ddm = 0;
+        if (q->leaf) {
+            for (int i = q->start; i < q->start + q->count; ++i) {
+                double m = T.pm[i];
+                M += m; sx += m * T.px[i]; sy += m * T.py[i]; sz += m * T.pz[i];
+                dm = fmaxf(dm, T.disp[i]);
+                double w = T.dpm[i];
+                dM += w; dsx += w * T.dpx[i]; dsy += w * T.dpy[i]; dsz += w * T.dpz[i];
+                ddm = fmax(ddm, T.ddisp[i]);
+            }
+        } else {
+            for (int c = k + 1; c < q->next; c = T.nodes[c].next) {
+                const ONode *cn = &T.nodes[c];
+                M += cn->m; sx += (double)cn->m * cn->cx; sy += (double)cn->m * cn->cy;
+                sz += (double)cn->m * cn->cz;
+                dm = fmaxf(dm, T.node_disp[c]);
+                dM += cn->dm; dsx += cn->dm * cn->dcx; dsy += cn->dm * cn->dcy;
+                dsz += cn->dm * cn->dcz;
+                ddm = fmax(ddm, T.dnode_disp[c]);
+            }
+        }
+        T.node_disp[k] = dm;
+        T.dnode_disp[k] = ddm;
+        double inv = M > 0 ? 1.0 / M : 0.0;
+        q->cx = (float)(sx * inv); q->cy = (float)(sy * inv); q->cz = (float)(sz * inv);
+        q->m = (float)M;
+        double dx = sx * inv - q->gx, dy = sy * inv - q->gy, dz = sz * inv - q->gz;
+        q->open_r = (float)(2 * (q->half + dm) / (double)T.theta + sqrt(dx * dx + dy * dy + dz * dz));
+        double dinv = dM > 0 ? 1.0 / dM : 0.0;
+        q->dcx = dsx * dinv; q->dcy = dsy * dinv; q->dcz = dsz * dinv; q->dm = dM;
+        double ex = q->dcx - q->bcx, ey = q->dcy - q->bcy, ez = q->dcz - q->bcz;
+        q->dopen_r = 2 * (q->bhalf + ddm) / (double)T.theta + sqrt(ex * ex + ey * ey + ez * ez);
+    }
+}
+
+typedef struct {
+    float *x, *y, *z, *m;
+    double *dx, *dy, *dz, *dm;
+    int n, cap;
+} OList;
+
+static void list_push(OList *L, double x, double y, double z, double m, float fx, float fy,
+                      float fz, float fm) {
+    if (L->n == L->cap) {
+        L->cap = L->cap ? 2 * L->cap : 8192;
+        size_t fs = sizeof(float) * (size_t)L->cap, ds = sizeof(double) * (size_t)L->cap;
+        L->x = (float *)xrealloc(L->x, fs); L->y = (float *)xrealloc(L->y, fs);
+        L->z = (float *)xrealloc(L->z, fs); L->m = (float *)xrealloc(L->m, fs);
+        L->dx = (double *)xrealloc(L->dx, ds); L->dy = (double *)xrealloc(L->dy, ds);
+        L->dz = (double *)xrealloc(L->dz, ds); L->dm = (double *)xrealloc(L->dm, ds);
+    }
+    L->x[L->n] = fx; L->y[L->n] = fy; L->z[L->n] = fz; L->m[L->n] = fm;
+    L->dx[L->n] = x; L->dy[L->n] = y; L->dz[L->n] = z; L->dm[L->n] = m;
+    L->n++;
+}
+
+/* nbody.cpp:323-411 (compute_accel): group walk + list evaluation. */
+static long long walk_eval(double *acc, double eps, const uint8_t *active, int mode) {
+    const float eps2 = (float)(eps * eps);
+    const double deps2 = eps * eps;
+    const ONode *nd = T.nodes;
+    const int nn = T.nn, ng = T.ng;
+    long long inter = 0;
+#pragma omp parallel reduction(+ : inter)
+    {
+        OList L;
+        memset(&L, 0, sizeof(L));
+#pragma omp for schedule(dynamic, 4)
+        for (int g = 0; g < ng; ++g) {
+            const ONode *grp = &nd[T.groups[g]];
+            const int b = grp->start, e = grp->start + grp->count;
+            if (active) {
+                int any = 0;
+                for (int i = b; i < e && !any; ++i) any = active[T.idx[i]] != 0;
+                if (!any) continue;
+            }
+            L.n = 0;
+            if (mode == 0) {
+                float x0 = 1e30f, y0 = 1e30f, z0 = 1e30f, x1 = -1e30f, y1 = -1e30f, z1 = -1e30f;
+                for (int i = b; i < e; ++i) {
+                    if (T.pm[i] <= 0) continue;
+                    x0 = fminf(x0, T.px[i]); x1 = fmaxf(x1, T.px[i]);
+                    y0 = fminf(y0, T.py[i]); y1 = fmaxf(y1, T.py[i]);
+                    z0 = fminf(z0, T.pz[i]); z1 = fmaxf(z1, T.pz[i]);
+                }
+                const float bx = 0.5f * (x0 + x1), by = 0.5f * (y0 + y1), bz = 0.5f * (z0 + z1);
+                float br2 = 0;
+                for (int i = b; i < e; ++i) {
+                    if (T.pm[i] <= 0) continue;
+                    float dx = T.px[i] - bx, dy = T.py[i] - by, dz = T.pz[i] - bz;
+                    br2 = fmaxf(br2, dx * dx + dy * dy + dz * dz);
+                }
+                if (x0 > x1) continue;
+                const float br = sqrtf(br2);
+                int i = 0;
+                while (i < nn) {
+                    const ONode *q = &nd[i];
+                    float dx = q->cx - bx, dy = q->cy - by, dz = q->cz - bz;
+                    float t = q->open_r + br;
+                    if (dx * dx + dy * dy + dz * dz > t * t) {
+                        list_push(&L, 0, 0, 0, 0, q->cx, q->cy, q->cz, q->m);
+                        i = q->next;
+                    } else if (q->leaf) {
+                        for (int j = q->start; j < q->start + q->count; ++j)
+                            list_push(&L, 0, 0, 0, 0, T.px[j], T.py[j], T.pz[j], T.pm[j]);
+                        i = q->next;
+                    } else {
+                        i = i + 1;
+                    }
+                }
+                for (int p = b; p < e; ++p) {
+                    if (active && !active[T.idx[p]]) continue;
+                    const float x = T.px[p], y = T.py[p], z = T.pz[p];
+                    float sx = 0, sy = 0, sz = 0;
+                    for (int j = 0; j < L.n; ++j) {
+                        float ex = L.x[j] - x, ey = L.y[j] - y, ez = L.z[j] - z;
+                        float r2 = ex * ex + ey * ey + ez * ez + eps2;
+                        float inv = 1.0f / sqrtf(r2);
+                        float f = L.m[j] * inv * inv * inv;
+                        sx += f * ex; sy += f * ey; sz += f * ez;
+                    }
+                    const int k = T.idx[p];
+                    acc[3 * k] = sx; acc[3 * k + 1] = sy; acc[3 * k + 2] = sz;
+                    inter += L.n;
+                }
+            } else {
+                double x0 = 1e300, y0 = 1e300, z0 = 1e300, x1 = -1e300, y1 = -1e300, z1 = -1e300;
+                for (int i = b; i < e; ++i) {
+                    if (T.dpm[i] <= 0) continue;
+                    x0 = fmin(x0, T.dpx[i]); x1 = fmax(x1, T.dpx[i]);
+                    y0 = fmin(y0, T.dpy[i]); y1 = fmax(y1, T.dpy[i]);
+                    z0 = fmin(z0, T.dpz[i]); z1 = fmax(z1, T.dpz[i]);
+                }
+                if (x0 > x1) continue;
+                const double bx = 0.5 * (x0 + x1), by = 0.5 * (y0 + y1), bz = 0.5 * (z0 + z1);
+                double br2 = 0;
+                for (int i = b; i < e; ++i) {
+                    if (T.dpm[i] <= 0) continue;
+                    double dx = T.dpx[i] - bx, dy = T.dpy[i] - by, dz = T.dpz[i] - bz;
+                    br2 = fmax(br2, dx * dx + dy * dy + dz * dz);
+                }
+                const double br = sqrt(br2);
+                int i = 0;
+                while (i < nn) {
+                    const ONode *q = &nd[i];
+                    double dx = q->dcx - bx, dy = q->dcy - by, dz = q->dcz - bz;
+                    double t = q->dopen_r + br;
+                    if (dx * dx + dy * dy + dz * dz > t * t) {
+                        list_push(&L, q->dcx, q->dcy, q->dcz, q->dm, 0, 0, 0, 0);
+                        i = q->next;
+                    } else if (q->leaf) {
+                        for (int j = q->start; j < q->start + q->count; ++j)
+                            list_push(&L, T.dpx[j], T.dpy[j], T.dpz[j], T.dpm[j], 0, 0, 0, 0);
+                        i = q->next;
+                    } else {
+                        i = i + 1;
+                    }
+                }
+                for (int p = b; p < e; ++p) {
+                    if (active && !active[T.idx[p]]) continue;
+                    const double x = T.dpx[p], y = T.dpy[p], z = T.dpz[p];
+                    double sx = 0, sy = 0, sz = 0;
+                    for (int j = 0; j < L.n; ++j) {
+                        double ex = L.dx[j] - x, ey = L.dy[j] - y, ez = L.dz[j] - z;
+                        double r2 = ex * ex + ey * ey + ez * ez + deps2;
+                        double inv = 1.0 / sqrt(r2);
+                        double f = L.dm[j] * inv * inv * inv;
+                        sx += f * ex; sy += f * ey; sz += f * ez;
+                    }
+                    const int k = T.idx[p];
+                    acc[3 * k] = sx; acc[3 * k + 1] = sy; acc[3 * k + 2] = sz;
+                    inter += L.n;
+                }
+            }
+        }
+        free(L.x); free(L.y); free(L.z); free(L.m);
+        free(L.dx); free(L.dy); free(L.dz); free(L.dm);
+    }
+    return inter;
+}
+
+/* ---------------------------------------------------------------- exports */
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* nbody.cpp:422-425 semantics: only active rows of acc are written; rebuild=0
+ * refits the tree kept from the previous call (a change of n forces a build).
+ * mode 0 = reference FP32 arithmetic, 1 = FP64.  Returns the interaction count
+ * (sum over evaluated particles of their list length). */
+long long orc_accel_active(int n, const double *pos, const double *mass, double *acc, double theta,
+                           double eps, const uint8_t *active, int rebuild, int mode) {
+    if (rebuild || T.n != n || T.nn == 0)
+        tree_build(n, pos, mass, (float)theta);
+    else
+        tree_refit(n, pos, mass);
+    return walk_eval(acc, eps, active, mode);
+}
+
+/* nbody.cpp:428-441: FP64 direct sum for the targets tgt[0..nt) (all if tgt is
+ * NULL, then nt must equal n); acc has nt rows. */
+void orc_accel_direct(int n, const double *pos, const double *mass, int nt, const int64_t *tgt,
+                      double *acc, double eps) {
+    const double eps2 = eps * eps;
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < nt; ++t) {
+        int i = tgt ? (int)tgt[t] : t;
+        double ax = 0, ay = 0, az = 0;
+        for (int j = 0; j < n; ++j) {
+            double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
+                   dz = pos[3 * j + 2] - pos[3 * i + 2];
+            double r2 = dx * dx + dy * dy + dz * dz + eps2;
+            double f = mass[j] / (r2 * sqrt(r2));
+            ax += f * dx; ay += f * dy; az += f * dz;
+        }
+        acc[3 * t] = ax; acc[3 * t + 1] = ay; acc[3 * t + 2] = az;
+    }
+}
+
+/* Tree export of the last build/refit (DFS order, as nbody.cpp lays it out).
+ * Any output pointer may be NULL.  Returns the node count; *ngroups receives
+ * the group count. */
+int orc_tree_export(int *start, int *count, int *next, int *leaf, int *depth, float *com4,
+                    float *open_r, double *dcom4, double *dopen_r, double *box4, int *idx,
+                    int *groups, int *ngroups) {
+    for (int k = 0; k < T.nn; ++k) {
+        const ONode *q = &T.nodes[k];
+        if (start) start[k] = q->start;
+        if (count) count[k] = q->count;
+        if (next) next[k] = q->next;
+        if (leaf) leaf[k] = q->leaf;
+        if (depth) depth[k] = q->depth;
+        if (com4) { com4[4 * k] = q->cx; com4[4 * k + 1] = q->cy; com4[4 * k + 2] = q->cz; com4[4 * k + 3] = q->m; }
+        if (open_r) open_r[k] = q->open_r;
+        if (dcom4) { dcom4[4 * k] = q->dcx; dcom4[4 * k + 1] = q->dcy; dcom4[4 * k + 2] = q->dcz; dcom4[4 * k + 3] = q->dm; }
+        if (dopen_r) dopen_r[k] = q->dopen_r;
+        if (box4) { box4[4 * k] = q->bcx; box4[4 * k + 1] = q->bcy; box4[4 * k + 2] = q->bcz; box4[4 * k + 3] = q->bhalf; }
+    }
+    if (idx) memcpy(idx, T.idx, sizeof(int) * (size_t)T.n);
+    if (groups) memcpy(groups, T.groups, sizeof(int) * (size_t)T.ng);
+    if (ngroups) *ngroups = T.ng;
+    return T.nn;
+}
+
+/* ------------------------------------------------------------------ SPH */
+
+static const double PI_D = 3.14159265358979323846;
+
+/* sph.py:31-40 */
+double orc_kernel(double r, double h) {
+    double q = r / h;
+    double s = 1.0 / (PI_D * (h * h * h));
+    if (q < 1.0) return s * (1 - 1.5 * q * q + 0.75 * (q * q * q));
+    if (q < 2.0) {
+        double t = 2 - q;
+        return s * 0.25 * (t * t * t);
+    }
+    return 0.0;
+}
+
+/* sph.py:43-54: (1/r) dW/dr */
+double orc_kernel_grad(double r, double h) {
+    double q = r / h;
+    double s = 1.0 / (PI_D * (h * h * h * h));
+    double g;
+    if (q < 1.0) {
+        g = s * (-3 * q + 2.25 * q * q);
+    } else if (q < 2.0) {
+        double t = 2 - q;
+        g = -s * 0.75 * (t * t);
+    } else {
+        return 0.0;
+    }
+    return g / fmax(r, 1e-30);
+}
+
+/* sph.py:57-106 (sph_forces), FP64, same loop structure.  nb is (n, k) int64
+ * of local indices (self included).  divv (may be NULL) receives the SPH
+ * velocity divergence -(1/rho_i) sum_j m_j (v_ij . r_ij) (1/r) dW/dr(h_i). */
+void orc_sph_forces(int n, int k, const double *pos, const double *vel, const double *mass,
+                    const int64_t *nb, const double *h, const double *u, double gamma,
+                    double alpha, double beta, double *rho, double *acc, double *dudt,
+                    double *dt_i, double *divv) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int jj = 0; jj < k; ++jj) {
+            int64_t j = nb[(size_t)i * k + jj];
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            s += mass[j] * orc_kernel(sqrt(dx * dx + dy * dy + dz * dz), h[i]);
+        }
+        rho[i] = s;
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double Pi = (gamma - 1) * rho[i] * u[i];
+        double ci = sqrt(gamma * (gamma - 1) * u[i]);
+        double ax = 0, ay = 0, az = 0, du = 0, mu_max = 0, dv = 0;
+        for (int jj = 0; jj < k; ++jj) {
+            int64_t j = nb[(size_t)i * k + jj];
+            if (j == i) continue;
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double r = sqrt(dx * dx + dy * dy + dz * dz);
+            double gi = orc_kernel_grad(r, h[i]);
+            double gj = orc_kernel_grad(r, h[j]);
+            double Pj = (gamma - 1) * rho[j] * u[j];
+            double cj = sqrt(gamma * (gamma - 1) * u[j]);
+            double vr = (vel[3 * i] - vel[3 * j]) * dx + (vel[3 * i + 1] - vel[3 * j + 1]) * dy +
+                        (vel[3 * i + 2] - vel[3 * j + 2]) * dz;
+            double visc = 0.0;
+            if (vr < 0) {
+                double hb = 0.5 * (h[i] + h[j]);
+                double mu = hb * vr / (r * r + 0.01 * hb * hb);
+                mu_max = fmax(mu_max, -mu);
+                visc = (-alpha * 0.5 * (ci + cj) * mu + beta * mu * mu) / (0.5 * (rho[i] + rho[j]));
+            }
+            double gb = 0.5 * (gi + gj);
+            double pi2 = Pi / (rho[i] * rho[i]);
+            double f = pi2 * gi + Pj / (rho[j] * rho[j]) * gj + visc * gb;
+            ax -= mass[j] * f * dx;
+            ay -= mass[j] * f * dy;
+            az -= mass[j] * f * dz;
+            du += mass[j] * (pi2 * gi + 0.5 * visc * gb) * vr;
+            dv += mass[j] * vr * gi;
+        }
+        acc[3 * i] = ax; acc[3 * i + 1] = ay; acc[3 * i + 2] = az;
+        dudt[i] = du;
+        dt_i[i] = 0.3 * h[i] / (ci + 1.2 * (alpha * ci + beta * mu_max) + 1e-30);
+        if (divv) divv[i] = -dv / rho[i];
+    }
+}
+
+/* Strict order on (d^2, index): the order the exact kNN outputs are sorted by. */
+static int key_less(double da, int64_t ia, double db, int64_t ib) {
+    return da < db || (da == db && ia < ib);
+}
+
+/* Exact kNN by brute force (sph.py:165-166 semantics: the particle itself is
+ * column 0, distances ascending; ties broken by index).  O(n^2): oracle sizes
+ * only.  dist receives sqrt(d^2). */
+void orc_knn_brute(int n, const double *pos, int k, int64_t *nb, double *dist) {
+#pragma omp parallel
+    {
+        double *bd = (double *)xrealloc(NULL, sizeof(double) * (size_t)(k + 1));
+        int64_t *bi = (int64_t *)xrealloc(NULL, sizeof(int64_t) * (size_t)(k + 1));
+#pragma omp for schedule(dynamic, 16)
+        for (int i = 0; i < n; ++i) {
+            int cnt = 0;
+            for (int j = 0; j < n; ++j) {
+                double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                       dz = pos[3 * i + 2] - pos[3 * j + 2];
+                double d2 = dx * dx + dy * dy + dz * dz;
+                if (cnt == k && !key_less(d2, j, bd[k - 1], bi[k - 1])) continue;
+                int p = cnt < k ? cnt++ : k - 1;
+                while (p > 0 && key_less(d2, j, bd[p - 1], bi[p - 1])) {
+                    bd[p] = bd[p - 1]; bi[p] = bi[p - 1]; --p;
+                }
+                bd[p] = d2; bi[p] = j;
+            }
+            for (int c = 0; c < k; ++c) {
+                nb[(size_t)i * k + c] = c < cnt ? bi[c] : -1;
+                dist[(size_t)i * k + c] = c < cnt ? sqrt(bd[c]) : INFINITY;
+            }
+        }
+        free(bd); free(bi);
+    }
+}
+
+/* sph.py:109-135 (knn_from_candidates): k nearest among each row's candidates
+ * (-1 = none), ascending (d^2, index).  Returns 1 if every row had >= k valid
+ * candidates, else 0 (rows that fail are left untouched). */
+int orc_knn_from_candidates(int n, int m, const double *pos, const int64_t *cand, int k,
+                            int64_t *nb, double *dist) {
+    int ok = 1;
+#pragma omp parallel
+    {
+        double *bd = (double *)xrealloc(NULL, sizeof(double) * (size_t)(m + 1));
+        int64_t *bi = (int64_t *)xrealloc(NULL, sizeof(int64_t) * (size_t)(m + 1));
+#pragma omp for schedule(static) reduction(& : ok)
+        for (int i = 0; i < n; ++i) {
+            int cnt = 0;
+            for (int jj = 0; jj < m; ++jj) {
+                int64_t j = cand[(size_t)i * m + jj];
+                if (j < 0) continue;
+                double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                       dz = pos[3 * i + 2] - pos[3 * j + 2];
+                double d2 = dx * dx + dy * dy + dz * dz;
+                int p = cnt++;
+                while (p > 0 && key_less(d2, j, bd[p - 1], bi[p - 1])) {
+                    bd[p] = bd[p - 1]; bi[p] = bi[p - 1]; --p;
+                }
+                bd[p] = d2; bi[p] = j;
+            }
+            if (cnt < k) { ok = 0; continue; }
+            for (int c = 0; c < k; ++c) {
+                nb[(size_t)i * k + c] = bi[c];
+                dist[(size_t)i * k + c] = sqrt(bd[c]);
+            }
+        }
+        free(bd); free(bi);
+    }
+    return ok;
+}

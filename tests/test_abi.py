@@ -1,0 +1,42 @@
+"""CPU checks of the C-ABI library: it loads without a GPU and exports every
+function include/gf.h declares, with the reference engine's five symbols
+(cpp/nbody.cpp:415-505) among them."""
+
+import ctypes
+import os
+import re
+
+from galaxy_formation_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    src = open(os.path.join(ROOT, "include", "gf.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(?:int|long long|const char\*|void)\s+\**(\w+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared()
+    assert len(names) >= 5
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_reference_engine_symbols_present():
+    names = declared()
+    for s in ("nbody_num_threads", "nbody_accel_active", "nbody_accel_direct", "nbody_render",
+              "nbody_tonemap"):
+        assert s in names
+
+
+def test_python_signatures_cover_the_header():
+    assert set(declared()) <= set(_lib.SIGNATURES)
+
+
+def test_loader_declares_argtypes():
+    L = _lib.load()
+    assert L.nbody_accel_active.argtypes is not None
+    assert L.gf_version() >= 1

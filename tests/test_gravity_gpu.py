@@ -1,0 +1,232 @@
+"""GPU parity of the Barnes-Hut gravity path (libgf.so) against the reference
+fixtures (tests/golden, written by the reference cpp/nbody.cpp built from
+source) and the CPU oracle (oracle/gf_oracle.c)."""
+
+import numpy as np
+import pytest
+
+from galaxy_formation_amd import ics
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from galaxy_formation_amd.engine import Engine
+    return Engine("fp64")
+
+
+def relerr(a, b):
+    nb = np.linalg.norm(b, axis=1)
+    d = np.linalg.norm(a - b, axis=1)
+    return np.where(nb > 0, d / np.where(nb > 0, nb, 1), d)
+
+
+def run(eng, pos, mass, theta, eps, precision, active=None, rebuild=True, acc=None):
+    eng.precision = precision
+    if acc is None:
+        acc = np.zeros_like(pos)
+    eng.gravity(pos, mass, acc, theta, eps, active=active, rebuild=rebuild)
+    return acc
+
+
+CASES = {
+    "bh_gauss3k_t05": lambda: ics.gaussian_blob(3000, 1),
+    "bh_cloud10k_t05": lambda: (ics.config("cloud10k").pos, ics.config("cloud10k").mass),
+}
+
+
+def _massless2k():
+    rng = np.random.default_rng(5)
+    pos = np.ascontiguousarray(rng.uniform(-1, 1, size=(2000, 3)))
+    mass = rng.uniform(0.5, 1.5, 2000) / 2000
+    mass[rng.random(2000) < 0.1] = 0.0
+    return pos, mass
+
+
+CASES["bh_massless2k_t05"] = _massless2k
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_tree_topology_equals_reference(eng, golden, name):
+    """Node ranges, leaf flags, next pointers, tree order and groups are the
+    reference's (cpp/nbody.cpp:202-256), bit for bit."""
+    g = golden(name)
+    pos, mass = CASES[name]()
+    assert ics.sha256(pos, mass) == str(g["input_sha"])
+    run(eng, pos, mass, float(g["theta"]), float(g["eps"]), "fp64")
+    t = eng.tree_export()
+    for k in ("start", "count", "next", "leaf", "idx", "groups"):
+        np.testing.assert_array_equal(t[k], g["tree_" + k], err_msg=k)
+    # reference CoM: FP64 sums of FP32-rounded children, stored FP32: equal
+    # to a few FP32 ulps at the coordinate scale
+    scale = np.abs(pos).max()
+    np.testing.assert_allclose(t["com4"][:, :3].astype(np.float32), g["tree_com4"][:, :3],
+                               rtol=0, atol=4 * 2.0 ** -23 * scale)
+    np.testing.assert_allclose(t["com4"][:, 3].astype(np.float32), g["tree_com4"][:, 3], rtol=4e-7)
+
+
+@pytest.mark.parametrize("name", ["bh_cloud200k_t05", "bh_disk1m_t05", "bh_gauss20k_t07"])
+def test_tree_topology_hash_equals_reference(eng, golden, name):
+    g = golden(name)
+    if name.startswith("bh_gauss"):
+        pos, mass = ics.gaussian_blob(20000, 0)
+    else:
+        c = ics.config(name.split("_")[1])
+        pos, mass = c.pos, c.mass
+    assert ics.sha256(pos, mass) == str(g["input_sha"])
+    run(eng, pos, mass, float(g["theta"]), float(g["eps"]), "fp64")
+    nn, ng, deep = eng.tree_info()
+    assert deep == 0
+    assert (nn, ng) == (int(g["n_nodes"]), int(g["n_groups"]))
+    t = eng.tree_export()
+    assert ics.sha256(t["start"], t["count"], t["next"], t["leaf"]) == str(g["topo_sha"])
+    assert ics.sha256(t["idx"]) == str(g["idx_sha"])
+    assert ics.sha256(t["groups"]) == str(g["groups_sha"])
+
+
+def test_tree_fp64_centres_equal_oracle(eng):
+    """FP64 centres of mass and opening radii are the oracle's, bit for bit
+    (same summation order, no contraction)."""
+    c = ics.make_cloud(3000, 3000, seed=4)
+    run(eng, c.pos, c.mass, 0.5, c.eps, "fp64")
+    t = eng.tree_export()
+    O.accel(c.pos, c.mass, 0.5, c.eps, mode=1)
+    o = O.tree_export()
+    np.testing.assert_array_equal(t["box4"], o["box4"])
+    np.testing.assert_array_equal(t["com4"], o["dcom4"])
+    np.testing.assert_array_equal(t["open_r"], o["dopen_r"])
+
+
+@pytest.mark.parametrize("theta", [0.4, 0.5, 0.7])
+def test_fp64_equals_oracle(eng, theta):
+    c = ics.config("cloud10k")
+    acc = run(eng, c.pos, c.mass, theta, c.eps, "fp64")
+    inter = eng.last_interactions()
+    ref, oi = O.accel(c.pos, c.mass, theta, c.eps, mode=1)
+    assert inter == oi
+    assert relerr(acc, ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("theta", [0.4, 0.5, 0.7])
+def test_fp32_close_to_oracle_fp64(eng, theta):
+    c = ics.config("cloud10k")
+    acc = run(eng, c.pos, c.mass, theta, c.eps, "fp32")
+    ref, _ = O.accel(c.pos, c.mass, theta, c.eps, mode=1)
+    e = relerr(acc, ref)
+    assert np.sqrt(np.mean(e ** 2)) < 2e-6 and e.max() < 5e-5
+
+
+@pytest.mark.parametrize("name", ["bh_cloud10k_t04", "bh_cloud10k_t05", "bh_cloud10k_t07",
+                                  "bh_cloud200k_t04", "bh_cloud200k_t05", "bh_cloud200k_t07",
+                                  "bh_disk1m_t05"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_accel_close_to_reference(eng, golden, name, precision):
+    """Against the reference's own FP32 Barnes-Hut at the same theta: the
+    same tree and acceptance rule, so differences are FP32 rounding of the
+    reference plus the rare node whose acceptance flips under it."""
+    g = golden(name)
+    c = ics.config(name.split("_")[1])
+    acc = run(eng, c.pos, c.mass, float(g["theta"]), float(g["eps"]), precision)
+    e = relerr(acc[g["sub"]], g["acc_sub"])
+    assert np.median(e) < 1e-5
+    assert np.quantile(e, 0.99) < 1e-4
+    assert e.max() < 5e-3
+
+
+def test_barnes_hut_matches_direct_sum(eng):
+    """tests/test_physics.py:20-30 of the reference: median < 1 %."""
+    pos, mass = ics.gaussian_blob(20000, 0)
+    for p in ("fp32", "fp64"):
+        acc = run(eng, pos, mass, 0.7, 0.01, p)
+        exact = eng.gravity_direct(pos, mass, eps=0.01)
+        assert np.median(relerr(acc, exact)) < 1e-2
+
+
+def test_direct_equals_oracle(eng):
+    c = ics.make_cloud(3000, 3000, seed=9)
+    a = eng.gravity_direct(c.pos, c.mass, c.eps)
+    b = O.direct(c.pos, c.mass, c.eps)
+    assert relerr(a, b).max() < 1e-13
+    sub = np.array([5, 17, 4000, 5999])
+    np.testing.assert_allclose(eng.gravity_direct_subset(c.pos, c.mass, c.eps, sub), b[sub],
+                               rtol=1e-13)
+
+
+def test_theta_zero_is_direct_sum(eng):
+    """theta -> 0 opens every node down to the leaves: the direct sum."""
+    c = ics.make_cloud(2000, 2000, seed=2)
+    acc = run(eng, c.pos, c.mass, 1e-6, c.eps, "fp64")
+    assert relerr(acc, O.direct(c.pos, c.mass, c.eps)).max() < 1e-12
+
+
+def test_active_subset(eng):
+    """Inactive rows are left bit-unchanged; active rows equal the all-active
+    result (nbody.cpp:344-348, :393, :404-407)."""
+    c = ics.config("cloud10k")
+    full = run(eng, c.pos, c.mass, 0.5, c.eps, "fp32")
+    act = (np.random.default_rng(3).random(c.n) < 0.2).astype(np.uint8)
+    acc = np.full_like(c.pos, np.nan)
+    run(eng, c.pos, c.mass, 0.5, c.eps, "fp32", active=act, acc=acc)
+    on = act.astype(bool)
+    np.testing.assert_array_equal(acc[on], full[on])
+    assert np.isnan(acc[~on]).all()
+    acc0 = np.full_like(c.pos, 3.5)
+    run(eng, c.pos, c.mass, 0.5, c.eps, "fp32", active=np.zeros(c.n, np.uint8), acc=acc0)
+    assert (acc0 == 3.5).all()
+
+
+def test_refit_matches_oracle_and_reference(eng, golden):
+    """rebuild = 0 refits the previous tree (nbody.cpp:262-305): same cells,
+    new centres of mass, opening radii enlarged by the largest displacement."""
+    g = golden("bh_refit_cloud10k_t05")
+    c = ics.config("cloud10k")
+    assert ics.sha256(c.pos, c.mass) == str(g["input_sha"])
+    pos1 = np.ascontiguousarray(c.pos + float(g["drift"]) * c.vel)
+    act = g["active"]
+    for p in ("fp64", "fp32"):
+        run(eng, c.pos, c.mass, 0.5, c.eps, p)
+        acc1 = np.full((c.n, 3), 7.25)
+        run(eng, pos1, c.mass, 0.5, c.eps, p, active=act, rebuild=False, acc=acc1)
+        on = act.astype(bool)
+        assert (acc1[~on] == 7.25).all()
+        e = relerr(acc1[on], g["acc1"][on])
+        assert np.median(e) < 1e-5 and e.max() < 5e-3
+        O.accel(c.pos, c.mass, 0.5, c.eps, mode=1)
+        ref1 = np.full((c.n, 3), 7.25)
+        O.accel(pos1, c.mass, 0.5, c.eps, active=act, rebuild=False, mode=1, acc=ref1)
+        e = relerr(acc1[on], ref1[on])
+        assert e.max() < (1e-12 if p == "fp64" else 5e-5)
+    # refit with zero displacement is the build
+    run(eng, c.pos, c.mass, 0.5, c.eps, "fp64")
+    t0 = eng.tree_export()
+    a0 = run(eng, c.pos, c.mass, 0.5, c.eps, "fp64", rebuild=False)
+    t1 = eng.tree_export()
+    np.testing.assert_array_equal(t0["com4"], t1["com4"])
+    np.testing.assert_array_equal(t0["open_r"], t1["open_r"])
+    np.testing.assert_array_equal(a0, run(eng, c.pos, c.mass, 0.5, c.eps, "fp64"))
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 8, 9, 65, 700])
+def test_small_and_ragged_sizes(eng, n):
+    pos, mass = ics.gaussian_blob(n, 7)
+    for p in ("fp32", "fp64"):
+        acc = run(eng, pos, mass, 0.5, 0.05, p)
+        ref, _ = O.accel(pos, mass, 0.5, 0.05, mode=1)
+        assert relerr(acc, ref).max() < (1e-12 if p == "fp64" else 5e-5)
+
+
+def test_empty_input(eng):
+    acc = np.zeros((0, 3))
+    eng.gravity(np.zeros((0, 3)), np.zeros(0), acc, 0.5, 0.1)
+
+
+def test_clustered_duplicates(eng):
+    """Many particles at identical positions (deep cells)."""
+    rng = np.random.default_rng(12)
+    pos = np.ascontiguousarray(np.repeat(rng.normal(size=(300, 3)), 6, axis=0))
+    mass = np.full(len(pos), 1.0 / len(pos))
+    acc = run(eng, pos, mass, 0.5, 0.05, "fp64")
+    ref, _ = O.accel(pos, mass, 0.5, 0.05, mode=1)
+    assert relerr(acc, ref).max() < 1e-12
