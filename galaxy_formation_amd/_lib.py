@@ -37,12 +37,40 @@ SIGNATURES = {
     "gf_tree_info": (_ci, [_vp, _vp, _vp]),
     "gf_tree_export": (_ci, [_ci] + [_vp] * 10),
     "gf_gravity_dev": (_ci, [_ci, _vp, _vp, _vp, _cd, _cd, _vp, _ci, _ci, _vp]),
+    "gf_tree_build_dev": (_ci, [_ci, _vp, _vp, _cd, _vp]),
+    "gf_walk_dev": (_ci, [_ci, _vp, _vp, _cd, _ci, _vp]),
     "gf_direct_dev": (_ci, [_ci, _vp, _vp, _ci, _vp, _vp, _cd, _vp]),
     "gf_interactions_dev": (_ci, [_vp]),
     "gf_sync": (_ci, [_vp]),
+    # SPH
+    "gf_sph_kernel": (_ci, [_ci, _dbl, _dbl, _vp, _vp]),
+    "gf_sph_forces": (_ci, [_ci, _ci, _dbl, _dbl, _dbl, _i64, _dbl, _dbl, _cd, _cd, _cd,
+                            _vp, _vp, _vp, _vp, _vp]),
+    "gf_knn": (_ci, [_ci, _dbl, _ci, _vp, _i64, _dbl, _vp]),
+    "gf_knn_from_candidates_host": (_ci, [_ci, _ci, _dbl, _i64, _ci, _i64, _dbl]),
+    "gf_hydro_dev": (_ci, [_ci, _vp, _vp, _vp, _vp, _cd, _ci, _vp, _cd, _cd, _cd,
+                           _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gf_hydro_setup_dev": (_ci, [_ci, _vp, _vp, _vp, _vp, _vp]),
+    "gf_hydro_knn_dev": (_ci, [_ci, _vp, _vp]),
+    "gf_hydro_forces_dev": (_ci, [_cd, _cd, _cd, _cd, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gf_hydro_passes": (_ci, [_vp, _vp]),
 }
 
 _lib = None
+
+
+def _preload_torch_hip_runtime():
+    """One HIP runtime per process.  libgf.so needs libamdhip64.so.7; PyTorch
+    ships its own copy (same SONAME) and loads it under another file name.
+    Loading PyTorch's copy first, by path, makes libgf.so and a later
+    `import torch` share it; without PyTorch, /opt/rocm's is used."""
+    import importlib.util
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
 
 
 class GFError(RuntimeError):
@@ -57,6 +85,7 @@ def load():
             raise GFError(f"{LIB_PATH} is missing: build it first (python -c "
                           "'import __graft_entry__ as g; g.build()' or make -C "
                           "galaxy_formation_amd/csrc)")
+        _preload_torch_hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name, None)

@@ -2,6 +2,7 @@
 #pragma once
 #include "../../include/gf.h"
 #include "gf_gravity.h"
+#include "gf_sph.h"
 
 #define GF_ABI_VERSION 1
 
@@ -14,6 +15,8 @@ struct GfCtx {
     unsigned long long last_inter = 0;
     GfTree tree;
     GfWalkScratch walk;
+    GfSph sph;
+    int sph_k = 0;                  // k of the held neighbour solve
     // staging buffers of the host-pointer entry points (grown, never shrunk)
     double* d_pos = nullptr; size_t d_pos_cap = 0;
     double* d_mass = nullptr; size_t d_mass_cap = 0;
@@ -25,6 +28,14 @@ struct GfCtx {
     void* d_glow = nullptr; size_t d_glow_cap = 0;
     void* d_out8 = nullptr; size_t d_out8_cap = 0;
     void* d_small = nullptr; size_t d_small_cap = 0;
+    double* d_vel = nullptr; size_t d_vel_cap = 0;
+    double* d_h = nullptr; size_t d_h_cap = 0;
+    double* d_u = nullptr; size_t d_u_cap = 0;
+    double* d_out1 = nullptr; size_t d_out1_cap = 0;   // 5 n doubles of scalar outputs
+    int64_t* d_nb = nullptr; size_t d_nb_cap = 0;
+    double* d_dist = nullptr; size_t d_dist_cap = 0;
+    int* d_scratch = nullptr; size_t d_scratch_cap = 0;
+    int* d_flag = nullptr; size_t d_flag_cap = 0;
 };
 
 extern GfCtx g_ctx;

@@ -233,6 +233,33 @@ int gf_gravity_dev(int n, const double* pos, const double* mass, double* acc, do
                         gf_stream(stream));
 }
 
+int gf_tree_build_dev(int n, const double* pos, const double* mass, double theta, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (n < 0) {
+        gf_set_error("gf_tree_build_dev: n must be >= 0");
+        return -1;
+    }
+    return gf_tree_build(g_ctx.tree, n, pos, mass, float(theta), gf_stream(stream));
+}
+
+int gf_walk_dev(int n, const uint8_t* active, double* acc, double eps, int precision,
+                void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (n != g_ctx.tree.n || (n > 0 && !g_ctx.tree.built)) {
+        gf_set_error("gf_walk_dev: no tree built for n = %d", n);
+        return -1;
+    }
+    if (precision != GF_FP32 && precision != GF_FP64) {
+        gf_set_error("gf_walk_dev: bad precision");
+        return -1;
+    }
+    if (n == 0) return 0;
+    hipStream_t s = gf_stream(stream);
+    if (gf_walk_reserve(g_ctx.walk, g_ctx.tree.cap_nodes)) return -1;
+    GF_HIP(hipMemsetAsync(g_ctx.walk.inter, 0, sizeof(unsigned long long), s));
+    return gf_walk(g_ctx.tree, g_ctx.walk, active, acc, eps, precision, s);
+}
+
 int gf_direct_dev(int n, const double* pos, const double* mass, int nt, const int64_t* tgt,
                   double* acc, double eps, void* stream) {
     if (gf_ctx_init()) return -1;
@@ -301,6 +328,193 @@ int nbody_tonemap(int W, int H, int ngroups, const float* img, const float* glow
     GF_HIP(hipMemcpyAsync(out, g_ctx.d_out8, bout, hipMemcpyDeviceToHost, s));
     GF_HIP(hipStreamSynchronize(s));
     return 0;
+}
+
+// ------------------------------------------------------------------ SPH
+
+int gf_sph_kernel(int n, const double* r, const double* h, double* w, double* g) {
+    if (gf_ctx_init()) return -1;
+    if (n <= 0) return 0;
+    hipStream_t s = g_ctx.stream;
+    const size_t b = sizeof(double) * size_t(n);
+    if (STAGE(g_ctx.d_pos, 2 * b) || STAGE(g_ctx.d_out1, 2 * b)) return -1;
+    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, r, b, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(g_ctx.d_pos + n, h, b, hipMemcpyHostToDevice, s));
+    if (gf_sph_kernel_eval(n, g_ctx.d_pos, g_ctx.d_pos + n, w ? g_ctx.d_out1 : nullptr,
+                           g ? g_ctx.d_out1 + n : nullptr, s))
+        return -1;
+    if (w) GF_HIP(hipMemcpyAsync(w, g_ctx.d_out1, b, hipMemcpyDeviceToHost, s));
+    if (g) GF_HIP(hipMemcpyAsync(g, g_ctx.d_out1 + n, b, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int gf_sph_forces(int n, int k, const double* pos, const double* vel, const double* mass,
+                  const int64_t* nb, const double* h, const double* u, double gamma, double alpha,
+                  double beta, double* rho, double* acc, double* dudt, double* dt_i, double* divv) {
+    if (gf_ctx_init()) return -1;
+    if (n < 0 || k < 0) {
+        gf_set_error("gf_sph_forces: bad n or k");
+        return -1;
+    }
+    if (n == 0) return 0;
+    hipStream_t s = g_ctx.stream;
+    const size_t b1 = sizeof(double) * size_t(n), b3 = 3 * b1;
+    if (STAGE(g_ctx.d_pos, b3) || STAGE(g_ctx.d_vel, b3) || STAGE(g_ctx.d_mass, b1) ||
+        STAGE(g_ctx.d_h, b1) || STAGE(g_ctx.d_u, b1) || STAGE(g_ctx.d_acc, b3) ||
+        STAGE(g_ctx.d_out1, 4 * b1) || STAGE(g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k) ||
+        STAGE(g_ctx.d_scratch, gf_sph_table_scratch_bytes(n, k)))
+        return -1;
+    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(g_ctx.d_vel, vel, b3, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(g_ctx.d_mass, mass, b1, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(g_ctx.d_h, h, b1, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(g_ctx.d_u, u, b1, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(g_ctx.d_nb, nb, sizeof(int64_t) * size_t(n) * k, hipMemcpyHostToDevice, s));
+    double* o = g_ctx.d_out1;
+    if (gf_sph_forces_table(n, k, g_ctx.d_pos, g_ctx.d_vel, g_ctx.d_mass, g_ctx.d_nb, g_ctx.d_h,
+                            g_ctx.d_u, gamma, alpha, beta, o, g_ctx.d_acc, o + n, o + 2 * size_t(n),
+                            o + 3 * size_t(n), g_ctx.d_scratch, s))
+        return -1;
+    if (rho) GF_HIP(hipMemcpyAsync(rho, o, b1, hipMemcpyDeviceToHost, s));
+    if (acc) GF_HIP(hipMemcpyAsync(acc, g_ctx.d_acc, b3, hipMemcpyDeviceToHost, s));
+    if (dudt) GF_HIP(hipMemcpyAsync(dudt, o + n, b1, hipMemcpyDeviceToHost, s));
+    if (dt_i) GF_HIP(hipMemcpyAsync(dt_i, o + 2 * size_t(n), b1, hipMemcpyDeviceToHost, s));
+    if (divv) GF_HIP(hipMemcpyAsync(divv, o + 3 * size_t(n), b1, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int gf_knn(int n, const double* pos, int k, const double* r_init, int64_t* nb, double* dist,
+           double* mean_passes) {
+    if (gf_ctx_init()) return -1;
+    if (n <= 0) return 0;
+    if (k < 1 || k > n) {
+        gf_set_error("gf_knn: k = %d must be in [1, n = %d]", k, n);
+        return -1;
+    }
+    hipStream_t s = g_ctx.stream;
+    const size_t b1 = sizeof(double) * size_t(n), b3 = 3 * b1;
+    if (STAGE(g_ctx.d_pos, b3) || STAGE(g_ctx.d_mass, b1) || STAGE(g_ctx.d_h, b1) ||
+        STAGE(g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k) ||
+        STAGE(g_ctx.d_dist, sizeof(double) * size_t(n) * k))
+        return -1;
+    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemsetAsync(g_ctx.d_mass, 0, b1, s));
+    if (r_init) GF_HIP(hipMemcpyAsync(g_ctx.d_h, r_init, b1, hipMemcpyHostToDevice, s));
+    GfSph& S = g_ctx.sph;
+    if (gf_sph_setup(S, n, g_ctx.d_pos, nullptr, g_ctx.d_mass, nullptr, s)) return -1;
+    if (gf_sph_knn(S, k, r_init ? g_ctx.d_h : nullptr, false, true, 64, s)) return -1;
+    g_ctx.sph_k = 0;   // the held table holds input indices: not usable for forces
+    if (gf_sph_export_knn(S, k, g_ctx.d_nb, g_ctx.d_dist, s)) return -1;
+    GF_HIP(hipMemcpyAsync(nb, g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipMemcpyAsync(dist, g_ctx.d_dist, sizeof(double) * size_t(n) * k, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    if (mean_passes) {
+        int* p = new int[size_t(n)];
+        GF_HIP(hipMemcpy(p, S.passes, sizeof(int) * size_t(n), hipMemcpyDeviceToHost));
+        long long t = 0;
+        for (int i = 0; i < n; ++i) t += p[i];
+        delete[] p;
+        *mean_passes = double(t) / double(n);
+    }
+    return S.last_passes;
+}
+
+int gf_knn_from_candidates_host(int n, int m, const double* pos, const int64_t* cand, int k,
+                                int64_t* nb, double* dist) {
+    if (gf_ctx_init()) return -1;
+    if (n <= 0) return 1;
+    hipStream_t s = g_ctx.stream;
+    const size_t b3 = sizeof(double) * 3 * size_t(n);
+    if (STAGE(g_ctx.d_pos, b3) || STAGE(g_ctx.d_nb, sizeof(int64_t) * size_t(n) * (m + k)) ||
+        STAGE(g_ctx.d_dist, sizeof(double) * size_t(n) * k) || STAGE(g_ctx.d_flag, sizeof(int)))
+        return -1;
+    int one = 1;
+    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
+    int64_t* dcand = g_ctx.d_nb;
+    int64_t* dnb = g_ctx.d_nb + size_t(n) * m;
+    GF_HIP(hipMemcpyAsync(dcand, cand, sizeof(int64_t) * size_t(n) * m, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(dnb, nb, sizeof(int64_t) * size_t(n) * k, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(g_ctx.d_dist, dist, sizeof(double) * size_t(n) * k, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(g_ctx.d_flag, &one, sizeof(int), hipMemcpyHostToDevice, s));
+    if (gf_knn_from_candidates(n, m, g_ctx.d_pos, dcand, k, dnb, g_ctx.d_dist, g_ctx.d_flag, s))
+        return -1;
+    int ok = 0;
+    GF_HIP(hipMemcpyAsync(nb, dnb, sizeof(int64_t) * size_t(n) * k, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipMemcpyAsync(dist, g_ctx.d_dist, sizeof(double) * size_t(n) * k, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipMemcpyAsync(&ok, g_ctx.d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    return ok;
+}
+
+// device-resident hydro: gas tree, neighbour solve, h, density, forces
+int gf_hydro_setup_dev(int n, const double* pos, const double* vel, const double* mass,
+                       const double* u, void* stream) {
+    if (gf_ctx_init()) return -1;
+    g_ctx.sph_k = 0;
+    return gf_sph_setup(g_ctx.sph, n, pos, vel, mass, u, gf_stream(stream));
+}
+
+int gf_hydro_knn_dev(int k, const double* r_init, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (gf_sph_knn(g_ctx.sph, k, r_init, true, false, 64, gf_stream(stream))) return -1;
+    g_ctx.sph_k = k;
+    return g_ctx.sph.last_passes;
+}
+
+int gf_hydro_forces_dev(double eps, double gamma, double alpha, double beta, double* h,
+                        double* rho, double* acc, double* dudt, double* dt, double* divv,
+                        void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (g_ctx.sph.tree.n > 0 && g_ctx.sph_k <= 0) {
+        gf_set_error("gf_hydro_forces_dev: no neighbour solve held");
+        return -1;
+    }
+    hipStream_t s = gf_stream(stream);
+    if (gf_sph_smoothing(g_ctx.sph, g_ctx.sph_k, eps, s)) return -1;
+    return gf_sph_forces_tree(g_ctx.sph, g_ctx.sph_k, gamma, alpha, beta, rho, acc, dudt, dt, divv,
+                              h, s);
+}
+
+int gf_hydro_dev(int n, const double* pos, const double* vel, const double* mass, const double* u,
+                 double eps, int k, const double* r_init, double gamma, double alpha, double beta,
+                 double* h, double* rho, double* acc, double* dudt, double* dt, double* divv,
+                 void* stream) {
+    if (gf_hydro_setup_dev(n, pos, vel, mass, u, stream)) return -1;
+    if (n == 0) return 0;
+    const int passes = gf_hydro_knn_dev(k, r_init, stream);
+    if (passes < 0) return -1;
+    if (gf_hydro_forces_dev(eps, gamma, alpha, beta, h, rho, acc, dudt, dt, divv, stream)) return -1;
+    return passes;
+}
+
+int gf_hydro_passes(int* per_particle, long long* total) {
+    if (gf_ctx_init()) return -1;
+    const int n = g_ctx.sph.tree.n;
+    if (n == 0) {
+        if (total) *total = 0;
+        return 0;
+    }
+    int* p = new int[size_t(n)];
+    uint32_t* idx = new uint32_t[size_t(n)];
+    hipError_t e1 = hipMemcpy(p, g_ctx.sph.passes, sizeof(int) * size_t(n), hipMemcpyDeviceToHost);
+    hipError_t e2 = hipMemcpy(idx, g_ctx.sph.tree.idx, sizeof(uint32_t) * size_t(n), hipMemcpyDeviceToHost);
+    long long t = 0;
+    if (e1 == hipSuccess && e2 == hipSuccess) {
+        for (int i = 0; i < n; ++i) {
+            t += p[i];
+            if (per_particle) per_particle[idx[i]] = p[i];
+        }
+    }
+    delete[] p;
+    delete[] idx;
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        gf_set_error("gf_hydro_passes: copy failed");
+        return -1;
+    }
+    if (total) *total = t;
+    return g_ctx.sph.last_passes;
 }
 
 }  // extern "C"

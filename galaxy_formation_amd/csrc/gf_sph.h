@@ -1,0 +1,80 @@
+// SPH on the device (gf_sph.hip): exact k-nearest-neighbour solve, smoothing
+// lengths, density and pressure/viscosity forces of the gas.
+#pragma once
+#include "gf_tree.h"
+
+struct GfSph {
+    GfTree tree;                 // octree of the gas particles
+    int cap_n = 0, cap_k = 0;
+    // per gas particle, gas-tree order
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr, *u = nullptr;
+    double *r = nullptr;         // search radius of the current pass
+    double *dk = nullptr;        // distance to the k-th neighbour (self included)
+    double *h = nullptr, *rho = nullptr, *ih = nullptr, *sg = nullptr;
+    int8_t *done = nullptr;
+    int *passes = nullptr;
+    int *inv = nullptr;          // input index -> tree position
+    int *leaf_of = nullptr;      // leaf node of each tree position
+    int *nb = nullptr;           // [k][n] neighbours: tree positions (or input indices)
+    int *state = nullptr;        // [0] unresolved, [1] cursor, [2] cursor (global stacks),
+                                 // [3] queued, [4] global-stack overflow, [5] passes
+    int *ovf = nullptr;
+    int *gstack = nullptr;
+    int cap_groups = 0;
+    int last_passes = 0;         // pass launches of the last solve
+    long long last_particle_passes = 0;
+};
+
+void gf_sph_free(GfSph& S);
+
+// Builds the gas tree and gathers velocities and energies into tree order.
+// pos, vel (n, 3), mass, u (n) are device arrays in input order (vel and u
+// may be null when only neighbours are wanted).
+int gf_sph_setup(GfSph& S, int n, const double* pos, const double* vel, const double* mass,
+                 const double* u, hipStream_t s);
+
+// Exact kq nearest neighbours of every gas particle (the particle itself
+// included) by radius iteration: a pass gathers, per particle, every gas
+// particle within its search radius into a max-heap of (d^2, input index);
+// particles with fewer than kq inside grow their radius and take another
+// pass.  r_init (input order, may be null) is the first radius; without it
+// the radius comes from the density of the smallest tree cell holding >= kq
+// particles.  tree_pos_out: neighbours as tree positions (else input
+// indices); sorted: each row ascending in (d^2, index).
+int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool sorted,
+               int max_passes, hipStream_t s);
+
+// h = max(d_k / 2, eps / 2) (galaxy_formation/simulation.py:84) from the
+// k-th neighbour distance of the last solve.
+int gf_sph_smoothing(GfSph& S, int k, double eps, hipStream_t s);
+
+// Density and forces (galaxy_formation/sph.py:57-106) over the tree-order
+// neighbour table of the last solve (k columns).  Outputs are written in
+// input order: rho, dudt, dt, divv (n) and acc (n, 3); any may be null.
+int gf_sph_forces_tree(GfSph& S, int k, double gamma, double alpha, double beta, double* rho,
+                       double* acc, double* dudt, double* dt, double* divv, double* h_out,
+                       hipStream_t s);
+
+// Density and forces over a caller's neighbour table nb (n, k) int64 of
+// input indices, all arrays device, input order (sph.py:57-106 exactly).
+int gf_sph_forces_table(int n, int k, const double* pos, const double* vel, const double* mass,
+                        const int64_t* nb, const double* h, const double* u, double gamma,
+                        double alpha, double beta, double* rho, double* acc, double* dudt,
+                        double* dt, double* divv, int* scratch_nb, hipStream_t s);
+
+// Bytes of device scratch gf_sph_forces_table needs.
+size_t gf_sph_table_scratch_bytes(int n, int k);
+
+// Writes the neighbour table of the last solve as (n, k) int64 input indices
+// and (n, k) distances, rows in input order (requires a sorted solve).
+int gf_sph_export_knn(GfSph& S, int kq, int64_t* nb, double* dist, hipStream_t s);
+
+// knn_from_candidates (sph.py:109-135): per row, the k nearest of the m
+// candidates (-1 = none), ascending (d^2, index).  *ok = 1 when every row had
+// k valid candidates.  Device arrays.
+int gf_knn_from_candidates(int n, int m, const double* pos, const int64_t* cand, int k,
+                           int64_t* nb, double* dist, int* ok, hipStream_t s);
+
+// W(r, h) and (1/r) dW/dr (sph.py:31-54) for n (r, h) pairs.
+int gf_sph_kernel_eval(int n, const double* r, const double* h, double* w, double* g,
+                       hipStream_t s);

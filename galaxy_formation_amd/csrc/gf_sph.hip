@@ -1,0 +1,794 @@
+// SPH gas physics on the device: exact kNN, smoothing lengths, density and
+// pressure/viscosity forces (galaxy_formation/sph.py, simulation.py:74-95).
+//
+// Neighbour solve.  The reference takes the k nearest gas particles of each
+// gas particle (itself included) from a k-d tree (sph.py:165-166) and sets
+// h = max(d_k / 2, eps / 2) (simulation.py:84).  Here the gas particles get an
+// octree of their own (the gravity builder); one wave64 serves one group of
+// <= 64 gas particles.  Each lane owns a search radius r_i and a max-heap in
+// LDS of the kq best (d^2, input index) seen so far.  The wave walks the tree
+// once for the sphere around the group that holds every member's search
+// sphere, streams the particles of the leaves it touches through LDS, and each
+// lane inserts those within r_i.  A lane that saw >= kq particles inside r_i
+// holds the exact kq nearest; the others grow r_i and take another pass.
+// Distances are FP64 with the operations of the oracle (no contraction), so
+// neighbour sets and d_k are bit-identical to the brute-force oracle.
+//
+// Density and forces.  One lane per gas particle in Morton order, the
+// neighbour table stored column-major ([k][n]) so a wave reads 64 adjacent
+// particles' j-th neighbours together; neighbour data are mostly L2 hits.
+#include "gf_prim.h"
+#include "gf_sph.h"
+
+namespace {
+
+constexpr int TB = 256;
+constexpr int CH = 128;                                 // candidate chunk (LDS)
+constexpr int KSCAP = 1024;                             // LDS stack entries
+constexpr int KGSCAP = 512 * (GF_KEY_LEVELS + 2);       // global stack entries per block
+constexpr int KNN_BLOCKS = 4096;
+constexpr int KNN_GST_BLOCKS = 128;
+
+struct KnnArgs {
+    const int* start;
+    const int* count;
+    const int8_t* leaf;
+    const int* child8;
+    const double4* box;
+    const int* groups;
+    const int* counters;
+    const double* tx;
+    const double* ty;
+    const double* tz;
+    const uint32_t* idx;
+    const int* inv;
+    double* r;
+    double* dk;
+    int8_t* done;
+    int* passes;
+    int* nb;
+    int n, kq;
+    int tree_pos_out, sorted;
+    int* state;
+    int* ovf;
+    int* gstack;
+};
+
+__device__ __forceinline__ bool key_less(double da, int ia, double db, int ib) {
+    return da < db || (da == db && ia < ib);
+}
+
+template <int KQM, bool GST>
+__global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
+#pragma clang fp contract(off)
+    __shared__ double hd[KQM * 64];
+    __shared__ int hi[KQM * 64];
+    __shared__ double cx[CH], cy[CH], cz[CH];
+    __shared__ int ci[CH];
+    __shared__ int s_stack[GST ? 1 : KSCAP];
+    const int lane = __lane_id();
+    int* stack = GST ? a.gstack + size_t(blockIdx.x) * KGSCAP : s_stack;
+    const int scap = GST ? KGSCAP : KSCAP;
+    const int ngroups = GST ? a.state[3] : a.counters[1];
+    int* cursor = a.state + (GST ? 2 : 1);
+    const int kq = a.kq;
+
+    while (true) {
+        int gi = 0;
+        if (lane == 0) gi = atomicAdd(cursor, 1);
+        gi = __shfl(gi, 0, 64);
+        if (gi >= ngroups) break;
+        const int G = a.groups[GST ? a.ovf[gi] : gi];
+        const int b0 = a.start[G], np = a.count[G];
+        for (int c0 = 0; c0 < np; c0 += 64) {
+            const int t = b0 + c0 + lane;
+            const bool in = c0 + lane < np;
+            const bool mine = in && !a.done[t];
+            if (__ballot(mine) == 0) continue;
+            double xi = 0, yi = 0, zi = 0, ri = 0;
+            if (in) {
+                xi = a.tx[t]; yi = a.ty[t]; zi = a.tz[t];
+            }
+            if (mine) ri = a.r[t];
+            const double r2i = ri * ri;
+            // sphere (b, R) around every unresolved member's search sphere
+            double x0 = mine ? xi : 1e300, y0 = mine ? yi : 1e300, z0 = mine ? zi : 1e300;
+            double x1 = mine ? xi : -1e300, y1 = mine ? yi : -1e300, z1 = mine ? zi : -1e300;
+            x0 = gf_wave_min_d(x0); y0 = gf_wave_min_d(y0); z0 = gf_wave_min_d(z0);
+            x1 = gf_wave_max_d(x1); y1 = gf_wave_max_d(y1); z1 = gf_wave_max_d(z1);
+            const double bx = 0.5 * (x0 + x1), by = 0.5 * (y0 + y1), bz = 0.5 * (z0 + z1);
+            double reach = 0;
+            if (mine) {
+                const double dx = xi - bx, dy = yi - by, dz = zi - bz;
+                reach = sqrt(dx * dx + dy * dy + dz * dz) + ri;
+            }
+            const double R = gf_wave_max_d(reach) * (1.0 + 1e-12) + 1e-300;
+            const double R2 = R * R;
+
+            int hsize = 0, cnt = 0;
+            int sn = 1, cn = 0;
+            bool overflow = false;
+            if (lane == 0) stack[0] = 0;
+            __syncthreads();
+
+            // candidates in the chunk enter the heaps of the lanes whose
+            // search sphere holds them
+            auto flush = [&]() {
+                __syncthreads();
+                if (mine) {
+                    for (int c = 0; c < cn; ++c) {
+                        const double dx = xi - cx[c], dy = yi - cy[c], dz = zi - cz[c];
+                        const double d2 = dx * dx + dy * dy + dz * dz;
+                        if (d2 > r2i) continue;
+                        ++cnt;
+                        const int id = ci[c];
+                        if (hsize < kq) {
+                            int p = hsize++;
+                            while (p > 0) {
+                                const int q = (p - 1) >> 1;
+                                const double qd = hd[q * 64 + lane];
+                                const int qi = hi[q * 64 + lane];
+                                if (!key_less(qd, qi, d2, id)) break;
+                                hd[p * 64 + lane] = qd;
+                                hi[p * 64 + lane] = qi;
+                                p = q;
+                            }
+                            hd[p * 64 + lane] = d2;
+                            hi[p * 64 + lane] = id;
+                        } else if (key_less(d2, id, hd[lane], hi[lane])) {
+                            int p = 0;
+                            while (true) {
+                                int q = 2 * p + 1;
+                                if (q >= kq) break;
+                                double qd = hd[q * 64 + lane];
+                                int qi = hi[q * 64 + lane];
+                                if (q + 1 < kq) {
+                                    const double rd = hd[(q + 1) * 64 + lane];
+                                    const int rr = hi[(q + 1) * 64 + lane];
+                                    if (key_less(qd, qi, rd, rr)) {
+                                        ++q;
+                                        qd = rd;
+                                        qi = rr;
+                                    }
+                                }
+                                if (!key_less(d2, id, qd, qi)) break;
+                                hd[p * 64 + lane] = qd;
+                                hi[p * 64 + lane] = qi;
+                                p = q;
+                            }
+                            hd[p * 64 + lane] = d2;
+                            hi[p * 64 + lane] = id;
+                        }
+                    }
+                }
+                cn = 0;
+                __syncthreads();
+            };
+
+            while (sn > 0) {
+                const int take = min(64, sn);
+                sn -= take;
+                const int node = lane < take ? stack[sn + lane] : -1;
+                __syncthreads();
+                int push = 0, nent = 0, first = 0;
+                int4 c_lo = make_int4(-1, -1, -1, -1), c_hi = c_lo;
+                if (node >= 0) {
+                    const double4 bb = a.box[node];
+                    const double ex = fmax(0.0, fabs(bx - bb.x) - bb.w);
+                    const double ey = fmax(0.0, fabs(by - bb.y) - bb.w);
+                    const double ez = fmax(0.0, fabs(bz - bb.z) - bb.w);
+                    if (ex * ex + ey * ey + ez * ez <= R2) {
+                        if (a.leaf[node]) {
+                            first = a.start[node];
+                            nent = a.count[node];
+                        } else {
+                            const int4* cs = reinterpret_cast<const int4*>(a.child8 + 8 * size_t(node));
+                            c_lo = cs[0];
+                            c_hi = cs[1];
+                            push = (c_lo.x >= 0) + (c_lo.y >= 0) + (c_lo.z >= 0) + (c_lo.w >= 0) +
+                                   (c_hi.x >= 0) + (c_hi.y >= 0) + (c_hi.z >= 0) + (c_hi.w >= 0);
+                        }
+                    }
+                }
+                const int pinc = gf_wave_incl_scan(push);
+                const int ptot = __shfl(pinc, 63, 64);
+                if (sn + ptot > scap) {
+                    overflow = true;
+                    break;
+                }
+                {
+                    int o = sn + pinc - push;
+                    const int cc[8] = {c_lo.x, c_lo.y, c_lo.z, c_lo.w, c_hi.x, c_hi.y, c_hi.z, c_hi.w};
+#pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                        if (cc[k] >= 0) stack[o++] = cc[k];
+                }
+                sn += ptot;
+                int rem = nent, src = 0;
+                while (true) {
+                    const int einc = gf_wave_incl_scan(rem);
+                    const int etot = __shfl(einc, 63, 64);
+                    if (etot == 0) break;
+                    if (cn == CH) flush();
+                    const int space = CH - cn;
+                    const int my0 = einc - rem;
+                    const int canw = max(0, min(rem, space - my0));
+                    for (int k = 0; k < canw; ++k) {
+                        const int slot = cn + my0 + k;
+                        const int j = first + src + k;
+                        cx[slot] = a.tx[j];
+                        cy[slot] = a.ty[j];
+                        cz[slot] = a.tz[j];
+                        ci[slot] = int(a.idx[j]);
+                    }
+                    rem -= canw;
+                    src += canw;
+                    cn += min(etot, space);
+                }
+                __syncthreads();
+            }
+            if (overflow) {
+                if (!GST) {
+                    if (lane == 0) a.ovf[atomicAdd(&a.state[3], 1)] = gi;
+                } else if (lane == 0) {
+                    atomicAdd(&a.state[4], 1);
+                }
+                __syncthreads();
+                break;
+            }
+            if (cn > 0) flush();
+
+            if (mine) {
+                a.passes[t] += 1;
+                if (cnt >= kq) {
+                    if (a.sorted) {
+                        // heapsort in place: ascending (d^2, index)
+                        for (int end = kq - 1; end > 0; --end) {
+                            const double td = hd[end * 64 + lane];
+                            const int ti = hi[end * 64 + lane];
+                            hd[end * 64 + lane] = hd[lane];
+                            hi[end * 64 + lane] = hi[lane];
+                            int p = 0;
+                            while (true) {
+                                int q = 2 * p + 1;
+                                if (q >= end) break;
+                                double qd = hd[q * 64 + lane];
+                                int qi = hi[q * 64 + lane];
+                                if (q + 1 < end) {
+                                    const double rd = hd[(q + 1) * 64 + lane];
+                                    const int rr = hi[(q + 1) * 64 + lane];
+                                    if (key_less(qd, qi, rd, rr)) {
+                                        ++q;
+                                        qd = rd;
+                                        qi = rr;
+                                    }
+                                }
+                                if (!key_less(td, ti, qd, qi)) break;
+                                hd[p * 64 + lane] = qd;
+                                hi[p * 64 + lane] = qi;
+                                p = q;
+                            }
+                            hd[p * 64 + lane] = td;
+                            hi[p * 64 + lane] = ti;
+                        }
+                        a.dk[t] = sqrt(hd[(kq - 1) * 64 + lane]);
+                    } else {
+                        a.dk[t] = sqrt(hd[lane]);
+                    }
+                    for (int s = 0; s < kq; ++s) {
+                        const int id = hi[s * 64 + lane];
+                        a.nb[size_t(s) * a.n + t] = a.tree_pos_out ? a.inv[id] : id;
+                    }
+                    a.done[t] = 1;
+                } else {
+                    // plain iteration: grow the sphere towards kq particles
+                    const double want = double(kq) / double(max(cnt, 1));
+                    const double f = fmin(2.0, fmax(1.2, 1.1 * cbrt(want)));
+                    a.r[t] = ri * f;
+                    atomicAdd(&a.state[0], 1);
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void k_knn_reset(int* state) {
+    if (threadIdx.x == 0) {
+        state[0] = 0;
+        state[1] = 0;
+        state[2] = 0;
+        state[3] = 0;
+        state[4] = 0;
+    }
+}
+
+// First search radius: the caller's, or from the number density of the
+// smallest tree cell around the particle that holds >= kq particles.
+__global__ __launch_bounds__(TB) void k_knn_init(int n, int kq, const double* __restrict__ r_init,
+                                                 const uint32_t* __restrict__ idx,
+                                                 const int* __restrict__ leaf_of,
+                                                 const int* __restrict__ parent,
+                                                 const int* __restrict__ count,
+                                                 const double4* __restrict__ box,
+                                                 double* __restrict__ r, int8_t* __restrict__ done,
+                                                 int* __restrict__ passes) {
+    const int t = blockIdx.x * TB + threadIdx.x;
+    if (t >= n) return;
+    double rr;
+    if (r_init) {
+        rr = r_init[idx[t]];
+    } else {
+        int node = leaf_of[t];
+        while (count[node] < kq && parent[node] >= 0) node = parent[node];
+        const double side = 2.0 * box[node].w;
+        const double dens = double(count[node]) / (side * side * side);
+        rr = 1.1 * cbrt(3.0 * kq / (4.0 * M_PI * dens));
+    }
+    r[t] = rr;
+    done[t] = 0;
+    passes[t] = 0;
+}
+
+__global__ __launch_bounds__(TB) void k_leaf_of(const int* __restrict__ counters,
+                                                const int* __restrict__ start,
+                                                const int* __restrict__ count,
+                                                const int8_t* __restrict__ leaf,
+                                                int* __restrict__ leaf_of) {
+    const int nn = counters[0];
+    for (int id = blockIdx.x * TB + threadIdx.x; id < nn; id += gridDim.x * TB) {
+        if (!leaf[id]) continue;
+        for (int i = start[id], e = start[id] + count[id]; i < e; ++i) leaf_of[i] = id;
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_sph_gather(int n, const uint32_t* __restrict__ idx,
+                                                   const double* __restrict__ vel,
+                                                   const double* __restrict__ u,
+                                                   double* __restrict__ vx, double* __restrict__ vy,
+                                                   double* __restrict__ vz, double* __restrict__ uu,
+                                                   int* __restrict__ inv) {
+    const int t = blockIdx.x * TB + threadIdx.x;
+    if (t >= n) return;
+    const int j = int(idx[t]);
+    inv[j] = t;
+    if (vel) {
+        vx[t] = vel[3 * j];
+        vy[t] = vel[3 * j + 1];
+        vz[t] = vel[3 * j + 2];
+    }
+    if (u) uu[t] = u[j];
+}
+
+// -------------------------------------------------------- density / forces
+// sph.py:31-40 and :43-54 with the per-particle factors 1/h, 1/(pi h^3) and
+// 1/(pi h^4) computed once per particle.
+__device__ __forceinline__ double kernel_w(double r, double ihv, double s3) {
+#pragma clang fp contract(off)
+    const double q = r * ihv;
+    if (q < 1.0) return s3 * (1 - 1.5 * q * q + 0.75 * (q * q * q));
+    if (q < 2.0) {
+        const double t = 2 - q;
+        return s3 * 0.25 * (t * t * t);
+    }
+    return 0.0;
+}
+
+__device__ __forceinline__ double kernel_g(double r, double ihv, double s4) {
+#pragma clang fp contract(off)
+    const double q = r * ihv;
+    double g;
+    if (q < 1.0) {
+        g = s4 * (-3 * q + 2.25 * q * q);
+    } else if (q < 2.0) {
+        const double t = 2 - q;
+        g = -s4 * 0.75 * (t * t);
+    } else {
+        return 0.0;
+    }
+    return g / fmax(r, 1e-30);
+}
+
+struct SphView {
+    const double *x, *y, *z, *vx, *vy, *vz, *m, *u, *h;
+    int n;
+};
+
+__global__ __launch_bounds__(TB) void k_h_factors(int n, const double* __restrict__ h,
+                                                  double* __restrict__ ih, double* __restrict__ sg) {
+    const int t = blockIdx.x * TB + threadIdx.x;
+    if (t >= n) return;
+    const double hv = h[t];
+    ih[t] = 1.0 / hv;
+    sg[t] = 1.0 / (M_PI * (hv * hv * hv * hv));
+}
+
+__global__ __launch_bounds__(TB) void k_density(SphView v, const int* __restrict__ nb, int k,
+                                                const double* __restrict__ ih,
+                                                double* __restrict__ rho) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * TB + threadIdx.x;
+    if (t >= v.n) return;
+    const double x = v.x[t], y = v.y[t], z = v.z[t], hv = v.h[t];
+    const double ihv = ih[t], s3 = 1.0 / (M_PI * (hv * hv * hv));
+    double s = 0.0;
+    for (int jj = 0; jj < k; ++jj) {
+        const int j = nb[size_t(jj) * v.n + t];
+        const double dx = x - v.x[j], dy = y - v.y[j], dz = z - v.z[j];
+        s += v.m[j] * kernel_w(sqrt(dx * dx + dy * dy + dz * dz), ihv, s3);
+    }
+    rho[t] = s;
+}
+
+__global__ __launch_bounds__(TB) void k_force(SphView v, const int* __restrict__ nb, int k,
+                                              const double* __restrict__ rho,
+                                              const double* __restrict__ ih,
+                                              const double* __restrict__ sg, double gamma,
+                                              double alpha, double beta,
+                                              const uint32_t* __restrict__ out_idx,
+                                              double* __restrict__ acc, double* __restrict__ dudt,
+                                              double* __restrict__ dt, double* __restrict__ divv,
+                                              double* __restrict__ rho_out,
+                                              double* __restrict__ h_out) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= v.n) return;
+    const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
+    const double vxi = v.vx[i], vyi = v.vy[i], vzi = v.vz[i];
+    const double rhoi = rho[i], ui = v.u[i], hi_ = v.h[i];
+    const double ihi = ih[i], sgi = sg[i];
+    const double Pi = (gamma - 1) * rhoi * ui;
+    const double ci = sqrt(gamma * (gamma - 1) * ui);
+    const double pi2 = Pi / (rhoi * rhoi);
+    double ax = 0, ay = 0, az = 0, du = 0, mu_max = 0, dv = 0;
+    for (int jj = 0; jj < k; ++jj) {
+        const int j = nb[size_t(jj) * v.n + i];
+        if (j == i) continue;
+        const double dx = xi - v.x[j], dy = yi - v.y[j], dz = zi - v.z[j];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        const double gi = kernel_g(r, ihi, sgi);
+        const double gj = kernel_g(r, ih[j], sg[j]);
+        const double rhoj = rho[j], uj = v.u[j], hj = v.h[j];
+        const double Pj = (gamma - 1) * rhoj * uj;
+        const double cj = sqrt(gamma * (gamma - 1) * uj);
+        const double vr = (vxi - v.vx[j]) * dx + (vyi - v.vy[j]) * dy + (vzi - v.vz[j]) * dz;
+        double visc = 0.0;
+        if (vr < 0) {
+            const double hb = 0.5 * (hi_ + hj);
+            const double mu = hb * vr / (r * r + 0.01 * hb * hb);
+            mu_max = fmax(mu_max, -mu);
+            visc = (-alpha * 0.5 * (ci + cj) * mu + beta * mu * mu) / (0.5 * (rhoi + rhoj));
+        }
+        const double gb = 0.5 * (gi + gj);
+        const double f = pi2 * gi + Pj / (rhoj * rhoj) * gj + visc * gb;
+        const double mj = v.m[j];
+        ax -= mj * f * dx;
+        ay -= mj * f * dy;
+        az -= mj * f * dz;
+        du += mj * (pi2 * gi + 0.5 * visc * gb) * vr;
+        dv += mj * vr * gi;
+    }
+    const size_t o = out_idx ? size_t(out_idx[i]) : size_t(i);
+    if (acc) {
+        acc[3 * o] = ax;
+        acc[3 * o + 1] = ay;
+        acc[3 * o + 2] = az;
+    }
+    if (dudt) dudt[o] = du;
+    if (dt) dt[o] = 0.3 * hi_ / (ci + 1.2 * (alpha * ci + beta * mu_max) + 1e-30);
+    if (divv) divv[o] = -dv / rhoi;
+    if (rho_out) rho_out[o] = rhoi;
+    if (h_out) h_out[o] = hi_;
+}
+
+__global__ __launch_bounds__(TB) void k_smoothing(int n, const double* __restrict__ dk, double eps,
+                                                  double* __restrict__ h) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * TB + threadIdx.x;
+    if (t >= n) return;
+    h[t] = fmax(0.5 * dk[t], 0.5 * eps);
+}
+
+// caller's table: (n, k) int64 -> [k][n] int32, SoA copies of pos / vel
+__global__ __launch_bounds__(TB) void k_table_prep(int n, int k, const int64_t* __restrict__ nb,
+                                                   const double* __restrict__ pos,
+                                                   const double* __restrict__ vel,
+                                                   int* __restrict__ nbt, double* __restrict__ soa) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    for (int jj = 0; jj < k; ++jj) nbt[size_t(jj) * n + i] = int(nb[size_t(i) * k + jj]);
+    for (int c = 0; c < 3; ++c) {
+        soa[size_t(c) * n + i] = pos[3 * size_t(i) + c];
+        soa[size_t(3 + c) * n + i] = vel[3 * size_t(i) + c];
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_export_knn(int n, int kq, const int* __restrict__ nb,
+                                                   const double* __restrict__ x,
+                                                   const double* __restrict__ y,
+                                                   const double* __restrict__ z,
+                                                   const uint32_t* __restrict__ idx,
+                                                   const int* __restrict__ inv,
+                                                   int64_t* __restrict__ onb,
+                                                   double* __restrict__ odist) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * TB + threadIdx.x;
+    if (t >= n) return;
+    const size_t row = size_t(idx[t]) * kq;
+    for (int s = 0; s < kq; ++s) {
+        const int id = nb[size_t(s) * n + t];
+        onb[row + s] = id;
+        const int j = inv[id];
+        const double dx = x[t] - x[j], dy = y[t] - y[j], dz = z[t] - z[j];
+        odist[row + s] = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+}
+
+// sph.py:109-135: k nearest of each row's candidates, ascending (d^2, index).
+constexpr int CAND_MAX = 128;
+__global__ __launch_bounds__(64) void k_knn_cand(int n, int m, const double* __restrict__ pos,
+                                                 const int64_t* __restrict__ cand, int k,
+                                                 int64_t* __restrict__ nb, double* __restrict__ dist,
+                                                 int* __restrict__ ok) {
+#pragma clang fp contract(off)
+    __shared__ double d2s[CAND_MAX * 64];
+    __shared__ int ids[CAND_MAX * 64];
+    const int lane = threadIdx.x;
+    const int i = blockIdx.x * 64 + lane;
+    if (i >= n) return;
+    const double x = pos[3 * size_t(i)], y = pos[3 * size_t(i) + 1], z = pos[3 * size_t(i) + 2];
+    int cnt = 0;
+    for (int jj = 0; jj < m; ++jj) {
+        const int64_t j = cand[size_t(i) * m + jj];
+        if (j < 0) continue;
+        const double dx = x - pos[3 * j], dy = y - pos[3 * j + 1], dz = z - pos[3 * j + 2];
+        d2s[cnt * 64 + lane] = dx * dx + dy * dy + dz * dz;
+        ids[cnt * 64 + lane] = int(j);
+        ++cnt;
+    }
+    if (cnt < k) {
+        atomicAnd(ok, 0);
+        return;
+    }
+    for (int a0 = 0; a0 < cnt; ++a0) {
+        const double da = d2s[a0 * 64 + lane];
+        const int ia = ids[a0 * 64 + lane];
+        // equal keys (a candidate listed twice) keep their column order, as
+        // a stable sort would
+        int rank = 0;
+        for (int b = 0; b < cnt; ++b) {
+            const double db = d2s[b * 64 + lane];
+            const int ib = ids[b * 64 + lane];
+            rank += key_less(db, ib, da, ia) || (b < a0 && db == da && ib == ia);
+        }
+        if (rank < k) {
+            nb[size_t(i) * k + rank] = ia;
+            dist[size_t(i) * k + rank] = sqrt(da);
+        }
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_kernel_eval(int n, const double* __restrict__ r,
+                                                    const double* __restrict__ h,
+                                                    double* __restrict__ w,
+                                                    double* __restrict__ g) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * TB + threadIdx.x;
+    if (t >= n) return;
+    const double hv = h[t], rv = r[t];
+    // the reference's own operation order (q = r / h), sph.py:34-35, :46-47
+    const double q = rv / hv;
+    if (w) {
+        const double s = 1.0 / (M_PI * (hv * hv * hv));
+        double out = 0.0;
+        if (q < 1.0) out = s * (1 - 1.5 * q * q + 0.75 * (q * q * q));
+        else if (q < 2.0) { const double tt = 2 - q; out = s * 0.25 * (tt * tt * tt); }
+        w[t] = out;
+    }
+    if (g) {
+        const double s = 1.0 / (M_PI * (hv * hv * hv * hv));
+        double gg = 0.0;
+        if (q < 1.0) gg = s * (-3 * q + 2.25 * q * q) / fmax(rv, 1e-30);
+        else if (q < 2.0) { const double tt = 2 - q; gg = -s * 0.75 * (tt * tt) / fmax(rv, 1e-30); }
+        g[t] = gg;
+    }
+}
+
+template <typename T>
+int dalloc(T*& p, size_t count) {
+    if (p) GF_HIP(hipFree(p));
+    p = nullptr;
+    GF_HIP(hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (count ? count : 1)));
+    return 0;
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int sph_reserve(GfSph& S, int n, int k) {
+    if (!S.state) {
+        if (dalloc(S.state, 16) || dalloc(S.gstack, size_t(KNN_GST_BLOCKS) * KGSCAP)) return -1;
+    }
+    if (n > S.cap_n) {
+        const int cn = std::max(n, 1024);
+        if (dalloc(S.vx, cn) || dalloc(S.vy, cn) || dalloc(S.vz, cn) || dalloc(S.u, cn) ||
+            dalloc(S.r, cn) || dalloc(S.dk, cn) || dalloc(S.h, cn) || dalloc(S.rho, cn) ||
+            dalloc(S.ih, cn) || dalloc(S.sg, cn) || dalloc(S.done, cn) || dalloc(S.passes, cn) ||
+            dalloc(S.inv, cn) || dalloc(S.leaf_of, cn))
+            return -1;
+        S.cap_n = cn;
+        S.cap_k = 0;
+    }
+    if (k > S.cap_k) {
+        if (dalloc(S.nb, size_t(S.cap_n) * size_t(k))) return -1;
+        S.cap_k = k;
+    }
+    const int ng = gf_tree_node_bound(S.cap_n);
+    if (ng > S.cap_groups) {
+        if (dalloc(S.ovf, ng)) return -1;
+        S.cap_groups = ng;
+    }
+    return 0;
+}
+
+}  // namespace
+
+void gf_sph_free(GfSph& S) {
+    gf_tree_free(S.tree);
+    dfree(S.vx); dfree(S.vy); dfree(S.vz); dfree(S.u); dfree(S.r); dfree(S.dk); dfree(S.h);
+    dfree(S.rho); dfree(S.ih); dfree(S.sg); dfree(S.done); dfree(S.passes); dfree(S.inv); dfree(S.leaf_of);
+    dfree(S.nb); dfree(S.state); dfree(S.ovf); dfree(S.gstack);
+    S = GfSph();
+}
+
+int gf_sph_setup(GfSph& S, int n, const double* pos, const double* vel, const double* mass,
+                 const double* u, hipStream_t s) {
+    if (sph_reserve(S, n, 0)) return -1;
+    if (gf_tree_build(S.tree, n, pos, mass, 0.5f, s, /*with_com=*/false)) return -1;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_sph_gather, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, S.tree.idx, vel, u,
+                       S.vx, S.vy, S.vz, S.u, S.inv);
+    GF_LAUNCH_CHECK();
+    return 0;
+}
+
+int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool sorted,
+               int max_passes, hipStream_t s) {
+    const int n = S.tree.n;
+    if (n == 0) return 0;
+    if (kq < 1 || kq > n) {
+        gf_set_error("kNN: k = %d must be in [1, n = %d]", kq, n);
+        return -1;
+    }
+    if (kq > 128) {
+        gf_set_error("kNN: k = %d exceeds 128", kq);
+        return -1;
+    }
+    if (sph_reserve(S, n, kq)) return -1;
+    GfTree& T = S.tree;
+    if (!r_init) {
+        // leaf of each particle, for the density estimate of the first radius
+        hipLaunchKernelGGL(k_leaf_of, dim3(std::min(2048, gf_div_up(T.cap_nodes, TB))), dim3(TB), 0,
+                           s, T.counters, T.start, T.count, T.leaf, S.leaf_of);
+    }
+    hipLaunchKernelGGL(k_knn_init, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, kq, r_init, T.idx,
+                       S.leaf_of, T.parent, T.count, T.box, S.r, S.done, S.passes);
+    GF_LAUNCH_CHECK();
+    KnnArgs a;
+    a.start = T.start; a.count = T.count; a.leaf = T.leaf; a.child8 = T.child8; a.box = T.box;
+    a.groups = T.groups; a.counters = T.counters; a.tx = T.tx; a.ty = T.ty; a.tz = T.tz;
+    a.idx = T.idx; a.inv = S.inv; a.r = S.r; a.dk = S.dk; a.done = S.done; a.passes = S.passes;
+    a.nb = S.nb; a.n = n; a.kq = kq; a.tree_pos_out = tree_pos_out; a.sorted = sorted;
+    a.state = S.state; a.ovf = S.ovf; a.gstack = S.gstack;
+    int pass = 0;
+    while (true) {
+        if (pass >= max_passes) {
+            gf_set_error("kNN: particles still unresolved after %d passes", max_passes);
+            return -1;
+        }
+        hipLaunchKernelGGL(k_knn_reset, dim3(1), dim3(64), 0, s, S.state);
+        if (kq <= 64) {
+            hipLaunchKernelGGL((k_knn_pass<64, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_knn_pass<64, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);
+        } else {
+            hipLaunchKernelGGL((k_knn_pass<128, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_knn_pass<128, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);
+        }
+        GF_LAUNCH_CHECK();
+        ++pass;
+        int st[5];
+        GF_HIP(hipMemcpyAsync(st, S.state, sizeof(st), hipMemcpyDeviceToHost, s));
+        GF_HIP(hipStreamSynchronize(s));
+        if (st[4] != 0) {
+            gf_set_error("kNN: %d groups overflowed the global stack", st[4]);
+            return -1;
+        }
+        if (st[0] == 0) break;
+    }
+    S.last_passes = pass;
+    return 0;
+}
+
+int gf_sph_smoothing(GfSph& S, int k, double eps, hipStream_t s) {
+    (void)k;
+    const int n = S.tree.n;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_smoothing, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, S.dk, eps, S.h);
+    hipLaunchKernelGGL(k_h_factors, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, S.h, S.ih, S.sg);
+    GF_LAUNCH_CHECK();
+    return 0;
+}
+
+int gf_sph_forces_tree(GfSph& S, int k, double gamma, double alpha, double beta, double* rho,
+                       double* acc, double* dudt, double* dt, double* divv, double* h_out,
+                       hipStream_t s) {
+    const int n = S.tree.n;
+    if (n == 0) return 0;
+    SphView v{S.tree.tx, S.tree.ty, S.tree.tz, S.vx, S.vy, S.vz, S.tree.tm, S.u, S.h, n};
+    const int g = gf_div_up(n, TB);
+    hipLaunchKernelGGL(k_density, dim3(g), dim3(TB), 0, s, v, S.nb, k, S.ih, S.rho);
+    hipLaunchKernelGGL(k_force, dim3(g), dim3(TB), 0, s, v, S.nb, k, S.rho, S.ih, S.sg, gamma,
+                       alpha, beta, S.tree.idx, acc, dudt, dt, divv, rho, h_out);
+    GF_LAUNCH_CHECK();
+    return 0;
+}
+
+int gf_sph_forces_table(int n, int k, const double* pos, const double* vel, const double* mass,
+                        const int64_t* nb, const double* h, const double* u, double gamma,
+                        double alpha, double beta, double* rho, double* acc, double* dudt,
+                        double* dt, double* divv, int* scratch, hipStream_t s) {
+    if (n == 0) return 0;
+    // scratch: [k][n] ints, then 6n doubles (SoA pos, vel), then 3n doubles
+    int* nbt = scratch;
+    double* soa = reinterpret_cast<double*>(scratch + ((size_t(k) * n + 1) & ~size_t(1)));
+    double* ih = soa + 6 * size_t(n);
+    double* sg = ih + n;
+    double* rr = sg + n;
+    const int g = gf_div_up(n, TB);
+    hipLaunchKernelGGL(k_table_prep, dim3(g), dim3(TB), 0, s, n, k, nb, pos, vel, nbt, soa);
+    hipLaunchKernelGGL(k_h_factors, dim3(g), dim3(TB), 0, s, n, h, ih, sg);
+    SphView v{soa, soa + n, soa + 2 * size_t(n), soa + 3 * size_t(n), soa + 4 * size_t(n),
+              soa + 5 * size_t(n), mass, u, h, n};
+    hipLaunchKernelGGL(k_density, dim3(g), dim3(TB), 0, s, v, nbt, k, ih, rr);
+    hipLaunchKernelGGL(k_force, dim3(g), dim3(TB), 0, s, v, nbt, k, rr, ih, sg, gamma, alpha, beta,
+                       nullptr, acc, dudt, dt, divv, rho, nullptr);
+    GF_LAUNCH_CHECK();
+    return 0;
+}
+
+size_t gf_sph_table_scratch_bytes(int n, int k) {
+    return sizeof(int) * (size_t(k) * n + 2) + sizeof(double) * 9 * size_t(n);
+}
+
+int gf_sph_export_knn(GfSph& S, int kq, int64_t* nb, double* dist, hipStream_t s) {
+    const int n = S.tree.n;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_export_knn, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, kq, S.nb,
+                       S.tree.tx, S.tree.ty, S.tree.tz, S.tree.idx, S.inv, nb, dist);
+    GF_LAUNCH_CHECK();
+    return 0;
+}
+
+int gf_knn_from_candidates(int n, int m, const double* pos, const int64_t* cand, int k,
+                           int64_t* nb, double* dist, int* ok, hipStream_t s) {
+    if (m > CAND_MAX) {
+        gf_set_error("knn_from_candidates: %d candidates per row exceed %d", m, CAND_MAX);
+        return -1;
+    }
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_knn_cand, dim3(gf_div_up(n, 64)), dim3(64), 0, s, n, m, pos, cand, k, nb,
+                       dist, ok);
+    GF_LAUNCH_CHECK();
+    return 0;
+}
+
+int gf_sph_kernel_eval(int n, const double* r, const double* h, double* w, double* g,
+                       hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_kernel_eval, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, r, h, w, g);
+    GF_LAUNCH_CHECK();
+    return 0;
+}

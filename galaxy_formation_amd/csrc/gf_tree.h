@@ -25,7 +25,8 @@ struct GfTree {
     int *start = nullptr, *count = nullptr, *next = nullptr, *parent = nullptr;
     int8_t *leaf = nullptr, *depth = nullptr;
     int *child8 = nullptr;                         // 8 slots per node, octant order, -1 = empty
-    int *arrive = nullptr;
+    int *dlist = nullptr;                          // node ids grouped by depth
+    int *dmeta = nullptr;                          // per depth: count, offset (+1), cursor
     double4 *com = nullptr;                        // centre of mass and mass (FP64)
     double *open_r = nullptr;                      // opening radius (FP64)
     double4 *box = nullptr;                        // geometric centre and half-width (FP64)
@@ -49,8 +50,9 @@ int gf_tree_reserve(GfTree& T, int n);
 void gf_tree_free(GfTree& T);
 
 // Builds the tree of pos (n, 3) / mass (n) (device, input order).
+// with_com = false skips the centres of mass (a tree used for neighbour search).
 int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float theta,
-                  hipStream_t s);
+                  hipStream_t s, bool with_com = true);
 // Refit (cpp/nbody.cpp:262-305): same cells, new masses, centres of mass and
 // opening radii enlarged by the largest displacement since the build.
 int gf_tree_refit(GfTree& T, const double* pos, const double* mass, hipStream_t s);

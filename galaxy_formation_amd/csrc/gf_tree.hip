@@ -16,8 +16,8 @@
 // of nodes that start at each position, their DFS indices (an exclusive
 // scan: preorder is the order of (start, depth)) and `next` (the DFS index
 // of the first node starting at the cell's end) are all computed in parallel.
-// Centres of mass are summed bottom-up in FP64 by "last child to arrive
-// finishes the parent", children in octant order.
+// Centres of mass are summed bottom-up in FP64, one launch per depth,
+// children in octant order.
 #include "gf_prim.h"
 #include "gf_tree.h"
 
@@ -56,14 +56,27 @@ __global__ __launch_bounds__(TB) void k_bbox_partial(const double* __restrict__ 
     }
 }
 
-// Root box exactly as nbody.cpp:218-219.
-__global__ void k_bbox_final(const double* __restrict__ part, int nb, double* __restrict__ root) {
+// Root box exactly as nbody.cpp:218-219 (one block reduces the partials).
+__global__ __launch_bounds__(TB) void k_bbox_final(const double* __restrict__ part, int nb,
+                                                   double* __restrict__ root) {
 #pragma clang fp contract(off)
-    if (threadIdx.x != 0) return;
+    __shared__ double lds[4][6];
     double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
-    for (int b = 0; b < nb; ++b) {
+    for (int b = threadIdx.x; b < nb; b += TB) {
         for (int k = 0; k < 3; ++k) v[k] = fmin(v[k], part[6 * b + k]);
         for (int k = 3; k < 6; ++k) v[k] = fmax(v[k], part[6 * b + k]);
+    }
+    const int w = threadIdx.x >> 6;
+    for (int k = 0; k < 3; ++k) v[k] = gf_wave_min_d(v[k]);
+    for (int k = 3; k < 6; ++k) v[k] = gf_wave_max_d(v[k]);
+    if (__lane_id() == 0)
+        for (int k = 0; k < 6; ++k) lds[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 6; ++k) {
+        double r = lds[0][k];
+        for (int ww = 1; ww < 4; ++ww) r = k < 3 ? fmin(r, lds[ww][k]) : fmax(r, lds[ww][k]);
+        v[k] = r;
     }
     root[0] = 0.5 * (v[0] + v[3]);
     root[1] = 0.5 * (v[1] + v[4]);
@@ -151,14 +164,14 @@ __global__ void k_copy_count(const int* __restrict__ base, int n, int* __restric
 
 __global__ __launch_bounds__(TB) void k_node_init(const int* __restrict__ counters,
                                                   int* __restrict__ child8,
-                                                  int* __restrict__ arrive) {
+                                                  int* __restrict__ dmeta) {
     const int nn = counters[0];
     for (int i = blockIdx.x * TB + threadIdx.x; i < nn; i += gridDim.x * TB) {
         int4* c = reinterpret_cast<int4*>(child8 + 8 * size_t(i));
         c[0] = make_int4(-1, -1, -1, -1);
         c[1] = make_int4(-1, -1, -1, -1);
-        arrive[i] = 0;
     }
+    if (blockIdx.x == 0 && threadIdx.x < 32) dmeta[threadIdx.x] = 0;
 }
 
 // Writes the nodes that start at position j: range, depth, leaf flag, next,
@@ -297,55 +310,85 @@ __device__ __forceinline__ void finalize_node(int id, double M, double sx, doubl
     ndisp[id] = dm;
 }
 
-__global__ __launch_bounds__(TB) void k_com_climb(
-    const int* __restrict__ counters, const int* __restrict__ start, const int* __restrict__ count,
-    const int* __restrict__ parent, const int8_t* __restrict__ leaf,
-    const int* __restrict__ child8, int* __restrict__ arrive, const double4* __restrict__ box,
-    const double* __restrict__ tx, const double* __restrict__ ty, const double* __restrict__ tz,
-    const double* __restrict__ tm, const double* __restrict__ tdisp, double4* com,
-    double* open_r, double* ndisp, double theta_d) {
-#pragma clang fp contract(off)
+// Nodes listed by depth (deepest level first is the bottom-up order).
+__global__ __launch_bounds__(TB) void k_depth_count(const int* __restrict__ counters,
+                                                    const int8_t* __restrict__ depth,
+                                                    int* __restrict__ dcount) {
+    __shared__ int h[GF_KEY_LEVELS + 1];
+    if (threadIdx.x <= GF_KEY_LEVELS) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int nn = counters[0];
+    for (int id = blockIdx.x * TB + threadIdx.x; id < nn; id += gridDim.x * TB)
+        atomicAdd(&h[depth[id]], 1);
+    __syncthreads();
+    if (threadIdx.x <= GF_KEY_LEVELS && h[threadIdx.x]) atomicAdd(&dcount[threadIdx.x], h[threadIdx.x]);
+}
+
+// dcount[0..L] -> doff[0..L+1] (exclusive); cursors reset.
+__global__ void k_depth_offsets(int* __restrict__ dcount, int* __restrict__ doff,
+                                int* __restrict__ dcur) {
+    if (threadIdx.x != 0) return;
+    int run = 0;
+    for (int d = 0; d <= GF_KEY_LEVELS; ++d) {
+        doff[d] = run;
+        run += dcount[d];
+        dcur[d] = 0;
+    }
+    doff[GF_KEY_LEVELS + 1] = run;
+}
+
+__global__ __launch_bounds__(TB) void k_depth_scatter(const int* __restrict__ counters,
+                                                      const int8_t* __restrict__ depth,
+                                                      const int* __restrict__ doff,
+                                                      int* __restrict__ dcur,
+                                                      int* __restrict__ dlist) {
     const int nn = counters[0];
     for (int id = blockIdx.x * TB + threadIdx.x; id < nn; id += gridDim.x * TB) {
-        if (!leaf[id]) continue;
+        const int d = depth[id];
+        dlist[doff[d] + atomicAdd(&dcur[d], 1)] = id;
+    }
+}
+
+// Centres of mass, opening radii and largest displacements of the nodes of
+// one depth; the deeper levels are done.  Leaves sum their particles in tree
+// order (nbody.cpp:124-131), internal nodes their children in octant order
+// (nbody.cpp:136-147), both in FP64.
+__global__ __launch_bounds__(TB) void k_level_com(
+    int d, const int* __restrict__ doff, const int* __restrict__ dlist,
+    const int* __restrict__ start, const int* __restrict__ count, const int8_t* __restrict__ leaf,
+    const int* __restrict__ child8, const double4* __restrict__ box,
+    const double* __restrict__ tx, const double* __restrict__ ty, const double* __restrict__ tz,
+    const double* __restrict__ tm, const double* __restrict__ tdisp, double4* __restrict__ com,
+    double* __restrict__ open_r, double* __restrict__ ndisp, double theta_d) {
+#pragma clang fp contract(off)
+    const int b0 = doff[d], b1 = doff[d + 1];
+    for (int k = b0 + blockIdx.x * TB + threadIdx.x; k < b1; k += gridDim.x * TB) {
+        const int id = dlist[k];
         double M = 0, sx = 0, sy = 0, sz = 0, dm = 0;
-        const int b = start[id], e = b + count[id];
-        for (int i = b; i < e; ++i) {
-            const double m = tm[i];
-            M += m;
-            sx += m * tx[i];
-            sy += m * ty[i];
-            sz += m * tz[i];
-            dm = fmax(dm, tdisp[i]);
-        }
-        finalize_node(id, M, sx, sy, sz, dm, theta_d, box, com, open_r, ndisp);
-        int cur = id;
-        while (true) {
-            const int p = parent[cur];
-            if (p < 0) break;
-            const int* slots = child8 + 8 * size_t(p);
-            int nch = 0;
-            for (int o = 0; o < 8; ++o) nch += slots[o] >= 0;
-            __threadfence();
-            const int old = atomicAdd(&arrive[p], 1);
-            if (old != nch - 1) break;
-            __threadfence();
-            const volatile double* vc = reinterpret_cast<const volatile double*>(com);
-            const volatile double* vd = ndisp;
-            M = 0; sx = 0; sy = 0; sz = 0; dm = 0;
+        if (leaf[id]) {
+            const int b = start[id], e = b + count[id];
+            for (int i = b; i < e; ++i) {
+                const double m = tm[i];
+                M += m;
+                sx += m * tx[i];
+                sy += m * ty[i];
+                sz += m * tz[i];
+                dm = fmax(dm, tdisp[i]);
+            }
+        } else {
+            const int* slots = child8 + 8 * size_t(id);
             for (int o = 0; o < 8; ++o) {
                 const int c = slots[o];
                 if (c < 0) continue;
-                const double cm = vc[4 * size_t(c) + 3];
-                M += cm;
-                sx += cm * vc[4 * size_t(c)];
-                sy += cm * vc[4 * size_t(c) + 1];
-                sz += cm * vc[4 * size_t(c) + 2];
-                dm = fmax(dm, vd[c]);
+                const double4 cm = com[c];
+                M += cm.w;
+                sx += cm.w * cm.x;
+                sy += cm.w * cm.y;
+                sz += cm.w * cm.z;
+                dm = fmax(dm, ndisp[c]);
             }
-            finalize_node(p, M, sx, sy, sz, dm, theta_d, box, com, open_r, ndisp);
-            cur = p;
         }
+        finalize_node(id, M, sx, sy, sz, dm, theta_d, box, com, open_r, ndisp);
     }
 }
 
@@ -406,7 +449,9 @@ int gf_tree_node_bound(int n) {
 
 int gf_tree_reserve(GfTree& T, int n) {
     if (!T.counters) {
-        if (dalloc(T.counters, 4) || dalloc(T.root, 4) || dalloc(T.bbox_part, 6 * 1024)) return -1;
+        if (dalloc(T.counters, 4) || dalloc(T.root, 4) || dalloc(T.bbox_part, 6 * 1024) ||
+            dalloc(T.dmeta, 96))
+            return -1;
     }
     if (n > T.cap_n) {
         const int cn = std::max(n, 1024);
@@ -424,7 +469,7 @@ int gf_tree_reserve(GfTree& T, int n) {
     if (nb > T.cap_nodes) {
         if (dalloc(T.start, nb) || dalloc(T.count, nb) || dalloc(T.next, nb) ||
             dalloc(T.parent, nb) || dalloc(T.leaf, nb) || dalloc(T.depth, nb) ||
-            dalloc(T.child8, 8 * size_t(nb)) || dalloc(T.arrive, nb) || dalloc(T.com, nb) ||
+            dalloc(T.child8, 8 * size_t(nb)) || dalloc(T.dlist, nb) || dalloc(T.com, nb) ||
             dalloc(T.open_r, nb) || dalloc(T.box, nb) || dalloc(T.ndisp, nb) ||
             dalloc(T.gflag, size_t(nb) + 1) || dalloc(T.groups, nb))
             return -1;
@@ -438,20 +483,29 @@ void gf_tree_free(GfTree& T) {
     dfree(T.key); dfree(T.key_alt); dfree(T.idx); dfree(T.idx_alt); dfree(T.lcp); dfree(T.d0);
     dfree(T.chain); dfree(T.tx); dfree(T.ty); dfree(T.tz); dfree(T.tm); dfree(T.tdisp);
     dfree(T.xb); dfree(T.start); dfree(T.count); dfree(T.next); dfree(T.parent); dfree(T.leaf);
-    dfree(T.depth); dfree(T.child8); dfree(T.arrive); dfree(T.com); dfree(T.open_r);
+    dfree(T.depth); dfree(T.child8); dfree(T.dlist); dfree(T.dmeta); dfree(T.com); dfree(T.open_r);
     dfree(T.box); dfree(T.ndisp); dfree(T.gflag); dfree(T.groups); dfree(T.counters);
     dfree(T.root); dfree(T.bbox_part); dfree(T.scan_tmp); dfree(T.sort_tmp);
     T = GfTree();
 }
 
-__global__ __launch_bounds__(TB) void k_zero_arrive(const int* __restrict__ counters,
-                                                    int* __restrict__ arrive) {
-    const int nn = counters[0];
-    for (int i = blockIdx.x * TB + threadIdx.x; i < nn; i += gridDim.x * TB) arrive[i] = 0;
+// Bottom-up centre-of-mass pass over the held topology (build and refit):
+// one launch per depth, deepest first.  A level of depth d holds at most 8^d
+// nodes, which sizes the grids of the shallow levels.
+static int com_pass(GfTree& T, hipStream_t s) {
+    const int* doff = T.dmeta + 32;
+    for (int d = GF_KEY_LEVELS; d >= 0; --d) {
+        const int g = d >= 6 ? 1024 : std::max(1, 1 << std::max(0, 3 * d - 8));
+        hipLaunchKernelGGL(k_level_com, dim3(g), dim3(TB), 0, s, d, doff, T.dlist, T.start,
+                           T.count, T.leaf, T.child8, T.box, T.tx, T.ty, T.tz, T.tm, T.tdisp,
+                           T.com, T.open_r, T.ndisp, double(T.theta));
+    }
+    GF_LAUNCH_CHECK();
+    return 0;
 }
 
 int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float theta,
-                  hipStream_t s) {
+                  hipStream_t s, bool with_com) {
     if (gf_tree_reserve(T, n)) return -1;
     T.n = n;
     T.theta = theta;
@@ -460,7 +514,7 @@ int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float
     const int g = gf_div_up(n, TB);
     const int nbb = std::min(1024, g);
     hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(TB), 0, s, pos, n, T.bbox_part, T.idx);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, T.bbox_part, nbb, T.root);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(TB), 0, s, T.bbox_part, nbb, T.root);
     hipLaunchKernelGGL(k_keys, dim3(g), dim3(TB), 0, s, pos, n, T.root, T.key);
     GF_LAUNCH_CHECK();
     int in_alt = 0;
@@ -477,7 +531,7 @@ int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float
     if (gf_scan_excl(T.chain, T.chain, n, T.scan_tmp, s)) return -1;
     const int ng = node_grid(T.cap_nodes);
     hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, s, T.chain, n, T.counters);
-    hipLaunchKernelGGL(k_node_init, dim3(ng), dim3(TB), 0, s, T.counters, T.child8, T.arrive);
+    hipLaunchKernelGGL(k_node_init, dim3(ng), dim3(TB), 0, s, T.counters, T.child8, T.dmeta);
     hipLaunchKernelGGL(k_emit, dim3(g), dim3(TB), 0, s, T.key, T.chain, T.d0, n, T.root, T.start,
                        T.count, T.next, T.parent, T.leaf, T.depth, T.box, T.child8, T.counters);
     hipLaunchKernelGGL(k_leaf_sort, dim3(ng), dim3(TB), 0, s, T.counters, T.start, T.count,
@@ -485,9 +539,13 @@ int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float
     GF_HIP(hipMemcpyAsync(T.xb, pos, sizeof(double) * 3 * size_t(n), hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_gather, dim3(g), dim3(TB), 0, s, pos, mass, T.idx, n, nullptr, T.tx,
                        T.ty, T.tz, T.tm, T.tdisp);
-    hipLaunchKernelGGL(k_com_climb, dim3(ng), dim3(TB), 0, s, T.counters, T.start, T.count,
-                       T.parent, T.leaf, T.child8, T.arrive, T.box, T.tx, T.ty, T.tz, T.tm,
-                       T.tdisp, T.com, T.open_r, T.ndisp, double(theta));
+    hipLaunchKernelGGL(k_depth_count, dim3(ng), dim3(TB), 0, s, T.counters, T.depth, T.dmeta);
+    hipLaunchKernelGGL(k_depth_offsets, dim3(1), dim3(64), 0, s, T.dmeta, T.dmeta + 32,
+                       T.dmeta + 64);
+    hipLaunchKernelGGL(k_depth_scatter, dim3(ng), dim3(TB), 0, s, T.counters, T.depth,
+                       T.dmeta + 32, T.dmeta + 64, T.dlist);
+    GF_LAUNCH_CHECK();
+    if (with_com && com_pass(T, s)) return -1;
     // group offsets: the node count lives on the device, so the scan runs over
     // the node bound of this n, with zero flags past the node count
     const int len = gf_tree_node_bound(n);
@@ -513,10 +571,7 @@ int gf_tree_refit(GfTree& T, const double* pos, const double* mass, hipStream_t 
     const int ng = node_grid(T.cap_nodes);
     hipLaunchKernelGGL(k_gather, dim3(g), dim3(TB), 0, s, pos, mass, T.idx, n, T.xb, T.tx, T.ty,
                        T.tz, T.tm, T.tdisp);
-    hipLaunchKernelGGL(k_zero_arrive, dim3(ng), dim3(TB), 0, s, T.counters, T.arrive);
-    hipLaunchKernelGGL(k_com_climb, dim3(ng), dim3(TB), 0, s, T.counters, T.start, T.count,
-                       T.parent, T.leaf, T.child8, T.arrive, T.box, T.tx, T.ty, T.tz, T.tm,
-                       T.tdisp, T.com, T.open_r, T.ndisp, double(T.theta));
     GF_LAUNCH_CHECK();
-    return 0;
+    (void)ng;
+    return com_pass(T, s);
 }

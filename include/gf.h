@@ -87,6 +87,12 @@ int gf_tree_export(int cap, int* start, int* count, int* next, int* leaf, int* d
  * precision per call. */
 int gf_gravity_dev(int n, const double* pos, const double* mass, double* acc, double theta,
                    double eps, const uint8_t* active, int rebuild, int precision, void* stream);
+/* The two halves of gf_gravity_dev, so callers can time them apart: build
+ * the tree of (pos, mass) (cpp/nbody.cpp:202-256), then walk it for the
+ * active particles (cpp/nbody.cpp:335-411). */
+int gf_tree_build_dev(int n, const double* pos, const double* mass, double theta, void* stream);
+int gf_walk_dev(int n, const uint8_t* active, double* acc, double eps, int precision,
+                void* stream);
 /* Device-pointer FP64 direct sum for nt targets. */
 int gf_direct_dev(int n, const double* pos, const double* mass, int nt, const int64_t* tgt,
                   double* acc, double eps, void* stream);
@@ -94,6 +100,56 @@ int gf_direct_dev(int n, const double* pos, const double* mass, int nt, const in
 int gf_interactions_dev(long long* out);
 /* Waits for the stream and reports kernel-side errors. */
 int gf_sync(void* stream);
+
+/* ------------------------------------------------------------------ SPH
+ * The reference's SPH is Python + Numba (galaxy_formation/sph.py), with no
+ * native ABI; these entry points are what its functions call here. */
+
+/* kernel / kernel_grad (sph.py:31-54) for n (r, h) pairs: w = W(r, h),
+ * g = (1/r) dW/dr; either output may be NULL. */
+int gf_sph_kernel(int n, const double* r, const double* h, double* w, double* g);
+
+/* sph_forces (sph.py:57-106): density, pressure + viscosity acceleration,
+ * du/dt and Courant time step over the neighbour table nb (n, k) int64 of
+ * local indices (the particle itself included).  divv (may be NULL) receives
+ * the SPH velocity divergence -(1/rho_i) sum_j m_j (v_ij . r_ij) (1/r) dW/dr. */
+int gf_sph_forces(int n, int k, const double* pos, const double* vel, const double* mass,
+                  const int64_t* nb, const double* h, const double* u, double gamma, double alpha,
+                  double beta, double* rho, double* acc, double* dudt, double* dt_i, double* divv);
+
+/* Exact k nearest neighbours of every particle, the particle itself first
+ * (replaces the cKDTree query of sph.py:165-166): nb (n, k) int64, dist
+ * (n, k), rows ascending in (distance, index).  r_init (may be NULL): first
+ * search radius per particle.  Returns the number of passes of the radius
+ * iteration; *mean_passes (may be NULL) gets the mean per particle. */
+int gf_knn(int n, const double* pos, int k, const double* r_init, int64_t* nb, double* dist,
+           double* mean_passes);
+
+/* knn_from_candidates (sph.py:109-135): the k nearest of each row's m
+ * candidates (-1 = none), ascending.  Returns 1 when every row had >= k valid
+ * candidates, 0 otherwise (rows without enough are left untouched). */
+int gf_knn_from_candidates_host(int n, int m, const double* pos, const int64_t* cand, int k,
+                                int64_t* nb, double* dist);
+
+/* Device-resident hydro of the gas (simulation.py:74-95): gas octree, exact
+ * kNN by radius iteration (r_init: first radius per particle, NULL = from
+ * the tree's cell densities), h = max(d_k / 2, eps / 2) (simulation.py:84),
+ * density and forces (sph.py:57-106).  Inputs and outputs are device arrays
+ * in input order; outputs may be NULL.  Returns the number of passes. */
+int gf_hydro_dev(int n, const double* pos, const double* vel, const double* mass, const double* u,
+                 double eps, int k, const double* r_init, double gamma, double alpha, double beta,
+                 double* h, double* rho, double* acc, double* dudt, double* dt, double* divv,
+                 void* stream);
+/* The three stages of gf_hydro_dev, so callers can time them apart. */
+int gf_hydro_setup_dev(int n, const double* pos, const double* vel, const double* mass,
+                       const double* u, void* stream);
+int gf_hydro_knn_dev(int k, const double* r_init, void* stream);
+int gf_hydro_forces_dev(double eps, double gamma, double alpha, double beta, double* h,
+                        double* rho, double* acc, double* dudt, double* dt, double* divv,
+                        void* stream);
+/* Passes each particle of the last solve took (host array, input order; may
+ * be NULL) and their total.  Returns the number of pass launches. */
+int gf_hydro_passes(int* per_particle, long long* total);
 
 #ifdef __cplusplus
 }
