@@ -1,9 +1,19 @@
 """Force-evaluation throughput of the MI355X path on one GPU.
 
 A step is one full force evaluation of the workload with every particle
-active, inputs resident in HBM: tree build (keys, sort, nodes, FP64 centres of
-mass) and the grouped Barnes-Hut walk for all N particles.  Timed with HIP
-events (torch.cuda.Event on the stream the library runs on).
+active and the inputs resident in HBM:
+  gravity  tree build (keys, radix sort, nodes, FP64 centres of mass) and the
+           grouped Barnes-Hut walk for all N particles;
+  hydro    gas tree, exact k-nearest-neighbour solve (radius iteration),
+           h = max(d_k / 2, eps / 2), density, pressure/viscosity forces.
+Two paths are timed on the same state:
+  plain     FP64 pair arithmetic; the neighbour solve starts from radii
+            estimated from the gas tree's cell densities;
+  fast      FP32 pair arithmetic (FP64 tree, positions and centres of mass);
+            the neighbour solve starts from the learned radius proposal
+            (or, without a trained network, from d_k of the previous step).
+Phases are timed with HIP events (torch.cuda.Event) on the stream the
+library runs on.
 
     python bench.py [--steps K] [--warmup W] [--workload disk1m] [--theta 0.5]
 
@@ -25,19 +35,110 @@ FLOP_PER_INTERACTION = 20          # Nyland, Harris & Prins (GPU Gems 3, ch. 31)
 PEAK_FP32 = 157.3e12               # MI355X vector FP32, FLOP/s
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
+GAMMA, ALPHA, BETA = 5.0 / 3.0, 1.0, 2.0
+PHASES = ("tree", "walk", "gas_tree", "knn", "sph")
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="disk1m")
     p.add_argument("--theta", type=float, default=0.5)
+    p.add_argument("--k", type=int, default=64)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-sample", type=int, default=1000000,
-                   help="particles of the workload the CPU baseline evaluates")
     return p.parse_args()
+
+
+class Bench:
+    def __init__(self, a):
+        import torch
+
+        from galaxy_formation_amd import _lib, ics
+        self.torch, self._lib, self.a = torch, _lib, a
+        self.L = _lib.load()
+        self.dev = torch.device("cuda:0")
+        c = ics.config(a.workload)
+        self.c = c
+        self.n = c.n
+        self.gas = c.gas
+        self.ng = len(self.gas)
+        assert (self.gas == np.arange(self.ng)).all(), "workloads keep their gas first"
+        T = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(self.dev)  # noqa: E731
+        self.pos, self.vel, self.mass, self.u = T(c.pos), T(c.vel), T(c.mass), T(c.u)
+        z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.dev)  # noqa: E731
+        self.acc_g = {0: z(self.n, 3), 1: z(self.n, 3)}
+        self.out = {p: dict(h=z(self.ng), rho=z(self.ng), acc=z(self.ng, 3), dudt=z(self.ng),
+                            dt=z(self.ng), divv=z(self.ng)) for p in (0, 1)}
+        # a stream of its own (default priority): the library then runs on the
+        # stream the events are recorded on (handle 0 would select its own)
+        self.stream = torch.cuda.Stream(device=self.dev)
+        torch.cuda.set_stream(self.stream)
+        self.sp = self.stream.cuda_stream
+
+    def p(self, t):
+        return None if t is None else t.data_ptr()
+
+    def hydro(self, prec, pos, r_init, ev=None, rec=None):
+        L, chk, sp, o = self.L, self._lib.check, self.sp, self.out[prec]
+        rec(2) if rec else None
+        chk(L.gf_hydro_setup_dev(self.ng, self.p(pos), self.p(self.vel), self.p(self.mass),
+                                 self.p(self.u), sp))
+        rec(3) if rec else None
+        passes = chk(L.gf_hydro_knn_dev(self.a.k, self.p(r_init), sp))
+        rec(4) if rec else None
+        chk(L.gf_hydro_forces_dev(self.c.eps, GAMMA, ALPHA, BETA,
+                                  *[self.p(o[x]) for x in ("h", "rho", "acc", "dudt", "dt", "divv")],
+                                  sp))
+        rec(5) if rec else None
+        return passes
+
+    def step(self, prec, r_init, ev=None):
+        L, chk, sp = self.L, self._lib.check, self.sp
+
+        def rec(i):
+            if ev is not None:
+                ev[i].record(self.stream)
+        rec(0)
+        chk(L.gf_tree_build_dev(self.n, self.p(self.pos), self.p(self.mass), self.a.theta, sp))
+        rec(1)
+        chk(L.gf_walk_dev(self.n, None, self.p(self.acc_g[prec]), self.c.eps, prec, sp))
+        return self.hydro(prec, self.pos, r_init, ev, rec)
+
+    def timed(self, prec, r_init):
+        torch = self.torch
+        for _ in range(self.a.warmup):
+            self.step(prec, r_init)
+        torch.cuda.synchronize()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(self.a.steps)]
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        passes = []
+        w0 = time.perf_counter()
+        t0.record(self.stream)
+        for k in range(self.a.steps):
+            passes.append(self.step(prec, r_init, evs[k]))
+        t1.record(self.stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        self._lib.check(self.L.gf_sync(self.sp))
+        res = {"ms": t0.elapsed_time(t1) / self.a.steps, "wall_ms": 1e3 * wall / self.a.steps,
+               "passes": int(np.max(passes))}
+        for i, name in enumerate(PHASES):
+            res[name + "_ms"] = float(np.mean([e[i].elapsed_time(e[i + 1]) for e in evs]))
+        inter = np.zeros(1, np.int64)
+        self._lib.check(self.L.gf_interactions_dev(self._lib.ptr(inter)))
+        res["interactions"] = int(inter[0])
+        tot = np.zeros(1, np.int64)
+        self._lib.check(self.L.gf_hydro_passes(None, self._lib.ptr(tot)))
+        res["mean_passes_per_particle"] = float(tot[0]) / self.ng
+        st = np.zeros(6, np.int64)
+        self._lib.check(self.L.gf_hydro_stats(st, 6))
+        res["knn_groups"] = int(st[4])
+        res["knn_groups_to_heap_pass"] = int(st[1])
+        res["knn_groups_to_global_stack"] = int(st[2])
+        return res
 
 
 def main():
@@ -46,131 +147,129 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the multi-GPU "
               "decomposition is described, not built", file=sys.stderr)
         sys.exit(2)
-    import torch
+    B = Bench(a)
+    torch, n, ng, c = B.torch, B.n, B.ng, B.c
 
-    from galaxy_formation_amd import _lib, ics
+    # previous state of the gas: the same particles one step earlier (drifted
+    # back by a time step D = 10th percentile of the particles' admissible
+    # steps, the long step of simulation.py:117); its h seeds the fast path
+    B.step(1, None)
+    torch.cuda.synchronize()
+    dt_c = B.out[1]["dt"].cpu().numpy()
+    ag = B.acc_g[1].cpu().numpy()
+    agas = ag[:ng] + B.out[1]["acc"].cpu().numpy()
+    amag = np.linalg.norm(np.concatenate([agas, ag[ng:]]), axis=1)
+    dt_ind = 0.3 * np.sqrt(c.eps / np.maximum(amag, 1e-30))
+    dt_ind[:ng] = np.minimum(dt_ind[:ng], dt_c)
+    D = float(np.percentile(dt_ind, 10))
+    pos_prev = B.pos[:ng] - D * B.vel[:ng]
+    B.hydro(1, pos_prev, None)
+    dk_prev = torch.zeros(ng, dtype=torch.float64, device=B.dev)
+    B._lib.check(B.L.gf_hydro_dk_dev(B.p(dk_prev), B.sp))
+    torch.cuda.synchronize()
+    r_hprev = dk_prev.contiguous()
 
-    L = _lib.load()
-    dev = torch.device("cuda:0")
-    c = ics.config(a.workload)
-    n = c.n
-    pos = torch.from_numpy(c.pos).to(dev)
-    mass = torch.from_numpy(c.mass).to(dev)
-    acc = {p: torch.zeros((n, 3), dtype=torch.float64, device=dev) for p in (0, 1)}
-    # a stream of its own (default priority): the library then runs on the
-    # stream the events are recorded on (handle 0 would select its own stream)
-    stream = torch.cuda.Stream(device=dev)
-    torch.cuda.set_stream(stream)
-    sp = stream.cuda_stream
-    P = _lib.ptr
+    res = {"plain_fp64": B.timed(1, None)}
+    h_plain = B.out[1]["h"].cpu().numpy()
+    rho_plain = B.out[1]["rho"].cpu().numpy()
+    res["fast_fp32"] = B.timed(0, r_hprev)
+    h_fast = B.out[0]["h"].cpu().numpy()
+    rho_fast = B.out[0]["rho"].cpu().numpy()
 
-    def tp(t):
-        return t.data_ptr()
-
-    def step(prec, ev=None):
-        if ev:
-            ev[0].record(stream)
-        _lib.check(L.gf_tree_build_dev(n, tp(pos), tp(mass), a.theta, sp))
-        if ev:
-            ev[1].record(stream)
-        _lib.check(L.gf_walk_dev(n, None, tp(acc[prec]), c.eps, prec, sp))
-        if ev:
-            ev[2].record(stream)
-
-    res = {}
-    for prec, name in ((1, "fp64"), (0, "fp32")):
-        for _ in range(a.warmup):
-            step(prec)
-        torch.cuda.synchronize()
-        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(a.steps)]
-        t0 = torch.cuda.Event(enable_timing=True)
-        t1 = torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        wall0 = time.perf_counter()
-        t0.record(stream)
-        for k in range(a.steps):
-            step(prec, evs[k])
-        t1.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - wall0
-        _lib.check(L.gf_sync(sp))
-        ms = t0.elapsed_time(t1) / a.steps
-        build_ms = np.mean([e[0].elapsed_time(e[1]) for e in evs])
-        walk_ms = np.mean([e[1].elapsed_time(e[2]) for e in evs])
-        inter = np.zeros(1, np.int64)
-        _lib.check(L.gf_interactions_dev(P(inter)))
-        res[name] = dict(ms=ms, wall_ms=1e3 * wall / a.steps, build_ms=build_ms, walk_ms=walk_ms,
-                         interactions=int(inter[0]))
-
-    # accuracy on a seeded 2,000-particle subset against the FP64 direct sum
+    # accuracy: gravity on a seeded 2,000-particle subset vs the FP64 direct sum
     sub = np.sort(np.random.default_rng(0).choice(n, min(n, 2000), replace=False))
-    subt = torch.from_numpy(sub).to(dev)
-    exact = torch.zeros((len(sub), 3), dtype=torch.float64, device=dev)
-    _lib.check(L.gf_direct_dev(n, tp(pos), tp(mass), len(sub), tp(subt), tp(exact), c.eps, sp))
+    subt = torch.from_numpy(sub).to(B.dev)
+    exact = torch.zeros((len(sub), 3), dtype=torch.float64, device=B.dev)
+    B._lib.check(B.L.gf_direct_dev(n, B.p(B.pos), B.p(B.mass), len(sub), B.p(subt), B.p(exact),
+                                   c.eps, B.sp))
     torch.cuda.synchronize()
     ex = exact.cpu().numpy()
-    for name, p in (("fp64", 1), ("fp32", 0)):
-        ap = acc[p].cpu().numpy()[sub]
+    for name, prec in (("plain_fp64", 1), ("fast_fp32", 0)):
+        ap = B.acc_g[prec].cpu().numpy()[sub]
         e = np.linalg.norm(ap - ex, axis=1) / np.linalg.norm(ex, axis=1)
-        res[name]["rms_rel_err_vs_direct"] = float(np.sqrt(np.mean(e ** 2)))
-    a32 = acc[0].cpu().numpy()
-    a64 = acc[1].cpu().numpy()
+        res[name]["grav_rms_rel_err_vs_direct"] = float(np.sqrt(np.mean(e ** 2)))
+    a32, a64 = B.acc_g[0].cpu().numpy(), B.acc_g[1].cpu().numpy()
     d = np.linalg.norm(a32 - a64, axis=1) / np.linalg.norm(a64, axis=1)
-    res["fp32"]["rms_rel_diff_vs_fp64"] = float(np.sqrt(np.mean(d ** 2)))
+    res["fast_fp32"]["grav_rms_rel_diff_vs_fp64"] = float(np.sqrt(np.mean(d ** 2)))
+    res["fast_fp32"]["h_equal_plain"] = bool(np.array_equal(h_fast, h_plain))
+    res["fast_fp32"]["rho_max_rel_err_vs_plain"] = float(np.max(np.abs(rho_fast - rho_plain) / rho_plain))
 
-    main_p = res["fp32"]
-    inter = main_p["interactions"]
-    achieved = FLOP_PER_INTERACTION * inter / (main_p["walk_ms"] * 1e-3)
+    fast, plain = res["fast_fp32"], res["plain_fp64"]
+    inter = fast["interactions"]
+    achieved = FLOP_PER_INTERACTION * inter / (fast["walk_ms"] * 1e-3)
     out = {
-        "metric": "force-evals/s at 1M particles, FP32 interactions vs plain FP64 GPU",
-        "value": n / (main_p["ms"] * 1e-3),
+        "metric": "force-evals/s at 1M particles, FP32 + learned-h vs plain FP64 GPU",
+        "value": n / (fast["ms"] * 1e-3),
         "unit": "force-evals/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": main_p["ms"],
+        "ms_per_step": fast["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair arithmetic, f64 positions/tree",
+        "dtype": "f32 (pair arithmetic; f64 positions, tree, centres of mass, SPH sums)",
         "data": "synthetic",
-        "config": {"workload": f"{a.workload}: gravity (tree build + walk), all {n} active",
-                   "n": n, "theta": a.theta, "eps": c.eps},
+        "config": {"workload": f"{a.workload}: {n} particles ({ng} gas), all active, "
+                               "gravity + SPH (kNN, h, density, forces)",
+                   "n": n, "n_gas": ng, "theta": a.theta, "eps": c.eps, "k": a.k,
+                   "first_radius_fast": "d_k of the previous step"},
         "roofline": {"bound": "valu-fp32", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
-                     "kernel": "k_walk<false,false>", "interactions": inter,
-                     "flop_per_interaction": FLOP_PER_INTERACTION},
-        "fp64_plain": {"value": n / (res["fp64"]["ms"] * 1e-3), **res["fp64"]},
-        "fp32": res["fp32"],
-        "speedup_fp32_vs_fp64": res["fp64"]["ms"] / res["fp32"]["ms"],
+                     "kernel": "k_walk<false, false> (FP32 group walk + list evaluation)",
+                     "interactions": inter, "flop_per_interaction": FLOP_PER_INTERACTION},
+        "plain_fp64": {"value": n / (plain["ms"] * 1e-3), **plain},
+        "fast_fp32": fast,
+        "speedup_fast_vs_plain": plain["ms"] / fast["ms"],
+        "previous_step_D": D,
     }
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(c, a)
+        out["cpu_baseline"] = cpu_baseline(B)
     print(json.dumps(out))
 
 
-def cpu_baseline(c, a):
-    """The reference engine (cpp/nbody.cpp built from source, oracle/_ref) on
-    the host cores: nbody_accel_active, every particle active, rebuild = 1."""
-    from oracle import oracle as O
+def cpu_baseline(B):
+    """The reference path on the host cores, one full evaluation of the same
+    workload: cpp/nbody.cpp built from source (oracle/_ref; nbody_accel_active,
+    all active, rebuild = 1), scipy cKDTree kNN (the reference's own library,
+    sph.py:166, workers = -1), and sph_forces restated in C (the oracle; the
+    reference's Numba is not installed)."""
+    from scipy.spatial import cKDTree
 
+    from oracle import oracle as O
     L = O.ref_lib()
     if L is None:
         return None
-    ns = min(a.cpu_sample, c.n)
-    pos = np.ascontiguousarray(c.pos[:ns])
-    mass = np.ascontiguousarray(c.mass[:ns])
+    c, ng, k = B.c, B.ng, B.a.k
+    pos, mass = c.pos, c.mass
     acc = np.zeros_like(pos)
-    L.nbody_accel_active(ns, pos, mass, acc, a.theta, c.eps, np.ones(ns, np.uint8), 1)  # warm-up
-    times = []
-    for _ in range(3):
-        t = time.perf_counter()
-        L.nbody_accel_active(ns, pos, mass, acc, a.theta, c.eps, np.ones(ns, np.uint8), 1)
-        times.append(time.perf_counter() - t)
-    t = float(np.median(times))
-    return {"value": ns / t, "unit": "force-evals/s", "cores": int(L.nbody_num_threads()),
+    act = np.ones(c.n, np.uint8)
+    gp = np.ascontiguousarray(pos[:ng])
+    gv = np.ascontiguousarray(c.vel[:ng])
+    gm = np.ascontiguousarray(mass[:ng])
+    gu = np.ascontiguousarray(c.u[:ng])
+
+    def one():
+        t = [time.perf_counter()]
+        L.nbody_accel_active(c.n, pos, mass, acc, B.a.theta, c.eps, act, 1)
+        t.append(time.perf_counter())
+        d, nb = cKDTree(gp).query(gp, k=k, workers=-1)
+        t.append(time.perf_counter())
+        h = np.maximum(0.5 * d[:, -1], 0.5 * c.eps)
+        O.sph_forces(gp, gv, gm, nb, h, gu, GAMMA, ALPHA, BETA)
+        t.append(time.perf_counter())
+        return np.diff(t)
+
+    one()
+    runs = np.array([one() for _ in range(2)])
+    tm = np.median(runs, axis=0)
+    total = float(tm.sum())
+    return {"value": c.n / total, "unit": "force-evals/s", "cores": int(L.nbody_num_threads()),
             "kind": "reference",
-            "sample": f"first {ns} particles of {a.workload}, gravity only, median of 3 after 1 warm-up"}
+            "sample": f"full {B.a.workload} evaluation ({c.n} particles, {ng} gas, k={k}), "
+                      "median of 2 after 1 warm-up; gravity = reference nbody.cpp, kNN = "
+                      "scipy cKDTree, SPH sums = C restatement (Numba absent)",
+            "phases_s": {"gravity": float(tm[0]), "knn": float(tm[1]), "sph": float(tm[2])}}
 
 
 if __name__ == "__main__":

@@ -54,6 +54,9 @@ SIGNATURES = {
     "gf_hydro_knn_dev": (_ci, [_ci, _vp, _vp]),
     "gf_hydro_forces_dev": (_ci, [_cd, _cd, _cd, _cd, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_hydro_passes": (_ci, [_vp, _vp]),
+    "gf_hydro_dk_dev": (_ci, [_vp, _vp]),
+    "gf_hydro_stats": (_ci, [_i64, _ci]),
+    "gf_hydro_neighbours": (_ci, [_i64, _vp]),
 }
 
 _lib = None

@@ -489,6 +489,50 @@ int gf_hydro_dev(int n, const double* pos, const double* vel, const double* mass
     return passes;
 }
 
+int gf_hydro_dk_dev(double* dk, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (g_ctx.sph.tree.n > 0 && g_ctx.sph_k <= 0) {
+        gf_set_error("gf_hydro_dk_dev: no neighbour solve held");
+        return -1;
+    }
+    return gf_sph_dk_out(g_ctx.sph, dk, gf_stream(stream));
+}
+
+int gf_hydro_neighbours(int64_t* nb, double* dk) {
+    if (gf_ctx_init()) return -1;
+    const int n = g_ctx.sph.tree.n, k = g_ctx.sph_k;
+    if (n == 0) return 0;
+    if (k <= 0) {
+        gf_set_error("gf_hydro_neighbours: no neighbour solve held");
+        return -1;
+    }
+    hipStream_t s = g_ctx.stream;
+    if (STAGE(g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k) || STAGE(g_ctx.d_out1, sizeof(double) * n))
+        return -1;
+    GF_HIP(hipDeviceSynchronize());
+    if (gf_sph_table_out(g_ctx.sph, k, g_ctx.d_nb, s) || gf_sph_dk_out(g_ctx.sph, g_ctx.d_out1, s))
+        return -1;
+    GF_HIP(hipMemcpyAsync(nb, g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k, hipMemcpyDeviceToHost, s));
+    if (dk) GF_HIP(hipMemcpyAsync(dk, g_ctx.d_out1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    return k;
+}
+
+int gf_hydro_stats(long long* out, int len) {
+    if (gf_ctx_init()) return -1;
+    int nn = 0, ng = 0;
+    const GfSph& S = g_ctx.sph;
+    if (S.tree.counters) {
+        int c[4];
+        GF_HIP(hipMemcpy(c, S.tree.counters, sizeof(c), hipMemcpyDeviceToHost));
+        nn = c[0];
+        ng = c[1];
+    }
+    const long long v[6] = {S.last_passes, S.stat_queued, S.stat_queued2, nn, ng, S.tree.n};
+    for (int i = 0; i < len && i < 6; ++i) out[i] = v[i];
+    return 0;
+}
+
 int gf_hydro_passes(int* per_particle, long long* total) {
     if (gf_ctx_init()) return -1;
     const int n = g_ctx.sph.tree.n;

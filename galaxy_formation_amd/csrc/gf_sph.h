@@ -9,6 +9,7 @@ struct GfSph {
     // per gas particle, gas-tree order
     double *vx = nullptr, *vy = nullptr, *vz = nullptr, *u = nullptr;
     double *r = nullptr;         // search radius of the current pass
+    double *r_lo = nullptr, *r_hi = nullptr;   // radius bracket of the iteration
     double *dk = nullptr;        // distance to the k-th neighbour (self included)
     double *h = nullptr, *rho = nullptr, *ih = nullptr, *sg = nullptr;
     int8_t *done = nullptr;
@@ -16,13 +17,13 @@ struct GfSph {
     int *inv = nullptr;          // input index -> tree position
     int *leaf_of = nullptr;      // leaf node of each tree position
     int *nb = nullptr;           // [k][n] neighbours: tree positions (or input indices)
-    int *state = nullptr;        // [0] unresolved, [1] cursor, [2] cursor (global stacks),
-                                 // [3] queued, [4] global-stack overflow, [5] passes
-    int *ovf = nullptr;
+    int *state = nullptr;        // pass counters (layout in gf_sph.hip)
+    int *ovf = nullptr, *ovf2 = nullptr;
     int *gstack = nullptr;
     int cap_groups = 0;
     int last_passes = 0;         // pass launches of the last solve
-    long long last_particle_passes = 0;
+    long long stat_queued = 0;   // groups the fast pass handed on, summed over passes
+    long long stat_queued2 = 0;  // groups handed on to the global-stack pass
 };
 
 void gf_sph_free(GfSph& S);
@@ -68,6 +69,12 @@ size_t gf_sph_table_scratch_bytes(int n, int k);
 // Writes the neighbour table of the last solve as (n, k) int64 input indices
 // and (n, k) distances, rows in input order (requires a sorted solve).
 int gf_sph_export_knn(GfSph& S, int kq, int64_t* nb, double* dist, hipStream_t s);
+
+// d_k of the last solve, input order (device array of n).
+int gf_sph_dk_out(GfSph& S, double* dk, hipStream_t s);
+// Tree-position table of the last solve as (n, k) int64 input indices, rows
+// in input order, neighbours unordered (device array).
+int gf_sph_table_out(GfSph& S, int k, int64_t* nb, hipStream_t s);
 
 // knn_from_candidates (sph.py:109-135): per row, the k nearest of the m
 // candidates (-1 = none), ascending (d^2, index).  *ok = 1 when every row had

@@ -42,17 +42,33 @@ struct KnnArgs {
     const double* tz;
     const uint32_t* idx;
     const int* inv;
+    const int* parent_;
+    const int* leaf_of;
     double* r;
+    double* r_lo;   // largest radius known to hold fewer than kq particles (0 = none)
+    double* r_hi;   // smallest radius known to overflow the member list (inf = none)
     double* dk;
     int8_t* done;
     int* passes;
     int* nb;
     int n, kq;
     int tree_pos_out, sorted;
-    int* state;
-    int* ovf;
+    int* state;     // [0] unresolved, [1] cursor of the fast pass, [2] cursor of the
+                    // heap pass, [3] queue 1 length, [4] errors, [5] cursor of the
+                    // global-stack heap pass, [6] queue 2 length
+    int* ovf;       // queue 1: groups the fast pass hands to the heap pass
+    int* ovf2;      // queue 2: groups the heap pass hands to its global-stack variant
     int* gstack;
+    int use_queue;  // the heap pass walks queue 1 (else every group)
 };
+
+// Nodes to pop from a LIFO of capacity cap holding sn entries: a batch of 64
+// while 64 nodes' children fit with room to spare, else one (depth first).
+// Depth first from any state needs at most 7 more entries per tree level, so
+// a reserve of 7 * 64 + 8 * (GF_KEY_LEVELS + 1) keeps every push in bounds.
+__device__ __forceinline__ int stack_take(int sn, int cap) {
+    return (cap - sn >= 7 * 64 + 8 * (GF_KEY_LEVELS + 1)) ? min(64, sn) : 1;
+}
 
 __device__ __forceinline__ bool key_less(double da, int ia, double db, int ib) {
     return da < db || (da == db && ia < ib);
@@ -69,8 +85,9 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
     const int lane = __lane_id();
     int* stack = GST ? a.gstack + size_t(blockIdx.x) * KGSCAP : s_stack;
     const int scap = GST ? KGSCAP : KSCAP;
-    const int ngroups = GST ? a.state[3] : a.counters[1];
-    int* cursor = a.state + (GST ? 2 : 1);
+    const int* list = GST ? a.ovf2 : (a.use_queue ? a.ovf : nullptr);
+    const int ngroups = GST ? a.state[6] : (a.use_queue ? a.state[3] : a.counters[1]);
+    int* cursor = a.state + (GST ? 5 : 2);
     const int kq = a.kq;
 
     while (true) {
@@ -78,7 +95,7 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
         if (lane == 0) gi = atomicAdd(cursor, 1);
         gi = __shfl(gi, 0, 64);
         if (gi >= ngroups) break;
-        const int G = a.groups[GST ? a.ovf[gi] : gi];
+        const int G = a.groups[list ? list[gi] : gi];
         const int b0 = a.start[G], np = a.count[G];
         for (int c0 = 0; c0 < np; c0 += 64) {
             const int t = b0 + c0 + lane;
@@ -166,7 +183,7 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
             };
 
             while (sn > 0) {
-                const int take = min(64, sn);
+                const int take = stack_take(sn, scap);
                 sn -= take;
                 const int node = lane < take ? stack[sn + lane] : -1;
                 __syncthreads();
@@ -229,7 +246,7 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
             }
             if (overflow) {
                 if (!GST) {
-                    if (lane == 0) a.ovf[atomicAdd(&a.state[3], 1)] = gi;
+                    if (lane == 0) a.ovf2[atomicAdd(&a.state[6], 1)] = list ? list[gi] : gi;
                 } else if (lane == 0) {
                     atomicAdd(&a.state[4], 1);
                 }
@@ -294,12 +311,385 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
 }
 
 __global__ void k_knn_reset(int* state) {
-    if (threadIdx.x == 0) {
-        state[0] = 0;
-        state[1] = 0;
-        state[2] = 0;
-        state[3] = 0;
-        state[4] = 0;
+    if (threadIdx.x < 8) state[threadIdx.x] = 0;
+}
+
+// ----------------------------------------------------- fast neighbour pass
+// One block of four waves per gas group.  Candidates (gas particles inside a
+// search sphere) are gathered into LDS by a tree walk that starts at the
+// smallest node whose box strictly contains the sphere.
+//   group mode  (the group's bounding sphere b, R that holds every unresolved
+//               member's search sphere is tight: R <= 2 max r_i): wave 0
+//               gathers once for all members into the whole list;
+//   member mode (otherwise): each wave gathers its current member's own
+//               sphere into a quarter of the list.
+// Per member the 64 lanes of a wave compute the exact FP64 squared distances
+// to the list (their FP32 roundings are the radix keys: rounding to FP32
+// never reverses an order), count those within the member's radius and find
+// the kq-th smallest key by a 4-round, 8-bit MSB radix select over a 256-bin
+// LDS histogram.  Keys below it are in; among keys equal to it (FP32 ties) the
+// exact (d^2, index) order decides.  Rows are written unordered; d_k is the
+// largest exact d^2 taken.  A member with fewer than kq inside its radius
+// grows the radius; one whose sphere holds more candidates than the list
+// shrinks it; both take another pass.
+constexpr int FT = 256;
+constexpr int CMAX = 2048;
+constexpr int CPL = CMAX / 64;
+constexpr int CQ = CMAX / 4;                 // per-wave list in member mode
+constexpr int FSCAP = 1024;                  // per-wave stack
+constexpr int FAST_BLOCKS = 2048;
+constexpr uint32_t KEY_NONE = 0xFFFFFFFFu;
+
+// Smallest ancestor of `node` (node included) whose box strictly contains
+// the sphere; the root when none does.
+__device__ int containing_node(const KnnArgs& a, int node, double bx, double by, double bz,
+                               double R) {
+#pragma clang fp contract(off)
+    while (true) {
+        const int p = a.parent_ != nullptr ? a.parent_[node] : -1;
+        if (p < 0) return node;
+        const double4 bb = a.box[node];
+        const double lim = bb.w * (1.0 - 1e-12);
+        if (fabs(bx - bb.x) + R < lim && fabs(by - bb.y) + R < lim && fabs(bz - bb.z) + R < lim)
+            return node;
+        node = p;
+    }
+}
+
+// One wave gathers the particles within (b, R) of the subtree of `root` into
+// (lx, ly, lz, li)[0, cap).  Returns the count; when it exceeds cap, minus the
+// count reached (a lower bound); -1 - cap - 2 ... never -2, which flags a
+// depth-capped leaf.
+__device__ int gather_sphere(const KnnArgs& a, int root, double bx, double by, double bz,
+                             double R2, double* lx, double* ly, double* lz, int* li, int cap,
+                             int* stack) {
+#pragma clang fp contract(off)
+    const int lane = __lane_id();
+    const uint64_t lt = gf_lanemask_lt();
+    int sn = 1, nc = 0;
+    if (lane == 0) stack[0] = root;
+    __builtin_amdgcn_wave_barrier();
+    while (sn > 0) {
+        const int take = stack_take(sn, FSCAP);
+        sn -= take;
+        const int node = lane < take ? stack[sn + lane] : -1;
+        __builtin_amdgcn_wave_barrier();
+        int push = 0, first = 0, cntn = 0;
+        int4 c_lo = make_int4(-1, -1, -1, -1), c_hi = c_lo;
+        if (node >= 0) {
+            const double4 bb = a.box[node];
+            const double ex = fmax(0.0, fabs(bx - bb.x) - bb.w);
+            const double ey = fmax(0.0, fabs(by - bb.y) - bb.w);
+            const double ez = fmax(0.0, fabs(bz - bb.z) - bb.w);
+            if (ex * ex + ey * ey + ez * ez <= R2) {
+                if (a.leaf[node]) {
+                    first = a.start[node];
+                    cntn = a.count[node];
+                } else {
+                    const int4* cs = reinterpret_cast<const int4*>(a.child8 + 8 * size_t(node));
+                    c_lo = cs[0];
+                    c_hi = cs[1];
+                    push = (c_lo.x >= 0) + (c_lo.y >= 0) + (c_lo.z >= 0) + (c_lo.w >= 0) +
+                           (c_hi.x >= 0) + (c_hi.y >= 0) + (c_hi.z >= 0) + (c_hi.w >= 0);
+                }
+            }
+        }
+        const int pinc = gf_wave_incl_scan(push);
+        const int ptot = __shfl(pinc, 63, 64);
+        {
+            int o = sn + pinc - push;
+            const int cc[8] = {c_lo.x, c_lo.y, c_lo.z, c_lo.w, c_hi.x, c_hi.y, c_hi.z, c_hi.w};
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (cc[k] >= 0) stack[o++] = cc[k];
+        }
+        sn += ptot;
+        if (__ballot(cntn > GF_LEAF_SIZE) != 0) return -2;   // a depth-capped leaf
+        for (int k = 0; k < GF_LEAF_SIZE; ++k) {
+            if (__ballot(k < cntn) == 0) break;
+            bool tk = false;
+            double px = 0, py = 0, pz = 0;
+            int pj = 0;
+            if (k < cntn) {
+                pj = first + k;
+                px = a.tx[pj]; py = a.ty[pj]; pz = a.tz[pj];
+                const double dx = px - bx, dy = py - by, dz = pz - bz;
+                tk = dx * dx + dy * dy + dz * dz <= R2;
+            }
+            const uint64_t m = __ballot(tk);
+            const int o = nc + __popcll(m & lt);
+            if (tk && o < cap) {
+                lx[o] = px; ly[o] = py; lz[o] = pz;
+                li[o] = int(a.idx[pj]);
+            }
+            nc += __popcll(m);
+        }
+        if (nc > cap) return -nc;
+        __builtin_amdgcn_wave_barrier();
+    }
+    return nc;
+}
+
+// Exact kq nearest of member t among the list (lx, ly, lz, li)[0, nc), by
+// one wave.  Returns 1 when resolved, 0 when the radius was too small (grown),
+// -1 on more than 64 FP32 ties at the boundary (left for the heap pass).
+__device__ int select_member(const KnnArgs& a, int t, const double* lx, const double* ly,
+                             const double* lz, const int* li, int nc, int* hist, double* tie_d,
+                             int* tie_i) {
+#pragma clang fp contract(off)
+    const int lane = __lane_id();
+    const int kq = a.kq;
+    const double xi = a.tx[t], yi = a.ty[t], zi = a.tz[t], ri = a.r[t];
+    const double r2i = ri * ri;
+    const int ncl = (nc + 63) >> 6;
+    uint32_t key[CPL];
+    int cnt = 0;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        uint32_t kk = KEY_NONE;
+        if (c < ncl) {
+            const int j = lane + 64 * c;
+            if (j < nc) {
+                const double dx = xi - lx[j], dy = yi - ly[j], dz = zi - lz[j];
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 <= r2i) kk = __float_as_uint(float(d2));
+            }
+        }
+        key[c] = kk;
+        cnt += kk != KEY_NONE;
+    }
+    const int total = gf_wave_sum_i(cnt);
+    if (total < kq) {
+        if (lane == 0) {
+            a.passes[t] += 1;
+            const double want = double(kq) / double(max(total, 1));
+            double rn = ri * fmin(2.0, fmax(1.2, 1.1 * cbrt(want)));
+            a.r_lo[t] = ri;
+            const double hi = a.r_hi[t];
+            if (rn >= hi) rn = sqrt(ri * hi);       // bisect inside the bracket
+            a.r[t] = rn;
+            atomicAdd(&a.state[0], 1);
+        }
+        return 0;
+    }
+    uint32_t prefix = 0;
+    int need = kq;
+    for (int rd = 0; rd < 4; ++rd) {
+        const int shift = 24 - 8 * rd;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) hist[4 * lane + q] = 0;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const uint32_t kk = key[c];
+            if (c < ncl && kk != KEY_NONE &&
+                (rd == 0 || (kk >> (shift + 8)) == (prefix >> (shift + 8))))
+                atomicAdd(&hist[(kk >> shift) & 0xFF], 1);
+        }
+        __builtin_amdgcn_wave_barrier();
+        int h[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) h[q] = hist[4 * lane + q];
+        const int sum = h[0] + h[1] + h[2] + h[3];
+        const int incl = gf_wave_incl_scan(sum);
+        const int excl = incl - sum;
+        const bool here = excl < need && need <= incl;
+        int bsel = 0, nneed = 0;
+        if (here) {
+            int run = excl;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (nneed == 0 && run + h[q] >= need) {
+                    bsel = 4 * lane + q;
+                    nneed = need - run;
+                }
+                run += h[q];
+            }
+        }
+        const int src = __ffsll((unsigned long long)__ballot(here)) - 1;
+        bsel = __shfl(bsel, src, 64);
+        need = __shfl(nneed, src, 64);
+        prefix |= uint32_t(bsel) << shift;
+        __builtin_amdgcn_wave_barrier();
+    }
+    const uint32_t T = prefix;
+    int eqc = 0;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) eqc += (c < ncl) && key[c] == T;
+    const int eq = gf_wave_sum_i(eqc);
+    uint64_t tie_take = ~0ull;
+    if (eq > need) {
+        if (eq > 64) return -1;
+        const int te = gf_wave_incl_scan(eqc) - eqc;
+        int o = te;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            if (c < ncl && key[c] == T) {
+                const int j = lane + 64 * c;
+                const double dx = xi - lx[j], dy = yi - ly[j], dz = zi - lz[j];
+                tie_d[o] = dx * dx + dy * dy + dz * dz;
+                tie_i[o] = li[j];
+                ++o;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        tie_take = 0;
+        o = te;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            if (c < ncl && key[c] == T) {
+                const double dm = tie_d[o];
+                const int im = tie_i[o];
+                int rank = 0;
+                for (int q = 0; q < eq; ++q) rank += key_less(tie_d[q], tie_i[q], dm, im);
+                if (rank < need) tie_take |= 1ull << c;
+                ++o;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    int selc = 0;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+        selc += (c < ncl) && (key[c] < T || (key[c] == T && ((tie_take >> c) & 1)));
+    int o = gf_wave_incl_scan(selc) - selc;
+    double dmax = 0;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        if ((c < ncl) && (key[c] < T || (key[c] == T && ((tie_take >> c) & 1)))) {
+            const int j = lane + 64 * c;
+            const int id = li[j];
+            a.nb[size_t(o) * a.n + t] = a.tree_pos_out ? a.inv[id] : id;
+            ++o;
+            if (key[c] == T) {
+                const double dx = xi - lx[j], dy = yi - ly[j], dz = zi - lz[j];
+                dmax = fmax(dmax, dx * dx + dy * dy + dz * dz);
+            }
+        }
+    }
+    dmax = gf_wave_max_d(dmax);
+    if (lane == 0) {
+        a.dk[t] = sqrt(dmax);
+        a.done[t] = 1;
+        a.passes[t] += 1;
+    }
+    return 1;
+}
+
+__global__ __launch_bounds__(FT) void k_knn_fast(KnnArgs a) {
+#pragma clang fp contract(off)
+    __shared__ double cx[CMAX], cy[CMAX], cz[CMAX];
+    __shared__ int ci[CMAX];
+    __shared__ int stack[4][FSCAP];
+    __shared__ int hist[4][256];
+    __shared__ double tie_d[4][64];
+    __shared__ int tie_i[4][64];
+    __shared__ int ctl[8];
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    const int ngroups = a.counters[1];
+    if (threadIdx.x < 8) ctl[threadIdx.x] = 0;
+    __syncthreads();
+
+    while (true) {
+        if (threadIdx.x == 0) ctl[0] = atomicAdd(&a.state[1], 1);
+        __syncthreads();
+        const int gi = ctl[0];
+        if (gi >= ngroups) break;
+        const int G = a.groups[gi];
+        const int b0 = a.start[G], np = a.count[G];
+        if (np > 64) {
+            if (threadIdx.x == 0) a.ovf[atomicAdd(&a.state[3], 1)] = gi;
+            __syncthreads();
+            continue;
+        }
+        // wave 0: the group sphere; group mode gathers once here
+        if (w == 0) {
+            const int t = b0 + lane;
+            const bool mine = lane < np && !a.done[t];
+            int mode = 0;                       // 0 nothing to do, 1 group, 2 member
+            int nc = 0;
+            if (__ballot(mine) != 0) {
+                double xi = 0, yi = 0, zi = 0, ri = 0;
+                if (mine) {
+                    xi = a.tx[t]; yi = a.ty[t]; zi = a.tz[t]; ri = a.r[t];
+                }
+                double x0 = mine ? xi : 1e300, y0 = mine ? yi : 1e300, z0 = mine ? zi : 1e300;
+                double x1 = mine ? xi : -1e300, y1 = mine ? yi : -1e300, z1 = mine ? zi : -1e300;
+                x0 = gf_wave_min_d(x0); y0 = gf_wave_min_d(y0); z0 = gf_wave_min_d(z0);
+                x1 = gf_wave_max_d(x1); y1 = gf_wave_max_d(y1); z1 = gf_wave_max_d(z1);
+                const double bx = 0.5 * (x0 + x1), by = 0.5 * (y0 + y1), bz = 0.5 * (z0 + z1);
+                double reach = 0;
+                if (mine) {
+                    const double dx = xi - bx, dy = yi - by, dz = zi - bz;
+                    reach = sqrt(dx * dx + dy * dy + dz * dz) + ri;
+                }
+                const double R = gf_wave_max_d(reach) * (1.0 + 1e-12) + 1e-300;
+                const double rmax = gf_wave_max_d(ri);
+                mode = 2;
+                if (R <= 2.0 * rmax) {
+                    const int root = containing_node(a, G, bx, by, bz, R);
+                    nc = gather_sphere(a, root, bx, by, bz, R * R, cx, cy, cz, ci, CMAX, stack[0]);
+                    mode = nc >= 0 ? 1 : (nc == -2 ? 3 : 2);   // a full list: member mode
+                }
+            }
+            if (lane == 0) {
+                ctl[1] = mode;
+                ctl[2] = nc;
+            }
+        }
+        __syncthreads();
+        const int mode = ctl[1], ncg = ctl[2];
+        int handoff = mode == 3;
+        if (mode == 1 || mode == 2) {
+            for (int mm = w; mm < np; mm += 4) {
+                const int t = b0 + mm;
+                if (a.done[t]) continue;
+                double* lx = cx;
+                double* ly = cy;
+                double* lz = cz;
+                int* li = ci;
+                int nc = ncg;
+                if (mode == 2) {
+                    // the member's own sphere, into this wave's quarter
+                    lx = cx + w * CQ; ly = cy + w * CQ; lz = cz + w * CQ; li = ci + w * CQ;
+                    const double xi = a.tx[t], yi = a.ty[t], zi = a.tz[t];
+                    const double ri = a.r[t] * (1.0 + 1e-12) + 1e-300;
+                    const int root = containing_node(a, a.leaf_of[t], xi, yi, zi, ri);
+                    nc = gather_sphere(a, root, xi, yi, zi, ri * ri, lx, ly, lz, li, CQ, stack[w]);
+                    if (nc == -2) {
+                        handoff = 1;
+                        continue;
+                    }
+                    if (nc < 0) {
+                        // more candidates than the list holds (-nc of them at
+                        // least): shrink the radius towards 1.3 kq inside, or
+                        // bisect inside the bracket; a bracket narrower than 5 %
+                        // (the count jumps past the list) goes to the heap pass
+                        const double rr = a.r[t], lo = a.r_lo[t];
+                        const bool closed = lo > 0 && rr <= lo * 1.05;
+                        if (lane == 0) {
+                            a.passes[t] += 1;
+                            a.r_hi[t] = rr;
+                            double rn = rr * fmin(0.9, fmax(0.3, cbrt(1.3 * a.kq / double(-nc))));
+                            if (rn <= lo) rn = sqrt(lo * rr);
+                            if (!closed) a.r[t] = rn;
+                            atomicAdd(&a.state[0], 1);
+                        }
+                        if (closed) handoff = 1;
+                        continue;
+                    }
+                }
+                if (select_member(a, t, lx, ly, lz, li, nc, hist[w], tie_d[w], tie_i[w]) < 0)
+                    handoff = 1;
+            }
+        }
+        if (lane == 0 && handoff) atomicOr(&ctl[3], 1);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (ctl[3]) a.ovf[atomicAdd(&a.state[3], 1)] = gi;
+            ctl[3] = 0;
+        }
+        __syncthreads();
     }
 }
 
@@ -311,7 +701,9 @@ __global__ __launch_bounds__(TB) void k_knn_init(int n, int kq, const double* __
                                                  const int* __restrict__ parent,
                                                  const int* __restrict__ count,
                                                  const double4* __restrict__ box,
-                                                 double* __restrict__ r, int8_t* __restrict__ done,
+                                                 double* __restrict__ r, double* __restrict__ r_lo,
+                                                 double* __restrict__ r_hi,
+                                                 int8_t* __restrict__ done,
                                                  int* __restrict__ passes) {
     const int t = blockIdx.x * TB + threadIdx.x;
     if (t >= n) return;
@@ -326,6 +718,8 @@ __global__ __launch_bounds__(TB) void k_knn_init(int n, int kq, const double* __
         rr = 1.1 * cbrt(3.0 * kq / (4.0 * M_PI * dens));
     }
     r[t] = rr;
+    r_lo[t] = 0.0;
+    r_hi[t] = INFINITY;
     done[t] = 0;
     passes[t] = 0;
 }
@@ -617,7 +1011,8 @@ int sph_reserve(GfSph& S, int n, int k) {
         if (dalloc(S.vx, cn) || dalloc(S.vy, cn) || dalloc(S.vz, cn) || dalloc(S.u, cn) ||
             dalloc(S.r, cn) || dalloc(S.dk, cn) || dalloc(S.h, cn) || dalloc(S.rho, cn) ||
             dalloc(S.ih, cn) || dalloc(S.sg, cn) || dalloc(S.done, cn) || dalloc(S.passes, cn) ||
-            dalloc(S.inv, cn) || dalloc(S.leaf_of, cn))
+            dalloc(S.inv, cn) || dalloc(S.leaf_of, cn) ||
+            dalloc(S.r_lo, cn) || dalloc(S.r_hi, cn))
             return -1;
         S.cap_n = cn;
         S.cap_k = 0;
@@ -628,7 +1023,7 @@ int sph_reserve(GfSph& S, int n, int k) {
     }
     const int ng = gf_tree_node_bound(S.cap_n);
     if (ng > S.cap_groups) {
-        if (dalloc(S.ovf, ng)) return -1;
+        if (dalloc(S.ovf, ng) || dalloc(S.ovf2, ng)) return -1;
         S.cap_groups = ng;
     }
     return 0;
@@ -639,8 +1034,8 @@ int sph_reserve(GfSph& S, int n, int k) {
 void gf_sph_free(GfSph& S) {
     gf_tree_free(S.tree);
     dfree(S.vx); dfree(S.vy); dfree(S.vz); dfree(S.u); dfree(S.r); dfree(S.dk); dfree(S.h);
-    dfree(S.rho); dfree(S.ih); dfree(S.sg); dfree(S.done); dfree(S.passes); dfree(S.inv); dfree(S.leaf_of);
-    dfree(S.nb); dfree(S.state); dfree(S.ovf); dfree(S.gstack);
+    dfree(S.rho); dfree(S.ih); dfree(S.sg); dfree(S.done); dfree(S.passes); dfree(S.inv); dfree(S.leaf_of); dfree(S.r_lo); dfree(S.r_hi);
+    dfree(S.nb); dfree(S.state); dfree(S.ovf); dfree(S.ovf2); dfree(S.gstack);
     S = GfSph();
 }
 
@@ -669,27 +1064,33 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
     }
     if (sph_reserve(S, n, kq)) return -1;
     GfTree& T = S.tree;
-    if (!r_init) {
-        // leaf of each particle, for the density estimate of the first radius
-        hipLaunchKernelGGL(k_leaf_of, dim3(std::min(2048, gf_div_up(T.cap_nodes, TB))), dim3(TB), 0,
-                           s, T.counters, T.start, T.count, T.leaf, S.leaf_of);
-    }
+    // leaf of each particle: first radius and start of the member walks
+    hipLaunchKernelGGL(k_leaf_of, dim3(std::min(2048, gf_div_up(T.cap_nodes, TB))), dim3(TB), 0, s,
+                       T.counters, T.start, T.count, T.leaf, S.leaf_of);
     hipLaunchKernelGGL(k_knn_init, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, kq, r_init, T.idx,
-                       S.leaf_of, T.parent, T.count, T.box, S.r, S.done, S.passes);
+                       S.leaf_of, T.parent, T.count, T.box, S.r, S.r_lo, S.r_hi, S.done,
+                       S.passes);
     GF_LAUNCH_CHECK();
     KnnArgs a;
     a.start = T.start; a.count = T.count; a.leaf = T.leaf; a.child8 = T.child8; a.box = T.box;
     a.groups = T.groups; a.counters = T.counters; a.tx = T.tx; a.ty = T.ty; a.tz = T.tz;
-    a.idx = T.idx; a.inv = S.inv; a.r = S.r; a.dk = S.dk; a.done = S.done; a.passes = S.passes;
+    a.idx = T.idx; a.inv = S.inv; a.parent_ = T.parent; a.leaf_of = S.leaf_of; a.r = S.r;
+    a.r_lo = S.r_lo; a.r_hi = S.r_hi; a.dk = S.dk; a.done = S.done; a.passes = S.passes;
     a.nb = S.nb; a.n = n; a.kq = kq; a.tree_pos_out = tree_pos_out; a.sorted = sorted;
-    a.state = S.state; a.ovf = S.ovf; a.gstack = S.gstack;
+    a.state = S.state; a.ovf = S.ovf; a.ovf2 = S.ovf2; a.gstack = S.gstack;
+    // the fast pass writes unordered rows; sorted rows come from the heap pass
+    a.use_queue = sorted ? 0 : 1;
     int pass = 0;
+    S.stat_queued = 0;
+    S.stat_queued2 = 0;
     while (true) {
         if (pass >= max_passes) {
             gf_set_error("kNN: particles still unresolved after %d passes", max_passes);
             return -1;
         }
         hipLaunchKernelGGL(k_knn_reset, dim3(1), dim3(64), 0, s, S.state);
+        if (!sorted)
+            hipLaunchKernelGGL(k_knn_fast, dim3(FAST_BLOCKS), dim3(FT), 0, s, a);
         if (kq <= 64) {
             hipLaunchKernelGGL((k_knn_pass<64, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
             hipLaunchKernelGGL((k_knn_pass<64, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);
@@ -699,13 +1100,15 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
         }
         GF_LAUNCH_CHECK();
         ++pass;
-        int st[5];
+        int st[8];
         GF_HIP(hipMemcpyAsync(st, S.state, sizeof(st), hipMemcpyDeviceToHost, s));
         GF_HIP(hipStreamSynchronize(s));
         if (st[4] != 0) {
             gf_set_error("kNN: %d groups overflowed the global stack", st[4]);
             return -1;
         }
+        S.stat_queued += st[3];
+        S.stat_queued2 += st[6];
         if (st[0] == 0) break;
     }
     S.last_passes = pass;
@@ -768,6 +1171,39 @@ int gf_sph_export_knn(GfSph& S, int kq, int64_t* nb, double* dist, hipStream_t s
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_export_knn, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, kq, S.nb,
                        S.tree.tx, S.tree.ty, S.tree.tz, S.tree.idx, S.inv, nb, dist);
+    GF_LAUNCH_CHECK();
+    return 0;
+}
+
+__global__ __launch_bounds__(TB) void k_scatter_dk(int n, const double* __restrict__ dk,
+                                                   const uint32_t* __restrict__ idx,
+                                                   double* __restrict__ out) {
+    const int t = blockIdx.x * TB + threadIdx.x;
+    if (t < n) out[idx[t]] = dk[t];
+}
+
+__global__ __launch_bounds__(TB) void k_export_table(int n, int k, const int* __restrict__ nb,
+                                                     const uint32_t* __restrict__ idx,
+                                                     int64_t* __restrict__ out) {
+    const int t = blockIdx.x * TB + threadIdx.x;
+    if (t >= n) return;
+    const size_t row = size_t(idx[t]) * k;
+    for (int s = 0; s < k; ++s) out[row + s] = int64_t(idx[nb[size_t(s) * n + t]]);
+}
+
+int gf_sph_dk_out(GfSph& S, double* dk, hipStream_t s) {
+    const int n = S.tree.n;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_scatter_dk, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, S.dk, S.tree.idx, dk);
+    GF_LAUNCH_CHECK();
+    return 0;
+}
+
+int gf_sph_table_out(GfSph& S, int k, int64_t* nb, hipStream_t s) {
+    const int n = S.tree.n;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_export_table, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, k, S.nb,
+                       S.tree.idx, nb);
     GF_LAUNCH_CHECK();
     return 0;
 }

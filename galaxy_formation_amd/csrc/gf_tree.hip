@@ -337,15 +337,32 @@ __global__ void k_depth_offsets(int* __restrict__ dcount, int* __restrict__ doff
     doff[GF_KEY_LEVELS + 1] = run;
 }
 
+// Chunks of TB nodes: ranks inside a chunk from LDS counters, one global
+// atomic per depth and chunk reserves the chunk's slots (order inside a depth
+// does not matter).
 __global__ __launch_bounds__(TB) void k_depth_scatter(const int* __restrict__ counters,
                                                       const int8_t* __restrict__ depth,
                                                       const int* __restrict__ doff,
                                                       int* __restrict__ dcur,
                                                       int* __restrict__ dlist) {
+    __shared__ int cnt[GF_KEY_LEVELS + 1];
+    __shared__ int base[GF_KEY_LEVELS + 1];
     const int nn = counters[0];
-    for (int id = blockIdx.x * TB + threadIdx.x; id < nn; id += gridDim.x * TB) {
-        const int d = depth[id];
-        dlist[doff[d] + atomicAdd(&dcur[d], 1)] = id;
+    for (int c0 = blockIdx.x * TB; c0 < nn; c0 += gridDim.x * TB) {
+        if (threadIdx.x <= GF_KEY_LEVELS) cnt[threadIdx.x] = 0;
+        __syncthreads();
+        const int id = c0 + threadIdx.x;
+        int d = -1, r = 0;
+        if (id < nn) {
+            d = depth[id];
+            r = atomicAdd(&cnt[d], 1);
+        }
+        __syncthreads();
+        if (threadIdx.x <= GF_KEY_LEVELS && cnt[threadIdx.x] > 0)
+            base[threadIdx.x] = doff[threadIdx.x] + atomicAdd(&dcur[threadIdx.x], cnt[threadIdx.x]);
+        __syncthreads();
+        if (d >= 0) dlist[base[d] + r] = id;
+        __syncthreads();
     }
 }
 

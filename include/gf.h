@@ -147,6 +147,18 @@ int gf_hydro_knn_dev(int k, const double* r_init, void* stream);
 int gf_hydro_forces_dev(double eps, double gamma, double alpha, double beta, double* h,
                         double* rho, double* acc, double* dudt, double* dt, double* divv,
                         void* stream);
+/* d_k (distance to the k-th neighbour, itself included) of the last
+ * device-resident solve, input order, into a device array. */
+int gf_hydro_dk_dev(double* dk, void* stream);
+/* The neighbour table of the last device-resident solve as host arrays:
+ * nb (n, k) int64 input indices, rows unordered; dk (n) (may be NULL).
+ * Returns k. */
+int gf_hydro_neighbours(int64_t* nb, double* dk);
+/* Statistics of the last neighbour solve: out[0] pass launches, [1] groups
+ * the fast pass handed to the heap pass (summed over passes), [2] groups
+ * handed on to the global-stack pass, [3] gas-tree nodes, [4] gas groups,
+ * [5] gas particles (len entries are written, at most 6). */
+int gf_hydro_stats(long long* out, int len);
 /* Passes each particle of the last solve took (host array, input order; may
  * be NULL) and their total.  Returns the number of pass launches. */
 int gf_hydro_passes(int* per_particle, long long* total);

@@ -220,3 +220,48 @@ def test_special_cases():
     sph.sph_forces(pos2, vel2, np.ones(2), nb2, h2, u2, GAMMA, 1.0, 2.0, r2, a2, du2, dt2)
     ci = np.sqrt(GAMMA * (GAMMA - 1))
     np.testing.assert_allclose(dt2, 0.3 / (ci + 1.2 * ci), rtol=1e-14)   # mu_max = 0
+
+
+def _device_solve(pos, k, r_init=None):
+    """The device-resident path (fast pass + fallbacks): unordered neighbour
+    sets, d_k and the number of passes."""
+    import torch
+
+    from galaxy_formation_amd import _lib
+    L = _lib.load()
+    dev = torch.device("cuda:0")
+    n = len(pos)
+    tp = torch.from_numpy(np.ascontiguousarray(pos)).to(dev)
+    tm = torch.ones(n, dtype=torch.float64, device=dev)
+    r0 = None if r_init is None else torch.from_numpy(np.ascontiguousarray(r_init)).to(dev)
+    torch.cuda.synchronize()
+    _lib.check(L.gf_hydro_setup_dev(n, tp.data_ptr(), None, tm.data_ptr(), None, None))
+    passes = _lib.check(L.gf_hydro_knn_dev(k, None if r0 is None else r0.data_ptr(), None))
+    nb = np.empty((n, k), np.int64)
+    dk = np.empty(n)
+    _lib.check(L.gf_hydro_neighbours(nb, _lib.ptr(dk)))
+    return nb, dk, passes
+
+
+@pytest.mark.parametrize("k,scale", [(32, None), (64, 1.0), (64, 0.4), (64, 3.0), (96, None)])
+def test_device_neighbour_sets_equal_bruteforce(k, scale):
+    pos, *_ = disk_gas(20000, seed=11)
+    od, onb = O.knn(pos, k)
+    r0 = None if scale is None else scale * od[:, -1]
+    nb, dk, passes = _device_solve(pos, k, r0)
+    np.testing.assert_array_equal(np.sort(nb, axis=1), np.sort(onb, axis=1))
+    np.testing.assert_array_equal(dk, od[:, -1])
+    if scale is not None and scale < 1:
+        assert passes > 1
+
+
+def test_device_neighbours_with_exact_ties():
+    """A perfect lattice: many neighbours at exactly equal distances, broken
+    by index as the brute-force order (d^2, index) does."""
+    ax = np.arange(9, dtype=np.float64)
+    pos = np.ascontiguousarray(np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3))
+    for k in (32, 64):
+        od, onb = O.knn(pos, k)
+        nb, dk, _ = _device_solve(pos, k)
+        np.testing.assert_array_equal(np.sort(nb, axis=1), np.sort(onb, axis=1))
+        np.testing.assert_array_equal(dk, od[:, -1])
