@@ -11,7 +11,12 @@ Two paths are timed on the same state:
             estimated from the gas tree's cell densities;
   fast      FP32 pair arithmetic (FP64 tree, positions and centres of mass);
             the neighbour solve starts from the learned radius proposal
-            (or, without a trained network, from d_k of the previous step).
+            (galaxy_formation_amd/data/learned_h_k64.npz, trained on the
+            rotating-cloud runs by galaxy_formation_amd.train_learned_h).
+The state one long step earlier (drifted back by the 10th percentile of the
+admissible time steps) supplies d_k, div v and dt for the proposal.  The
+neighbour solve is also timed from the other first radii (cell densities,
+d_k of the previous step, d_k of the previous step with a fixed margin).
 Phases are timed with HIP events (torch.cuda.Event) on the stream the
 library runs on.
 
@@ -48,6 +53,8 @@ def parse():
     p.add_argument("--theta", type=float, default=0.5)
     p.add_argument("--k", type=int, default=64)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--model", default=os.path.join(ROOT, "galaxy_formation_amd", "data",
+                                                  "learned_h_k64.npz"))
     return p.parse_args()
 
 
@@ -69,6 +76,8 @@ class Bench:
         self.pos, self.vel, self.mass, self.u = T(c.pos), T(c.vel), T(c.mass), T(c.u)
         z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.dev)  # noqa: E731
         self.acc_g = {0: z(self.n, 3), 1: z(self.n, 3)}
+        self.cell_rho, self.cell_count, self.r_learned = z(self.ng), z(self.ng), z(self.ng)
+        self.dk_prev, self.divv_prev, self.dtc_prev, self.D = z(self.ng), z(self.ng), z(self.ng), 0.0
         self.out = {p: dict(h=z(self.ng), rho=z(self.ng), acc=z(self.ng, 3), dudt=z(self.ng),
                             dt=z(self.ng), divv=z(self.ng)) for p in (0, 1)}
         # a stream of its own (default priority): the library then runs on the
@@ -80,12 +89,23 @@ class Bench:
     def p(self, t):
         return None if t is None else t.data_ptr()
 
-    def hydro(self, prec, pos, r_init, ev=None, rec=None):
+    def hydro(self, prec, pos, first, ev=None, rec=None):
+        """first: None (cell densities), ("radius", r) or ("learned", model)."""
         L, chk, sp, o = self.L, self._lib.check, self.sp, self.out[prec]
         rec(2) if rec else None
         chk(L.gf_hydro_setup_dev(self.ng, self.p(pos), self.p(self.vel), self.p(self.mass),
                                  self.p(self.u), sp))
         rec(3) if rec else None
+        r_init = None
+        if first is not None and first[0] == "radius":
+            r_init = first[1]
+        elif first is not None and first[0] == "learned":
+            model = first[1]
+            chk(L.gf_hydro_cell_density_dev(self.a.k, self.p(self.cell_rho),
+                                            self.p(self.cell_count), sp))
+            x = model.features(self.a.k, self.dk_prev, self.cell_rho, self.cell_count,
+                               self.divv_prev, self.dtc_prev, self.D, stream=self.stream)
+            r_init = model.propose(x, self.dk_prev, out=self.r_learned, stream=self.stream)
         passes = chk(L.gf_hydro_knn_dev(self.a.k, self.p(r_init), sp))
         rec(4) if rec else None
         chk(L.gf_hydro_forces_dev(self.c.eps, GAMMA, ALPHA, BETA,
@@ -94,7 +114,7 @@ class Bench:
         rec(5) if rec else None
         return passes
 
-    def step(self, prec, r_init, ev=None):
+    def step(self, prec, first, ev=None):
         L, chk, sp = self.L, self._lib.check, self.sp
 
         def rec(i):
@@ -104,26 +124,27 @@ class Bench:
         chk(L.gf_tree_build_dev(self.n, self.p(self.pos), self.p(self.mass), self.a.theta, sp))
         rec(1)
         chk(L.gf_walk_dev(self.n, None, self.p(self.acc_g[prec]), self.c.eps, prec, sp))
-        return self.hydro(prec, self.pos, r_init, ev, rec)
+        return self.hydro(prec, self.pos, first, ev, rec)
 
-    def timed(self, prec, r_init):
+    def timed(self, prec, first, steps=None):
         torch = self.torch
+        steps = steps or self.a.steps
         for _ in range(self.a.warmup):
-            self.step(prec, r_init)
+            self.step(prec, first)
         torch.cuda.synchronize()
-        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(self.a.steps)]
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(steps)]
         t0 = torch.cuda.Event(enable_timing=True)
         t1 = torch.cuda.Event(enable_timing=True)
         passes = []
         w0 = time.perf_counter()
         t0.record(self.stream)
-        for k in range(self.a.steps):
-            passes.append(self.step(prec, r_init, evs[k]))
+        for k in range(steps):
+            passes.append(self.step(prec, first, evs[k]))
         t1.record(self.stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - w0
         self._lib.check(self.L.gf_sync(self.sp))
-        res = {"ms": t0.elapsed_time(t1) / self.a.steps, "wall_ms": 1e3 * wall / self.a.steps,
+        res = {"ms": t0.elapsed_time(t1) / steps, "wall_ms": 1e3 * wall / steps,
                "passes": int(np.max(passes))}
         for i, name in enumerate(PHASES):
             res[name + "_ms"] = float(np.mean([e[i].elapsed_time(e[i + 1]) for e in evs]))
@@ -164,17 +185,31 @@ def main():
     D = float(np.percentile(dt_ind, 10))
     pos_prev = B.pos[:ng] - D * B.vel[:ng]
     B.hydro(1, pos_prev, None)
-    dk_prev = torch.zeros(ng, dtype=torch.float64, device=B.dev)
-    B._lib.check(B.L.gf_hydro_dk_dev(B.p(dk_prev), B.sp))
+    B._lib.check(B.L.gf_hydro_dk_dev(B.p(B.dk_prev), B.sp))
+    B.divv_prev.copy_(B.out[1]["divv"])
+    B.dtc_prev.copy_(B.out[1]["dt"])
+    B.D = D
     torch.cuda.synchronize()
-    r_hprev = dk_prev.contiguous()
+
+    from galaxy_formation_amd.learned_h import LearnedH
+    model = LearnedH().load(a.model)
 
     res = {"plain_fp64": B.timed(1, None)}
     h_plain = B.out[1]["h"].cpu().numpy()
     rho_plain = B.out[1]["rho"].cpu().numpy()
-    res["fast_fp32"] = B.timed(0, r_hprev)
+    res["fast_fp32"] = B.timed(0, ("learned", model))
     h_fast = B.out[0]["h"].cpu().numpy()
     rho_fast = B.out[0]["rho"].cpu().numpy()
+    # the neighbour solve from the other first radii (FP32 path, 3 steps each)
+    ab = {"learned": {k: res["fast_fp32"][k] for k in ("knn_ms", "passes",
+                                                        "mean_passes_per_particle")}}
+    import math
+    for name, first in (("cell_density", None), ("dk_prev", ("radius", B.dk_prev)),
+                        ("dk_prev_fixed_margin",
+                         ("radius", (B.dk_prev * math.exp(model.margin_fixed)).contiguous()))):
+        r = B.timed(0, first, steps=3)
+        ab[name] = {k: r[k] for k in ("knn_ms", "passes", "mean_passes_per_particle")}
+        ab[name]["h_equal_plain"] = bool(np.array_equal(B.out[0]["h"].cpu().numpy(), h_plain))
 
     # accuracy: gravity on a seeded 2,000-particle subset vs the FP64 direct sum
     sub = np.sort(np.random.default_rng(0).choice(n, min(n, 2000), replace=False))
@@ -213,7 +248,7 @@ def main():
         "config": {"workload": f"{a.workload}: {n} particles ({ng} gas), all active, "
                                "gravity + SPH (kNN, h, density, forces)",
                    "n": n, "n_gas": ng, "theta": a.theta, "eps": c.eps, "k": a.k,
-                   "first_radius_fast": "d_k of the previous step"},
+                   "first_radius_fast": "learned proposal"},
         "roofline": {"bound": "valu-fp32", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
                      "kernel": "k_walk<false, false> (FP32 group walk + list evaluation)",
@@ -221,6 +256,8 @@ def main():
         "plain_fp64": {"value": n / (plain["ms"] * 1e-3), **plain},
         "fast_fp32": fast,
         "speedup_fast_vs_plain": plain["ms"] / fast["ms"],
+        "first_radius_ab": ab,
+        "learned_margin": model.margin, "fixed_margin": model.margin_fixed,
         "previous_step_D": D,
     }
     if not a.no_cpu_baseline:

@@ -56,6 +56,17 @@ SIGNATURES = {
     "gf_hydro_passes": (_ci, [_vp, _vp]),
     "gf_hydro_dk_dev": (_ci, [_vp, _vp]),
     "gf_hydro_stats": (_ci, [_i64, _ci]),
+    # learned smoothing length
+    "gf_mlp_params": (_ci, []),
+    "gf_hydro_cell_density_dev": (_ci, [_ci, _vp, _vp, _vp]),
+    "gf_mlp_features_dev": (_ci, [_ci, _ci, _vp, _vp, _vp, _vp, _vp, _cf, _vp, _vp, _vp, _vp]),
+    "gf_mlp_forward_dev": (_ci, [_ci, _vp, _vp, _vp, _vp]),
+    "gf_mlp_propose_dev": (_ci, [_ci, _vp, _vp, _vp, _cf, _vp, _vp]),
+    "gf_mlp_grad_dev": (_ci, [_ci, _vp, _vp, _vp, _cf, _vp, _vp, _vp]),
+    "gf_mlp_train_step_dev": (_ci, [_ci, _vp, _vp, _vp, _vp, _vp, _ci, _cf, _cf, _cf, _cf, _cf,
+                                    _vp, _vp]),
+    "gf_mlp_grad_host": (_ci, [_ci, np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS"), _f32,
+                               _f32, _cf, _f32, _vp]),
     "gf_hydro_neighbours": (_ci, [_i64, _vp]),
 }
 

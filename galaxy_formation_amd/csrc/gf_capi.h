@@ -3,6 +3,7 @@
 #include "../../include/gf.h"
 #include "gf_gravity.h"
 #include "gf_sph.h"
+#include "gf_mlp.h"
 
 #define GF_ABI_VERSION 1
 
@@ -36,6 +37,8 @@ struct GfCtx {
     double* d_dist = nullptr; size_t d_dist_cap = 0;
     int* d_scratch = nullptr; size_t d_scratch_cap = 0;
     int* d_flag = nullptr; size_t d_flag_cap = 0;
+    float* d_mlp = nullptr; size_t d_mlp_cap = 0;
+    float* d_mlp_part = nullptr; size_t d_mlp_part_cap = 0;
 };
 
 extern GfCtx g_ctx;

@@ -45,9 +45,10 @@ int gf_ctx_init() {
     return 0;
 }
 
-hipStream_t gf_stream(void* s) {
-    return s ? reinterpret_cast<hipStream_t>(s) : g_ctx.stream;
-}
+// Stream of a *_dev call: the caller's; NULL is HIP's default stream (the
+// one PyTorch's default stream is), so device-array callers stay ordered
+// with their own work.
+hipStream_t gf_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 int gf_stage_reserve(void** p, size_t* cap, size_t bytes) {
     if (bytes <= *cap && *p) return 0;
@@ -559,6 +560,81 @@ int gf_hydro_passes(int* per_particle, long long* total) {
     }
     if (total) *total = t;
     return g_ctx.sph.last_passes;
+}
+
+// ------------------------------------------------------ learned h (MLP)
+
+int gf_hydro_cell_density_dev(int k, double* out, double* out_count, void* stream) {
+    if (gf_ctx_init()) return -1;
+    return gf_sph_cell_density(g_ctx.sph, k, out, out_count, gf_stream(stream));
+}
+
+int gf_mlp_params(void) { return GF_MLP_PARAMS; }
+
+int gf_mlp_forward_dev(int n, const uint16_t* x, const float* params, float* out, void* stream) {
+    if (gf_ctx_init()) return -1;
+    return gf_mlp_forward(n, x, params, out, gf_stream(stream));
+}
+
+int gf_mlp_grad_dev(int n, const uint16_t* x, const float* target, const float* params,
+                    float tau, float* grad, float* loss, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (STAGE(g_ctx.d_mlp_part, sizeof(float) * gf_mlp_partial_floats(n))) return -1;
+    return gf_mlp_grad(n, n, x, target, params, tau, grad, loss, g_ctx.d_mlp_part,
+                       gf_stream(stream));
+}
+
+int gf_mlp_train_step_dev(int n, const uint16_t* x, const float* target, float* params, float* m,
+                          float* v, int step, float lr, float beta1, float beta2, float eps,
+                          float tau, float* loss, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (step < 1) {
+        gf_set_error("gf_mlp_train_step_dev: step counts from 1");
+        return -1;
+    }
+    if (STAGE(g_ctx.d_mlp_part, sizeof(float) * gf_mlp_partial_floats(n))) return -1;
+    return gf_mlp_train_step(n, x, target, params, m, v, step, lr, beta1, beta2, eps, tau, loss,
+                             g_ctx.d_mlp_part, gf_stream(stream));
+}
+
+int gf_mlp_features_dev(int n, int k, const double* dk_prev, const double* cell_rho,
+                        const double* cell_count, const double* divv, const double* dtc_prev,
+                        float dt, const float* norm, uint16_t* x, float* raw, void* stream) {
+    if (gf_ctx_init()) return -1;
+    return gf_mlp_features(n, k, dk_prev, cell_rho, cell_count, divv, dtc_prev, dt, norm, x, raw,
+                           gf_stream(stream));
+}
+
+int gf_mlp_propose_dev(int n, const uint16_t* x, const float* params, const double* dk_prev,
+                       float margin, double* r, void* stream) {
+    if (gf_ctx_init()) return -1;
+    return gf_mlp_propose(n, x, params, dk_prev, margin, r, gf_stream(stream));
+}
+
+// host-pointer gradient (tests)
+int gf_mlp_grad_host(int n, const uint16_t* x, const float* target, const float* params,
+                     float tau, float* grad, float* loss) {
+    if (gf_ctx_init()) return -1;
+    hipStream_t s = g_ctx.stream;
+    const size_t bx = sizeof(uint16_t) * 4 * size_t(n), bt = sizeof(float) * size_t(n);
+    const size_t bp = sizeof(float) * GF_MLP_PARAMS;
+    const size_t need = bx + bt + 2 * bp + 64;
+    if (STAGE(g_ctx.d_mlp, need) || STAGE(g_ctx.d_mlp_part, sizeof(float) * gf_mlp_partial_floats(n)))
+        return -1;
+    char* base = reinterpret_cast<char*>(g_ctx.d_mlp);
+    float* dp = reinterpret_cast<float*>(base);
+    float* dg = dp + GF_MLP_PARAMS;
+    float* dl = dg + GF_MLP_PARAMS;
+    float* dt = dl + 4;
+    uint16_t* dx = reinterpret_cast<uint16_t*>(dt + n);
+    GF_HIP(hipMemcpyAsync(dp, params, bp, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(dt, target, bt, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(dx, x, bx, hipMemcpyHostToDevice, s));
+    if (gf_mlp_grad(n, n, dx, dt, dp, tau, dg, dl, g_ctx.d_mlp_part, s)) return -1;
+    GF_HIP(hipMemcpyAsync(grad, dg, bp, hipMemcpyDeviceToHost, s));
+    if (loss) GF_HIP(hipMemcpyAsync(loss, dl, sizeof(float), hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    return 0;
 }
 
 }  // extern "C"

@@ -70,6 +70,10 @@ size_t gf_sph_table_scratch_bytes(int n, int k);
 // and (n, k) distances, rows in input order (requires a sorted solve).
 int gf_sph_export_knn(GfSph& S, int kq, int64_t* nb, double* dist, hipStream_t s);
 
+// Number density of the smallest gas-tree cell holding >= kq particles
+// around each gas particle, and (out_count, may be null) that cell's particle
+// count (input order, device arrays of n).
+int gf_sph_cell_density(GfSph& S, int kq, double* out, double* out_count, hipStream_t s);
 // d_k of the last solve, input order (device array of n).
 int gf_sph_dk_out(GfSph& S, double* dk, hipStream_t s);
 // Tree-position table of the last solve as (n, k) int64 input indices, rows

@@ -333,7 +333,7 @@ __global__ void k_knn_reset(int* state) {
 // grows the radius; one whose sphere holds more candidates than the list
 // shrinks it; both take another pass.
 constexpr int FT = 256;
-constexpr int CMAX = 2048;
+constexpr int CMAX = 1024;
 constexpr int CPL = CMAX / 64;
 constexpr int CQ = CMAX / 4;                 // per-wave list in member mode
 constexpr int FSCAP = 1024;                  // per-wave stack
@@ -362,7 +362,7 @@ __device__ int containing_node(const KnnArgs& a, int node, double bx, double by,
 // depth-capped leaf.
 __device__ int gather_sphere(const KnnArgs& a, int root, double bx, double by, double bz,
                              double R2, double* lx, double* ly, double* lz, int* li, int cap,
-                             int* stack) {
+                             int* stack, int* sc) {
 #pragma clang fp contract(off)
     const int lane = __lane_id();
     const uint64_t lt = gf_lanemask_lt();
@@ -405,13 +405,27 @@ __device__ int gather_sphere(const KnnArgs& a, int root, double bx, double by, d
         }
         sn += ptot;
         if (__ballot(cntn > GF_LEAF_SIZE) != 0) return -2;   // a depth-capped leaf
-        for (int k = 0; k < GF_LEAF_SIZE; ++k) {
-            if (__ballot(k < cntn) == 0) break;
+        // the touched leaves' particles, one per lane: lane l's leaf owns
+        // entries [inc_l - cntn_l, inc_l) of this batch (owners looked up in
+        // the wave's LDS scratch, so no shuffle reads an inactive lane)
+        const int einc = gf_wave_incl_scan(cntn);
+        const int etot = __shfl(einc, 63, 64);
+        sc[lane] = einc;
+        sc[64 + lane] = first;
+        sc[128 + lane] = cntn;
+        __builtin_amdgcn_wave_barrier();
+        for (int e0 = 0; e0 < etot; e0 += 64) {
+            const int e = e0 + lane;
             bool tk = false;
             double px = 0, py = 0, pz = 0;
             int pj = 0;
-            if (k < cntn) {
-                pj = first + k;
+            if (e < etot) {
+                int lo = 0, hi = 63;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (sc[mid] > e) hi = mid; else lo = mid + 1;
+                }
+                pj = sc[64 + lo] + (e - (sc[lo] - sc[128 + lo]));
                 px = a.tx[pj]; py = a.ty[pj]; pz = a.tz[pj];
                 const double dx = px - bx, dy = py - by, dz = pz - bz;
                 tk = dx * dx + dy * dy + dz * dz <= R2;
@@ -424,6 +438,7 @@ __device__ int gather_sphere(const KnnArgs& a, int root, double bx, double by, d
             }
             nc += __popcll(m);
         }
+        __builtin_amdgcn_wave_barrier();
         if (nc > cap) return -nc;
         __builtin_amdgcn_wave_barrier();
     }
@@ -628,7 +643,8 @@ __global__ __launch_bounds__(FT) void k_knn_fast(KnnArgs a) {
                 mode = 2;
                 if (R <= 2.0 * rmax) {
                     const int root = containing_node(a, G, bx, by, bz, R);
-                    nc = gather_sphere(a, root, bx, by, bz, R * R, cx, cy, cz, ci, CMAX, stack[0]);
+                    nc = gather_sphere(a, root, bx, by, bz, R * R, cx, cy, cz, ci, CMAX, stack[0],
+                                       hist[0]);
                     mode = nc >= 0 ? 1 : (nc == -2 ? 3 : 2);   // a full list: member mode
                 }
             }
@@ -655,7 +671,8 @@ __global__ __launch_bounds__(FT) void k_knn_fast(KnnArgs a) {
                     const double xi = a.tx[t], yi = a.ty[t], zi = a.tz[t];
                     const double ri = a.r[t] * (1.0 + 1e-12) + 1e-300;
                     const int root = containing_node(a, a.leaf_of[t], xi, yi, zi, ri);
-                    nc = gather_sphere(a, root, xi, yi, zi, ri * ri, lx, ly, lz, li, CQ, stack[w]);
+                    nc = gather_sphere(a, root, xi, yi, zi, ri * ri, lx, ly, lz, li, CQ, stack[w],
+                                       hist[w]);
                     if (nc == -2) {
                         handoff = 1;
                         continue;
@@ -1189,6 +1206,37 @@ __global__ __launch_bounds__(TB) void k_export_table(int n, int k, const int* __
     if (t >= n) return;
     const size_t row = size_t(idx[t]) * k;
     for (int s = 0; s < k; ++s) out[row + s] = int64_t(idx[nb[size_t(s) * n + t]]);
+}
+
+// Number density of the smallest gas-tree cell around each particle that
+// holds >= kq particles (the plain first-radius estimate), input order.
+__global__ __launch_bounds__(TB) void k_cell_density(int n, int kq, const uint32_t* __restrict__ idx,
+                                                     const int* __restrict__ leaf_of,
+                                                     const int* __restrict__ parent,
+                                                     const int* __restrict__ count,
+                                                     const double4* __restrict__ box,
+                                                     double* __restrict__ out,
+                                                     double* __restrict__ out_count) {
+    const int t = blockIdx.x * TB + threadIdx.x;
+    if (t >= n) return;
+    int node = leaf_of[t];
+    while (count[node] < kq && parent[node] >= 0) node = parent[node];
+    const double side = 2.0 * box[node].w;
+    out[idx[t]] = double(count[node]) / (side * side * side);
+    if (out_count) out_count[idx[t]] = double(count[node]);
+}
+
+int gf_sph_cell_density(GfSph& S, int kq, double* out, double* out_count, hipStream_t s) {
+    const int n = S.tree.n;
+    if (n == 0) return 0;
+    if (sph_reserve(S, n, 0)) return -1;
+    GfTree& T = S.tree;
+    hipLaunchKernelGGL(k_leaf_of, dim3(std::min(2048, gf_div_up(T.cap_nodes, TB))), dim3(TB), 0, s,
+                       T.counters, T.start, T.count, T.leaf, S.leaf_of);
+    hipLaunchKernelGGL(k_cell_density, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, kq, T.idx,
+                       S.leaf_of, T.parent, T.count, T.box, out, out_count);
+    GF_LAUNCH_CHECK();
+    return 0;
 }
 
 int gf_sph_dk_out(GfSph& S, double* dk, hipStream_t s) {

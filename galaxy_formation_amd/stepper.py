@@ -1,0 +1,193 @@
+"""Device-resident force evaluation and a leapfrog driver over libgf.so.
+
+ForceEvaluator reproduces the call pattern of the reference's
+GalaxyFormation.accelerations / _hydro (galaxy_formation/simulation.py:61-95):
+Barnes-Hut gravity of every particle, then the gas: neighbours, h,
+density, pressure + viscosity forces, Courant time steps, and
+dt_ind = min(0.3 sqrt(eps / |a|), dt_courant) (simulation.py:68-72).  State
+stays in device memory (PyTorch tensors); all physics runs in libgf.so.
+
+The neighbour solve can start from three first radii:
+  "plain"    the gas tree's cell densities (no history);
+  "prev"     d_k of the previous evaluation;
+  "learned"  the MLP proposal of learned_h.LearnedH.
+The converged neighbours, h and densities do not depend on the choice.
+
+Stepper integrates with a global kick-drift-kick leapfrog whose step is the
+10th percentile of the particles' admissible steps (the long step of
+simulation.py:117) times a factor; it records what the learned proposal
+trains on.  (The reference's two-level sub-stepping, cooling and star
+formation are not part of this force path.)
+"""
+
+import numpy as np
+
+from . import _lib
+from .ics import U_PER_K
+
+GAMMA, ALPHA, BETA = 5.0 / 3.0, 1.0, 2.0
+
+
+class ForceEvaluator:
+    def __init__(self, n, n_gas, eps, theta=0.5, k=64, precision="fp32", device="cuda:0",
+                 stream=None):
+        import torch
+        self.torch = torch
+        self.L = _lib.load()
+        self.dev = torch.device(device)
+        self.n, self.ng = n, n_gas
+        self.eps, self.theta, self.k = eps, theta, k
+        self.prec = {"fp32": 0, "fp64": 1}[precision]
+        self.stream = stream
+        z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.dev)  # noqa: E731
+        self.acc_grav = z(n, 3)
+        self.acc = z(n, 3)
+        g = max(n_gas, 1)
+        self.h, self.rho, self.acc_sph = z(g), z(g), z(g, 3)
+        self.dudt, self.dt_c, self.divv = z(g), z(g), z(g)
+        self.dk, self.cell_rho, self.cell_count, self.r0 = z(g), z(g), z(g), z(g)
+        self.dt_ind = z(n)
+        self.passes = 0
+
+    @property
+    def sp(self):
+        if self.stream is not None:
+            return self.stream.cuda_stream
+        return self.torch.cuda.current_stream().cuda_stream or None
+
+    def gravity(self, pos, mass, active=None, rebuild=True, ev=None):
+        chk, L = _lib.check, self.L
+        if ev is not None:
+            ev[0].record()
+        chk(L.gf_tree_build_dev(self.n, pos.data_ptr(), mass.data_ptr(), self.theta, self.sp)
+            if rebuild else
+            L.gf_gravity_dev(self.n, pos.data_ptr(), mass.data_ptr(), self.acc_grav.data_ptr(),
+                             self.theta, self.eps, None if active is None else active.data_ptr(),
+                             0, self.prec, self.sp))
+        if ev is not None:
+            ev[1].record()
+        if rebuild:
+            chk(L.gf_walk_dev(self.n, None if active is None else active.data_ptr(),
+                              self.acc_grav.data_ptr(), self.eps, self.prec, self.sp))
+        if ev is not None:
+            ev[2].record()
+
+    def hydro(self, pos, vel, mass, u, first="plain", dk_prev=None, divv_prev=None,
+              dtc_prev=None, dt=None, model=None, ev=None):
+        """Gas of the first n_gas particles.  Returns the number of passes."""
+        chk, L, ng = _lib.check, self.L, self.ng
+        if ng == 0:
+            return 0
+        chk(L.gf_hydro_setup_dev(ng, pos.data_ptr(), vel.data_ptr(), mass.data_ptr(),
+                                 u.data_ptr(), self.sp))
+        if ev is not None:
+            ev[3].record()
+        r0 = None
+        if first == "prev":
+            r0 = dk_prev
+        elif first == "learned":
+            self.cell_density()
+            x = model.features(self.k, dk_prev, self.cell_rho, self.cell_count, divv_prev,
+                               dtc_prev, dt, stream=self.stream)
+            r0 = model.propose(x, dk_prev, out=self.r0, stream=self.stream)
+        elif first != "plain":
+            raise ValueError(f"unknown first-radius policy {first!r}")
+        self.passes = chk(L.gf_hydro_knn_dev(self.k, None if r0 is None else r0.data_ptr(),
+                                             self.sp))
+        if ev is not None:
+            ev[4].record()
+        chk(L.gf_hydro_forces_dev(self.eps, GAMMA, ALPHA, BETA, self.h.data_ptr(),
+                                  self.rho.data_ptr(), self.acc_sph.data_ptr(),
+                                  self.dudt.data_ptr(), self.dt_c.data_ptr(),
+                                  self.divv.data_ptr(), self.sp))
+        chk(L.gf_hydro_dk_dev(self.dk.data_ptr(), self.sp))
+        if ev is not None:
+            ev[5].record()
+        return self.passes
+
+    def cell_density(self):
+        """Density and particle count of the smallest gas-tree cell holding
+        >= k particles around each gas particle (current gas tree)."""
+        _lib.check(self.L.gf_hydro_cell_density_dev(self.k, self.cell_rho.data_ptr(),
+                                                    self.cell_count.data_ptr(), self.sp))
+        return self.cell_rho, self.cell_count
+
+    def evaluate(self, pos, vel, mass, u, **hydro_kw):
+        """acc = gravity + SPH; dt_ind (simulation.py:61-72)."""
+        torch = self.torch
+        self.gravity(pos, mass)
+        passes = self.hydro(pos[:self.ng], vel[:self.ng], mass[:self.ng], u[:self.ng], **hydro_kw)
+        self.acc.copy_(self.acc_grav)
+        if self.ng:
+            self.acc[:self.ng] += self.acc_sph
+        amag = torch.linalg.vector_norm(self.acc, dim=1)
+        self.dt_ind = 0.3 * torch.sqrt(self.eps / torch.clamp(amag, min=1e-30))
+        if self.ng:
+            self.dt_ind[:self.ng] = torch.minimum(self.dt_ind[:self.ng], self.dt_c)
+        return passes
+
+
+class Stepper:
+    """Global-step KDK leapfrog over a ForceEvaluator, recording per step
+    (t, d_k, div v, Courant dt, cell density, cell count) of the gas for the
+    learned proposal."""
+
+    def __init__(self, particles, theta=0.5, k=64, precision="fp32", device="cuda:0"):
+        import torch
+        self.torch = torch
+        dev = torch.device(device)
+        T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+        p = particles
+        self.ng = int((p.kind == 0).sum())
+        assert (np.flatnonzero(p.kind == 0) == np.arange(self.ng)).all(), "gas first"
+        self.pos, self.vel, self.mass, self.u = T(p.pos), T(p.vel), T(p.mass), T(p.u)
+        self.u_floor = 1e4 * U_PER_K        # the cooling floor, T = 1e4 K (cooling.py:21)
+        self.F = ForceEvaluator(p.n, self.ng, p.eps, theta, k, precision, device)
+        self.t = 0.0
+        self.records = []
+
+    def _record(self):
+        rho, cnt = self.F.cell_density()
+        self.records.append((self.t, self.F.dk.clone(), self.F.divv.clone(), self.F.dt_c.clone(),
+                             rho.clone(), cnt.clone()))
+
+    def run(self, steps, dt_factor=1.0, first="plain", model=None):
+        torch, F = self.torch, self.F
+        F.evaluate(self.pos, self.vel, self.mass, self.u)
+        self._record()
+        passes = []
+        for _ in range(steps):
+            dt = float(dt_factor * torch.quantile(F.dt_ind, 0.10).item())
+            self.vel += 0.5 * dt * F.acc
+            self.pos += dt * self.vel
+            if self.ng:
+                self.u[:self.ng] = torch.clamp(self.u[:self.ng] + dt * F.dudt, min=self.u_floor)
+            dk_prev, divv_prev, dtc_prev = F.dk.clone(), F.divv.clone(), F.dt_c.clone()
+            passes.append(F.evaluate(self.pos, self.vel, self.mass, self.u, first=first,
+                                     dk_prev=dk_prev, divv_prev=divv_prev, dtc_prev=dtc_prev,
+                                     dt=dt, model=model))
+            self.vel += 0.5 * dt * F.acc
+            self.t += dt
+            self._record()
+        return passes
+
+
+def training_pairs(records, model, k, max_gap=4):
+    """Samples from a run: for snapshot pairs (s, s + g), g <= max_gap, the
+    features (d_k, div v and Courant dt at s, the cell at s + g, dt =
+    t_{s+g} - t_s) and the target log d_k(s + g) - log d_k(s).  Returns the
+    BF16 features normalised with the model's current statistics (n, 4), the
+    raw FP32 features (n, 4) and the targets (n,), all on the device."""
+    import torch
+    xs, raws, targets = [], [], []
+    for s in range(len(records)):
+        for g in range(1, max_gap + 1):
+            if s + g >= len(records):
+                break
+            t0, dk0, dv0, dtc0, _, _ = records[s]
+            t1, dk1, _, _, rho1, cnt1 = records[s + g]
+            x, raw = model.features(k, dk0, rho1, cnt1, dv0, dtc0, float(t1 - t0), raw=True)
+            xs.append(x)
+            raws.append(raw)
+            targets.append((torch.log(dk1) - torch.log(dk0)).float())
+    return torch.cat(xs), torch.cat(raws), torch.cat(targets)

@@ -8,7 +8,7 @@
  * float64; neighbour tables (n, k) int64 of local indices.  Host-pointer
  * entry points copy their inputs to the GPU, run on the library's stream and
  * return when the outputs are back in host memory.  *_dev entry points take
- * device pointers and a hipStream_t (NULL = the library's stream) and return
+ * device pointers and a hipStream_t (NULL = HIP's default stream) and return
  * without synchronising.  The library keeps state between calls (the last
  * gravity tree, staging buffers), like the reference's global tree
  * (cpp/nbody.cpp:307), and is not re-entrant.
@@ -162,6 +162,46 @@ int gf_hydro_stats(long long* out, int len);
 /* Passes each particle of the last solve took (host array, input order; may
  * be NULL) and their total.  Returns the number of pass launches. */
 int gf_hydro_passes(int* per_particle, long long* total);
+
+/* ------------------------------------------------ learned smoothing length
+ * No counterpart in the reference (new component): a 4 -> 32 -> 32 -> 1 SiLU
+ * MLP (GF_MLP_PARAMS = 1249 FP32 parameters: W1[32][4], b1, W2[32][32], b2,
+ * W3[32], b3) on 4 BF16 features per gas particle that proposes the first
+ * search radius of the neighbour solve; the plain radius iteration stays the
+ * judge.  All pointers are device arrays unless marked host. */
+
+/* Number of MLP parameters. */
+int gf_mlp_params(void);
+/* Density of the smallest gas-tree cell holding >= k particles around each
+ * gas particle of the last gf_hydro_setup_dev, and (out_count, may be NULL)
+ * that cell's particle count (input order). */
+int gf_hydro_cell_density_dev(int k, double* out, double* out_count, void* stream);
+/* Features of each gas particle, free of length and time scales:
+ * log(d_k prev / r_cell) with r_cell = (3k / (4 pi n_cell))^(1/3),
+ * log(cell count / k), div v(prev) * dt, log(dt / dt_Courant(prev));
+ * normalised by norm[0..3] (means) and norm[4..7] (standard deviations) ->
+ * x (n, 4) BF16; raw (may be NULL) receives the unnormalised FP32 values. */
+int gf_mlp_features_dev(int n, int k, const double* dk_prev, const double* cell_rho,
+                        const double* cell_count, const double* divv, const double* dtc_prev,
+                        float dt, const float* norm, uint16_t* x, float* raw, void* stream);
+/* Network output y for n samples. */
+int gf_mlp_forward_dev(int n, const uint16_t* x, const float* params, float* out, void* stream);
+/* First search radius r = d_k(prev) exp(y + margin). */
+int gf_mlp_propose_dev(int n, const uint16_t* x, const float* params, const double* dk_prev,
+                       float margin, double* r, void* stream);
+/* Gradient (grad: 1249 floats) of the mean loss over the n samples and the
+ * summed loss (loss: 1 float); either may be NULL.  tau <= 0: squared error
+ * 0.5 (y - target)^2 (summed loss: squared errors); 0 < tau < 1: pinball
+ * loss of the tau-quantile, max(tau (t - y), (1 - tau) (y - t)). */
+int gf_mlp_grad_dev(int n, const uint16_t* x, const float* target, const float* params,
+                    float tau, float* grad, float* loss, void* stream);
+/* One Adam step (step counts from 1) on the mean loss of n samples. */
+int gf_mlp_train_step_dev(int n, const uint16_t* x, const float* target, float* params, float* m,
+                          float* v, int step, float lr, float beta1, float beta2, float eps,
+                          float tau, float* loss, void* stream);
+/* gf_mlp_grad_dev with host arrays (x as uint16 BF16 bit patterns). */
+int gf_mlp_grad_host(int n, const uint16_t* x, const float* target, const float* params,
+                     float tau, float* grad, float* loss);
 
 #ifdef __cplusplus
 }
