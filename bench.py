@@ -170,6 +170,14 @@ def main():
         sys.exit(2)
     B = Bench(a)
     torch, n, ng, c = B.torch, B.n, B.ng, B.c
+    # a plain copy of known size before the timed region: the reference point
+    # for rocprofv3's FETCH_SIZE / WRITE_SIZE counters (256 MiB read + written)
+    cal = torch.empty(32 << 20, dtype=torch.float64, device=B.dev)
+    cal.fill_(1.0)
+    cal2 = torch.empty_like(cal)
+    cal2.copy_(cal)
+    torch.cuda.synchronize()
+    del cal, cal2
 
     # previous state of the gas: the same particles one step earlier (drifted
     # back by a time step D = 10th percentile of the particles' admissible

@@ -29,7 +29,7 @@
 namespace {
 
 constexpr int SCAP = 1024;                          // LDS stack entries
-constexpr int LCAP = 512;                           // LDS list entries
+constexpr int LCAP = 256;                           // LDS list entries
 constexpr int GSCAP = 512 * (GF_KEY_LEVELS + 2);    // global stack entries per block
 constexpr int WALK_BLOCKS = 8192;
 constexpr int GST_BLOCKS = 256;
@@ -132,14 +132,16 @@ struct Acc<true> {
 };
 
 template <bool F64, bool GST>
-__global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
+__global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
     // group sphere and acceptance test: the operations as written (no FMA
     // contraction), so the interaction set is the one the FP64 oracle forms
 #pragma clang fp contract(off)
     using Ent = typename std::conditional<F64, double4, float4>::type;
     __shared__ int s_stack[GST ? 1 : SCAP];
     __shared__ Ent s_list[LCAP];
-    __shared__ double s_red[3][64];
+    // the slice reduction reuses the list memory once the list is evaluated
+    static_assert(sizeof(Ent) * LCAP >= 3 * 64 * sizeof(double), "reduction space");
+    double(*s_red)[64] = reinterpret_cast<double(*)[64]>(s_list);
     const int lane = __lane_id();
     int* stack = GST ? a.gstack + size_t(blockIdx.x) * GSCAP : s_stack;
     const int scap = GST ? GSCAP : SCAP;

@@ -11,7 +11,8 @@ struct GfSph {
     double *r = nullptr;         // search radius of the current pass
     double *r_lo = nullptr, *r_hi = nullptr;   // radius bracket of the iteration
     double *dk = nullptr;        // distance to the k-th neighbour (self included)
-    double *h = nullptr, *rho = nullptr, *ih = nullptr, *sg = nullptr;
+    double *h = nullptr, *rho = nullptr;
+    double *rec = nullptr;       // 16 doubles per particle: packed neighbour records
     int8_t *done = nullptr;
     int *passes = nullptr;
     int *inv = nullptr;          // input index -> tree position

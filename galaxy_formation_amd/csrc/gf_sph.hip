@@ -805,64 +805,82 @@ struct SphView {
     int n;
 };
 
-__global__ __launch_bounds__(TB) void k_h_factors(int n, const double* __restrict__ h,
-                                                  double* __restrict__ ih, double* __restrict__ sg) {
-    const int t = blockIdx.x * TB + threadIdx.x;
-    if (t >= n) return;
-    const double hv = h[t];
-    ih[t] = 1.0 / hv;
-    sg[t] = 1.0 / (M_PI * (hv * hv * hv * hv));
-}
+// What a gas particle contributes as a neighbour, packed in 128 bytes so one
+// neighbour read touches two cache lines instead of one per array:
+//   a = (x, y, z, m), b = (vx, vy, vz, c), c = (h, 1/h, 1/(pi h^4), rho),
+//   d = (P / rho^2, -, -, -),  with c = sqrt(gamma (gamma - 1) u) and
+//   P = (gamma - 1) rho u: the expressions of sph.py:73-74, :88-89, :98.
+struct SphRec {
+    double4 a, b, c, d;
+};
 
-__global__ __launch_bounds__(TB) void k_density(SphView v, const int* __restrict__ nb, int k,
-                                                const double* __restrict__ ih,
-                                                double* __restrict__ rho) {
+__global__ __launch_bounds__(TB) void k_pack(SphView v, double gamma, SphRec* __restrict__ rec) {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * TB + threadIdx.x;
     if (t >= v.n) return;
-    const double x = v.x[t], y = v.y[t], z = v.z[t], hv = v.h[t];
-    const double ihv = ih[t], s3 = 1.0 / (M_PI * (hv * hv * hv));
-    double s = 0.0;
-    for (int jj = 0; jj < k; ++jj) {
-        const int j = nb[size_t(jj) * v.n + t];
-        const double dx = x - v.x[j], dy = y - v.y[j], dz = z - v.z[j];
-        s += v.m[j] * kernel_w(sqrt(dx * dx + dy * dy + dz * dz), ihv, s3);
-    }
-    rho[t] = s;
+    const double hv = v.h[t];
+    SphRec r;
+    r.a = make_double4(v.x[t], v.y[t], v.z[t], v.m[t]);
+    r.b = make_double4(v.vx[t], v.vy[t], v.vz[t], sqrt(gamma * (gamma - 1) * v.u[t]));
+    r.c = make_double4(hv, 1.0 / hv, 1.0 / (M_PI * (hv * hv * hv * hv)), 0.0);
+    r.d = make_double4(0.0, 0.0, 0.0, 0.0);
+    rec[t] = r;
 }
 
-__global__ __launch_bounds__(TB) void k_force(SphView v, const int* __restrict__ nb, int k,
-                                              const double* __restrict__ rho,
-                                              const double* __restrict__ ih,
-                                              const double* __restrict__ sg, double gamma,
-                                              double alpha, double beta,
-                                              const uint32_t* __restrict__ out_idx,
+// rho_i = sum_j m_j W(r_ij, h_i) over the neighbour table (sph.py:63-71); also
+// completes the particle's record with rho and P / rho^2.
+__global__ __launch_bounds__(TB) void k_density(int n, const int* __restrict__ nb, int k,
+                                                const double* __restrict__ u, double gamma,
+                                                SphRec* __restrict__ rec,
+                                                double* __restrict__ rho) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * TB + threadIdx.x;
+    if (t >= n) return;
+    const double4 at = rec[t].a;
+    const double4 ct = rec[t].c;
+    const double hv = ct.x, ihv = ct.y, s3 = 1.0 / (M_PI * (hv * hv * hv));
+    double s = 0.0;
+    for (int jj = 0; jj < k; ++jj) {
+        const int j = nb[size_t(jj) * n + t];
+        const double4 aj = rec[j].a;
+        const double dx = at.x - aj.x, dy = at.y - aj.y, dz = at.z - aj.z;
+        s += aj.w * kernel_w(sqrt(dx * dx + dy * dy + dz * dz), ihv, s3);
+    }
+    rho[t] = s;
+    rec[t].c.w = s;
+    const double P = (gamma - 1) * s * u[t];
+    rec[t].d.x = P / (s * s);
+}
+
+// Pressure + viscosity acceleration, du/dt, Courant step (sph.py:72-106) and
+// div v = -(1/rho_i) sum_j m_j (v_ij . r_ij) (1/r) dW/dr(h_i).
+__global__ __launch_bounds__(TB) void k_force(int n, const int* __restrict__ nb, int k,
+                                              const SphRec* __restrict__ rec, double alpha,
+                                              double beta, const uint32_t* __restrict__ out_idx,
                                               double* __restrict__ acc, double* __restrict__ dudt,
                                               double* __restrict__ dt, double* __restrict__ divv,
                                               double* __restrict__ rho_out,
                                               double* __restrict__ h_out) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * TB + threadIdx.x;
-    if (i >= v.n) return;
-    const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
-    const double vxi = v.vx[i], vyi = v.vy[i], vzi = v.vz[i];
-    const double rhoi = rho[i], ui = v.u[i], hi_ = v.h[i];
-    const double ihi = ih[i], sgi = sg[i];
-    const double Pi = (gamma - 1) * rhoi * ui;
-    const double ci = sqrt(gamma * (gamma - 1) * ui);
-    const double pi2 = Pi / (rhoi * rhoi);
+    if (i >= n) return;
+    const SphRec ri = rec[i];
+    const double xi = ri.a.x, yi = ri.a.y, zi = ri.a.z;
+    const double vxi = ri.b.x, vyi = ri.b.y, vzi = ri.b.z, ci = ri.b.w;
+    const double hi_ = ri.c.x, ihi = ri.c.y, sgi = ri.c.z, rhoi = ri.c.w;
+    const double pi2 = ri.d.x;
     double ax = 0, ay = 0, az = 0, du = 0, mu_max = 0, dv = 0;
     for (int jj = 0; jj < k; ++jj) {
-        const int j = nb[size_t(jj) * v.n + i];
+        const int j = nb[size_t(jj) * n + i];
         if (j == i) continue;
-        const double dx = xi - v.x[j], dy = yi - v.y[j], dz = zi - v.z[j];
+        const double4 aj = rec[j].a, bj = rec[j].b, cj4 = rec[j].c;
+        const double pj2 = rec[j].d.x;
+        const double dx = xi - aj.x, dy = yi - aj.y, dz = zi - aj.z;
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
         const double gi = kernel_g(r, ihi, sgi);
-        const double gj = kernel_g(r, ih[j], sg[j]);
-        const double rhoj = rho[j], uj = v.u[j], hj = v.h[j];
-        const double Pj = (gamma - 1) * rhoj * uj;
-        const double cj = sqrt(gamma * (gamma - 1) * uj);
-        const double vr = (vxi - v.vx[j]) * dx + (vyi - v.vy[j]) * dy + (vzi - v.vz[j]) * dz;
+        const double gj = kernel_g(r, cj4.y, cj4.z);
+        const double rhoj = cj4.w, hj = cj4.x, cj = bj.w;
+        const double vr = (vxi - bj.x) * dx + (vyi - bj.y) * dy + (vzi - bj.z) * dz;
         double visc = 0.0;
         if (vr < 0) {
             const double hb = 0.5 * (hi_ + hj);
@@ -871,8 +889,8 @@ __global__ __launch_bounds__(TB) void k_force(SphView v, const int* __restrict__
             visc = (-alpha * 0.5 * (ci + cj) * mu + beta * mu * mu) / (0.5 * (rhoi + rhoj));
         }
         const double gb = 0.5 * (gi + gj);
-        const double f = pi2 * gi + Pj / (rhoj * rhoj) * gj + visc * gb;
-        const double mj = v.m[j];
+        const double f = pi2 * gi + pj2 * gj + visc * gb;
+        const double mj = aj.w;
         ax -= mj * f * dx;
         ay -= mj * f * dy;
         az -= mj * f * dz;
@@ -1027,9 +1045,9 @@ int sph_reserve(GfSph& S, int n, int k) {
         const int cn = std::max(n, 1024);
         if (dalloc(S.vx, cn) || dalloc(S.vy, cn) || dalloc(S.vz, cn) || dalloc(S.u, cn) ||
             dalloc(S.r, cn) || dalloc(S.dk, cn) || dalloc(S.h, cn) || dalloc(S.rho, cn) ||
-            dalloc(S.ih, cn) || dalloc(S.sg, cn) || dalloc(S.done, cn) || dalloc(S.passes, cn) ||
+            dalloc(S.done, cn) || dalloc(S.passes, cn) ||
             dalloc(S.inv, cn) || dalloc(S.leaf_of, cn) ||
-            dalloc(S.r_lo, cn) || dalloc(S.r_hi, cn))
+            dalloc(S.r_lo, cn) || dalloc(S.r_hi, cn) || dalloc(S.rec, 16 * size_t(cn)))
             return -1;
         S.cap_n = cn;
         S.cap_k = 0;
@@ -1051,7 +1069,7 @@ int sph_reserve(GfSph& S, int n, int k) {
 void gf_sph_free(GfSph& S) {
     gf_tree_free(S.tree);
     dfree(S.vx); dfree(S.vy); dfree(S.vz); dfree(S.u); dfree(S.r); dfree(S.dk); dfree(S.h);
-    dfree(S.rho); dfree(S.ih); dfree(S.sg); dfree(S.done); dfree(S.passes); dfree(S.inv); dfree(S.leaf_of); dfree(S.r_lo); dfree(S.r_hi);
+    dfree(S.rho); dfree(S.done); dfree(S.passes); dfree(S.inv); dfree(S.leaf_of); dfree(S.r_lo); dfree(S.r_hi); dfree(S.rec);
     dfree(S.nb); dfree(S.state); dfree(S.ovf); dfree(S.ovf2); dfree(S.gstack);
     S = GfSph();
 }
@@ -1137,7 +1155,6 @@ int gf_sph_smoothing(GfSph& S, int k, double eps, hipStream_t s) {
     const int n = S.tree.n;
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_smoothing, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, S.dk, eps, S.h);
-    hipLaunchKernelGGL(k_h_factors, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, S.h, S.ih, S.sg);
     GF_LAUNCH_CHECK();
     return 0;
 }
@@ -1149,9 +1166,11 @@ int gf_sph_forces_tree(GfSph& S, int k, double gamma, double alpha, double beta,
     if (n == 0) return 0;
     SphView v{S.tree.tx, S.tree.ty, S.tree.tz, S.vx, S.vy, S.vz, S.tree.tm, S.u, S.h, n};
     const int g = gf_div_up(n, TB);
-    hipLaunchKernelGGL(k_density, dim3(g), dim3(TB), 0, s, v, S.nb, k, S.ih, S.rho);
-    hipLaunchKernelGGL(k_force, dim3(g), dim3(TB), 0, s, v, S.nb, k, S.rho, S.ih, S.sg, gamma,
-                       alpha, beta, S.tree.idx, acc, dudt, dt, divv, rho, h_out);
+    SphRec* rec = reinterpret_cast<SphRec*>(S.rec);
+    hipLaunchKernelGGL(k_pack, dim3(g), dim3(TB), 0, s, v, gamma, rec);
+    hipLaunchKernelGGL(k_density, dim3(g), dim3(TB), 0, s, n, S.nb, k, S.u, gamma, rec, S.rho);
+    hipLaunchKernelGGL(k_force, dim3(g), dim3(TB), 0, s, n, S.nb, k, rec, alpha, beta,
+                       S.tree.idx, acc, dudt, dt, divv, rho, h_out);
     GF_LAUNCH_CHECK();
     return 0;
 }
@@ -1161,26 +1180,26 @@ int gf_sph_forces_table(int n, int k, const double* pos, const double* vel, cons
                         double alpha, double beta, double* rho, double* acc, double* dudt,
                         double* dt, double* divv, int* scratch, hipStream_t s) {
     if (n == 0) return 0;
-    // scratch: [k][n] ints, then 6n doubles (SoA pos, vel), then 3n doubles
+    // scratch: [k][n] ints, then 6n doubles (SoA pos, vel), then n records
     int* nbt = scratch;
     double* soa = reinterpret_cast<double*>(scratch + ((size_t(k) * n + 1) & ~size_t(1)));
-    double* ih = soa + 6 * size_t(n);
-    double* sg = ih + n;
-    double* rr = sg + n;
+    SphRec* rec = reinterpret_cast<SphRec*>(
+        (reinterpret_cast<uintptr_t>(soa + 6 * size_t(n)) + 255) & ~uintptr_t(255));
+    double* rr = soa + 6 * size_t(n) + 16 * size_t(n) + 64;
     const int g = gf_div_up(n, TB);
     hipLaunchKernelGGL(k_table_prep, dim3(g), dim3(TB), 0, s, n, k, nb, pos, vel, nbt, soa);
-    hipLaunchKernelGGL(k_h_factors, dim3(g), dim3(TB), 0, s, n, h, ih, sg);
     SphView v{soa, soa + n, soa + 2 * size_t(n), soa + 3 * size_t(n), soa + 4 * size_t(n),
               soa + 5 * size_t(n), mass, u, h, n};
-    hipLaunchKernelGGL(k_density, dim3(g), dim3(TB), 0, s, v, nbt, k, ih, rr);
-    hipLaunchKernelGGL(k_force, dim3(g), dim3(TB), 0, s, v, nbt, k, rr, ih, sg, gamma, alpha, beta,
-                       nullptr, acc, dudt, dt, divv, rho, nullptr);
+    hipLaunchKernelGGL(k_pack, dim3(g), dim3(TB), 0, s, v, gamma, rec);
+    hipLaunchKernelGGL(k_density, dim3(g), dim3(TB), 0, s, n, nbt, k, u, gamma, rec, rr);
+    hipLaunchKernelGGL(k_force, dim3(g), dim3(TB), 0, s, n, nbt, k, rec, alpha, beta, nullptr, acc,
+                       dudt, dt, divv, rho, nullptr);
     GF_LAUNCH_CHECK();
     return 0;
 }
 
 size_t gf_sph_table_scratch_bytes(int n, int k) {
-    return sizeof(int) * (size_t(k) * n + 2) + sizeof(double) * 9 * size_t(n);
+    return sizeof(int) * (size_t(k) * n + 2) + sizeof(double) * (6 + 16 + 1) * size_t(n) + 1024;
 }
 
 int gf_sph_export_knn(GfSph& S, int kq, int64_t* nb, double* dist, hipStream_t s) {
