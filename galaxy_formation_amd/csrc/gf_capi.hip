@@ -89,6 +89,11 @@ static int gravity_impl(int n, const double* pos, const double* mass, double* ac
     return gf_walk(T, g_ctx.walk, active, acc, eps, precision, s);
 }
 
+GfTestLimits& gf_test_limits() {
+    static GfTestLimits limits;
+    return limits;
+}
+
 extern "C" {
 
 const char* gf_last_error(void) { return g_err; }
@@ -107,6 +112,16 @@ int gf_device_info(char* buf, int len) {
 int nbody_num_threads(void) {
     if (gf_ctx_init()) return -1;
     return g_ctx.n_cu;
+}
+
+int gf_set_test_limits(int walk_stack, int knn_list) {
+    if (walk_stack < 0 || walk_stack > 1024 || knn_list < 0 || knn_list > 1024) {
+        gf_set_error("test limits: walk_stack and knn_list must be in [0, 1024] (0 = default)");
+        return -1;
+    }
+    gf_test_limits().walk_stack = walk_stack;
+    gf_test_limits().knn_list = knn_list;
+    return 0;
 }
 
 int gf_set_precision(int p) {

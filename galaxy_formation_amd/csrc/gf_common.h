@@ -89,3 +89,11 @@ __device__ __forceinline__ uint64_t gf_key_prefix(uint64_t key, int d) {
 }
 
 static inline int gf_div_up(long long a, long long b) { return int((a + b - 1) / b); }
+
+// Capacities that tests lower to drive the fallback paths (0 = default):
+// the walk's LDS stack entries and the neighbour solve's candidate list.
+struct GfTestLimits {
+    int walk_stack = 0;
+    int knn_list = 0;
+};
+GfTestLimits& gf_test_limits();

@@ -56,6 +56,7 @@ struct WalkArgs {
     int* ovf;
     int* ovf_count;            // [0] queued groups, [1] global-stack overflows (error)
     int* gstack;
+    int lds_scap;              // LDS stack entries used by the first launch (<= SCAP)
 };
 
 __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v) {
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
     double(*s_red)[64] = reinterpret_cast<double(*)[64]>(s_list);
     const int lane = __lane_id();
     int* stack = GST ? a.gstack + size_t(blockIdx.x) * GSCAP : s_stack;
-    const int scap = GST ? GSCAP : SCAP;
+    const int scap = GST ? GSCAP : a.lds_scap;
     const int ngroups = GST ? a.ovf_count[0] : a.counters[1];
     int* cursor = a.work + (GST ? 1 : 0);
     using Real = typename std::conditional<F64, double, float>::type;
@@ -452,6 +453,7 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
     a.active = active; a.acc = acc; a.eps2 = eps * eps;
     a.work = W.work; a.inter = W.inter; a.ovf = W.ovf; a.ovf_count = W.ovf_count;
     a.gstack = W.gstack;
+    a.lds_scap = gf_test_limits().walk_stack > 0 ? std::min(SCAP, gf_test_limits().walk_stack) : SCAP;
     hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, W.work, W.ovf_count);
     if (precision == GF_FP64) {
         hipLaunchKernelGGL((k_walk<true, false>), dim3(WALK_BLOCKS), dim3(64), 0, s, a);

@@ -13,6 +13,10 @@ struct GfSph {
     double *dk = nullptr;        // distance to the k-th neighbour (self included)
     double *h = nullptr, *rho = nullptr;
     double *rec = nullptr;       // 16 doubles per particle: packed neighbour records
+    double *p4 = nullptr;        // (x, y, z, -) per particle for the neighbour walks
+    double *knode = nullptr;     // 8 doubles per node: packed nodes for the neighbour walks
+    int *gmark = nullptr;        // per group: pass tag of its last retry request
+    int *retry = nullptr;        // two lists of groups that need another pass
     int8_t *done = nullptr;
     int *passes = nullptr;
     int *inv = nullptr;          // input index -> tree position

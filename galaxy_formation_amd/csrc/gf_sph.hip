@@ -4,19 +4,21 @@
 // Neighbour solve.  The reference takes the k nearest gas particles of each
 // gas particle (itself included) from a k-d tree (sph.py:165-166) and sets
 // h = max(d_k / 2, eps / 2) (simulation.py:84).  Here the gas particles get an
-// octree of their own (the gravity builder); one wave64 serves one group of
-// <= 64 gas particles.  Each lane owns a search radius r_i and a max-heap in
-// LDS of the kq best (d^2, input index) seen so far.  The wave walks the tree
-// once for the sphere around the group that holds every member's search
-// sphere, streams the particles of the leaves it touches through LDS, and each
-// lane inserts those within r_i.  A lane that saw >= kq particles inside r_i
-// holds the exact kq nearest; the others grow r_i and take another pass.
-// Distances are FP64 with the operations of the oracle (no contraction), so
-// neighbour sets and d_k are bit-identical to the brute-force oracle.
+// octree of their own (the gravity builder), and the solve is a radius
+// iteration: each particle owns a search radius r_i; a pass gathers the gas
+// particles inside r_i and, when at least kq are inside, selects the exact kq
+// nearest; otherwise r_i grows and the particle takes another pass.  The
+// fast pass (k_knn_fast) serves one group of <= 64 gas particles per block of
+// four waves; groups it cannot finish go to a heap pass (k_knn_pass: one wave
+// per group, a max-heap of the kq best (d^2, input index) per lane in LDS),
+// which also writes rows sorted when asked.  Distances are FP64 with the
+// operations of the oracle (no contraction), so neighbour sets and d_k are
+// bit-identical to the brute-force oracle whatever the first radius.
 //
 // Density and forces.  One lane per gas particle in Morton order, the
 // neighbour table stored column-major ([k][n]) so a wave reads 64 adjacent
-// particles' j-th neighbours together; neighbour data are mostly L2 hits.
+// particles' j-th neighbours together; each neighbour's data is one packed
+// 128-byte record, mostly an L2 hit.
 #include "gf_prim.h"
 #include "gf_sph.h"
 
@@ -29,7 +31,20 @@ constexpr int KGSCAP = 512 * (GF_KEY_LEVELS + 2);       // global stack entries 
 constexpr int KNN_BLOCKS = 4096;
 constexpr int KNN_GST_BLOCKS = 128;
 
+// A gas-tree node packed for the neighbour walks (64 bytes): the box, then
+// the 8 children in octant order (-1 = empty), or, for a node of at most
+// KNN_RANGE particles (contiguous in tree order) or a leaf, its particle
+// range (KNODE_RANGE, first, count, depth-capped leaf of > GF_LEAF_SIZE).
+constexpr int KNODE_RANGE = -2;
+constexpr int KNN_RANGE = 64;
+struct KNode {
+    double4 box;
+    int4 lo, hi;
+};
+
 struct KnnArgs {
+    const KNode* kn;
+    const double4* p4;      // (x, y, z, -) per tree position
     const int* start;
     const int* count;
     const int8_t* leaf;
@@ -60,6 +75,12 @@ struct KnnArgs {
     int* ovf2;      // queue 2: groups the heap pass hands to its global-stack variant
     int* gstack;
     int use_queue;  // the heap pass walks queue 1 (else every group)
+    const int* retry_in;    // groups of this pass (null = all)
+    int n_retry_in;
+    int* retry_out;         // groups that need another pass (count in state[7])
+    int* gmark;             // pass tag of the last retry request per group
+    int pass_tag;
+    int list_cap;           // candidate-list capacity (tests lower it)
 };
 
 // Nodes to pop from a LIFO of capacity cap holding sn entries: a batch of 64
@@ -68,6 +89,12 @@ struct KnnArgs {
 // a reserve of 7 * 64 + 8 * (GF_KEY_LEVELS + 1) keeps every push in bounds.
 __device__ __forceinline__ int stack_take(int sn, int cap) {
     return (cap - sn >= 7 * 64 + 8 * (GF_KEY_LEVELS + 1)) ? min(64, sn) : 1;
+}
+
+// Group position gp needs another pass (once per pass, whoever asks).
+__device__ __forceinline__ void request_retry(const KnnArgs& a, int gp) {
+    if (atomicExch(&a.gmark[gp], a.pass_tag) != a.pass_tag)
+        a.retry_out[atomicAdd(&a.state[7], 1)] = gp;
 }
 
 __device__ __forceinline__ bool key_less(double da, int ia, double db, int ib) {
@@ -95,7 +122,8 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
         if (lane == 0) gi = atomicAdd(cursor, 1);
         gi = __shfl(gi, 0, 64);
         if (gi >= ngroups) break;
-        const int G = a.groups[list ? list[gi] : gi];
+        const int gp = list ? list[gi] : gi;
+        const int G = a.groups[gp];
         const int b0 = a.start[G], np = a.count[G];
         for (int c0 = 0; c0 < np; c0 += 64) {
             const int t = b0 + c0 + lane;
@@ -246,7 +274,7 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
             }
             if (overflow) {
                 if (!GST) {
-                    if (lane == 0) a.ovf2[atomicAdd(&a.state[6], 1)] = list ? list[gi] : gi;
+                    if (lane == 0) a.ovf2[atomicAdd(&a.state[6], 1)] = gp;
                 } else if (lane == 0) {
                     atomicAdd(&a.state[4], 1);
                 }
@@ -305,40 +333,37 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
                     atomicAdd(&a.state[0], 1);
                 }
             }
+            if (__ballot(mine && cnt < kq) != 0 && lane == 0) request_retry(a, gp);
             __syncthreads();
         }
     }
 }
 
 __global__ void k_knn_reset(int* state) {
-    if (threadIdx.x < 8) state[threadIdx.x] = 0;
+    if (threadIdx.x < 8) state[threadIdx.x] = 0;   // [7]: retry list length
 }
 
 // ----------------------------------------------------- fast neighbour pass
 // One block of four waves per gas group.  Candidates (gas particles inside a
-// search sphere) are gathered into LDS by a tree walk that starts at the
-// smallest node whose box strictly contains the sphere.
-//   group mode  (the group's bounding sphere b, R that holds every unresolved
-//               member's search sphere is tight: R <= 2 max r_i): wave 0
-//               gathers once for all members into the whole list;
-//   member mode (otherwise): each wave gathers its current member's own
-//               sphere into a quarter of the list.
+// search sphere) are gathered into LDS by a tree walk over packed nodes that
+// starts at the smallest node whose box strictly contains the sphere.
 // Per member the 64 lanes of a wave compute the exact FP64 squared distances
-// to the list (their FP32 roundings are the radix keys: rounding to FP32
-// never reverses an order), count those within the member's radius and find
-// the kq-th smallest key by a 4-round, 8-bit MSB radix select over a 256-bin
-// LDS histogram.  Keys below it are in; among keys equal to it (FP32 ties) the
-// exact (d^2, index) order decides.  Rows are written unordered; d_k is the
-// largest exact d^2 taken.  A member with fewer than kq inside its radius
-// grows the radius; one whose sphere holds more candidates than the list
-// shrinks it; both take another pass.
+// to the list, count those within the member's radius, select the kq
+// nearest (radix rounds on fixed-point keys, then an exact rank of the
+// boundary bucket by (d^2, index)) and write the row unordered; d_k is the
+// kq-th exact distance.  A member with fewer than kq inside its radius grows
+// the radius; one whose sphere holds more candidates than the list shrinks
+// it; both take another pass.
 constexpr int FT = 256;
-constexpr int CMAX = 1024;
+constexpr int CMAX = 1024;                   // candidate list of a group gather
 constexpr int CPL = CMAX / 64;
-constexpr int CQ = CMAX / 4;                 // per-wave list in member mode
 constexpr int FSCAP = 1024;                  // per-wave stack
+constexpr int LMAX = FSCAP;                  // touched ranges of a group gather
+constexpr int EMAX = 4 * CMAX;               // particles in those ranges
 constexpr int FAST_BLOCKS = 2048;
 constexpr uint32_t KEY_NONE = 0xFFFFFFFFu;
+constexpr double KEY_SCALE = 4294967294.0;   // keys d^2 / r^2 * KEY_SCALE < KEY_NONE
+constexpr double GROUP_REACH = 3.0;          // gather once while R <= 3 max r_i
 
 // Smallest ancestor of `node` (node included) whose box strictly contains
 // the sphere; the root when none does.
@@ -356,106 +381,223 @@ __device__ int containing_node(const KnnArgs& a, int node, double bx, double by,
     }
 }
 
-// One wave gathers the particles within (b, R) of the subtree of `root` into
-// (lx, ly, lz, li)[0, cap).  Returns the count; when it exceeds cap, minus the
-// count reached (a lower bound); -1 - cap - 2 ... never -2, which flags a
-// depth-capped leaf.
-__device__ int gather_sphere(const KnnArgs& a, int root, double bx, double by, double bz,
-                             double R2, double* lx, double* ly, double* lz, int* li, int cap,
-                             int* stack, int* sc) {
+// One lane's visit of a packed node against the sphere (b, R2): a range
+// node that meets it gives its particles (first, cnt), an internal node its
+// children; capped marks a depth-capped leaf of more than GF_LEAF_SIZE.
+struct NodeHit {
+    int push, first, cnt, capped;
+    int4 c_lo, c_hi;
+};
+
+__device__ __forceinline__ bool box_meets(const double4& box, double x, double y, double z,
+                                          double r2) {
 #pragma clang fp contract(off)
-    const int lane = __lane_id();
-    const uint64_t lt = gf_lanemask_lt();
-    int sn = 1, nc = 0;
-    if (lane == 0) stack[0] = root;
-    __builtin_amdgcn_wave_barrier();
-    while (sn > 0) {
-        const int take = stack_take(sn, FSCAP);
-        sn -= take;
-        const int node = lane < take ? stack[sn + lane] : -1;
-        __builtin_amdgcn_wave_barrier();
-        int push = 0, first = 0, cntn = 0;
-        int4 c_lo = make_int4(-1, -1, -1, -1), c_hi = c_lo;
-        if (node >= 0) {
-            const double4 bb = a.box[node];
-            const double ex = fmax(0.0, fabs(bx - bb.x) - bb.w);
-            const double ey = fmax(0.0, fabs(by - bb.y) - bb.w);
-            const double ez = fmax(0.0, fabs(bz - bb.z) - bb.w);
-            if (ex * ex + ey * ey + ez * ez <= R2) {
-                if (a.leaf[node]) {
-                    first = a.start[node];
-                    cntn = a.count[node];
-                } else {
-                    const int4* cs = reinterpret_cast<const int4*>(a.child8 + 8 * size_t(node));
-                    c_lo = cs[0];
-                    c_hi = cs[1];
-                    push = (c_lo.x >= 0) + (c_lo.y >= 0) + (c_lo.z >= 0) + (c_lo.w >= 0) +
-                           (c_hi.x >= 0) + (c_hi.y >= 0) + (c_hi.z >= 0) + (c_hi.w >= 0);
-                }
-            }
-        }
-        const int pinc = gf_wave_incl_scan(push);
-        const int ptot = __shfl(pinc, 63, 64);
-        {
-            int o = sn + pinc - push;
-            const int cc[8] = {c_lo.x, c_lo.y, c_lo.z, c_lo.w, c_hi.x, c_hi.y, c_hi.z, c_hi.w};
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                if (cc[k] >= 0) stack[o++] = cc[k];
-        }
-        sn += ptot;
-        if (__ballot(cntn > GF_LEAF_SIZE) != 0) return -2;   // a depth-capped leaf
-        // the touched leaves' particles, one per lane: lane l's leaf owns
-        // entries [inc_l - cntn_l, inc_l) of this batch (owners looked up in
-        // the wave's LDS scratch, so no shuffle reads an inactive lane)
-        const int einc = gf_wave_incl_scan(cntn);
-        const int etot = __shfl(einc, 63, 64);
-        sc[lane] = einc;
-        sc[64 + lane] = first;
-        sc[128 + lane] = cntn;
-        __builtin_amdgcn_wave_barrier();
-        for (int e0 = 0; e0 < etot; e0 += 64) {
-            const int e = e0 + lane;
-            bool tk = false;
-            double px = 0, py = 0, pz = 0;
-            int pj = 0;
-            if (e < etot) {
-                int lo = 0, hi = 63;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (sc[mid] > e) hi = mid; else lo = mid + 1;
-                }
-                pj = sc[64 + lo] + (e - (sc[lo] - sc[128 + lo]));
-                px = a.tx[pj]; py = a.ty[pj]; pz = a.tz[pj];
-                const double dx = px - bx, dy = py - by, dz = pz - bz;
-                tk = dx * dx + dy * dy + dz * dz <= R2;
-            }
-            const uint64_t m = __ballot(tk);
-            const int o = nc + __popcll(m & lt);
-            if (tk && o < cap) {
-                lx[o] = px; ly[o] = py; lz[o] = pz;
-                li[o] = int(a.idx[pj]);
-            }
-            nc += __popcll(m);
-        }
-        __builtin_amdgcn_wave_barrier();
-        if (nc > cap) return -nc;
-        __builtin_amdgcn_wave_barrier();
-    }
-    return nc;
+    const double ex = fmax(0.0, fabs(x - box.x) - box.w);
+    const double ey = fmax(0.0, fabs(y - box.y) - box.w);
+    const double ez = fmax(0.0, fabs(z - box.z) - box.w);
+    return ex * ex + ey * ey + ez * ez <= r2;
 }
 
-// Exact kq nearest of member t among the list (lx, ly, lz, li)[0, nc), by
-// one wave.  Returns 1 when resolved, 0 when the radius was too small (grown),
-// -1 on more than 64 FP32 ties at the boundary (left for the heap pass).
-__device__ int select_member(const KnnArgs& a, int t, const double* lx, const double* ly,
-                             const double* lz, const int* li, int nc, int* hist, double* tie_d,
-                             int* tie_i) {
+// Does the point or box meet one of the nm spheres (x, y, z, r^2) of msph?
+__device__ __forceinline__ bool box_meets_any(const double4& box, const double4* msph, int nm) {
+    for (int q = 0; q < nm; ++q) {
+        const double4 m = msph[q];
+        if (box_meets(box, m.x, m.y, m.z, m.w)) return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool point_in_any(double x, double y, double z, const double4* msph,
+                                             int nm) {
+#pragma clang fp contract(off)
+    for (int q = 0; q < nm; ++q) {
+        const double4 m = msph[q];
+        const double dx = x - m.x, dy = y - m.y, dz = z - m.z;
+        if (dx * dx + dy * dy + dz * dz <= m.w) return true;
+    }
+    return false;
+}
+
+// One lane's visit of a packed node against the sphere (b, R2) and, when
+// nm > 0, the union of the member spheres msph[0, nm).
+__device__ __forceinline__ NodeHit visit_node(const KnnArgs& a, int node, double bx, double by,
+                                              double bz, double R2, const double4* msph, int nm) {
+#pragma clang fp contract(off)
+    NodeHit h{0, 0, 0, 0, make_int4(-1, -1, -1, -1), make_int4(-1, -1, -1, -1)};
+    if (node < 0) return h;
+    const KNode kn = a.kn[node];
+    if (box_meets(kn.box, bx, by, bz, R2) && (nm == 0 || box_meets_any(kn.box, msph, nm))) {
+        if (kn.lo.x == KNODE_RANGE) {
+            h.first = kn.lo.y;
+            h.cnt = kn.lo.z;
+            h.capped = kn.lo.w;
+        } else {
+            h.c_lo = kn.lo;
+            h.c_hi = kn.hi;
+            h.push = (kn.lo.x >= 0) + (kn.lo.y >= 0) + (kn.lo.z >= 0) + (kn.lo.w >= 0) +
+                     (kn.hi.x >= 0) + (kn.hi.y >= 0) + (kn.hi.z >= 0) + (kn.hi.w >= 0);
+        }
+    }
+    return h;
+}
+
+// Pushes the children of this batch's nodes at stack[sn ...] in lane order
+// (the stack stays sorted by depth); returns how many.
+__device__ __forceinline__ int push_children(const NodeHit& h, int* stack, int sn) {
+    const int pinc = gf_wave_incl_scan(h.push);
+    int o = sn + pinc - h.push;
+    const int cc[8] = {h.c_lo.x, h.c_lo.y, h.c_lo.z, h.c_lo.w, h.c_hi.x, h.c_hi.y, h.c_hi.z, h.c_hi.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (cc[k] >= 0) stack[o++] = cc[k];
+    return __shfl(pinc, 63, 64);
+}
+
+// The whole block gathers the particles within (b, R) of the subtree of
+// `root`, and, when nm > 0, within one of the member spheres msph[0, nm)
+// (x, y, z, r^2): wave 0 walks the tree and lists the ranges that meet them
+// (lf: first particle, lincl: inclusive prefix of their counts); then every
+// wave loads those ranges' particles and keeps the ones inside, in list
+// order, so the candidate order is deterministic.  Called by all threads.
+// Returns the count, -2 for a depth-capped leaf, -1 when the sphere holds
+// more than cap particles (ctl[7] = an estimate of their number, from the
+// share of entries inside) or its ranges more than LMAX / EMAX (ctl[7] = 0).
+__device__ int gather_group(const KnnArgs& a, int root, double bx, double by, double bz,
+                            double R2, const double4* msph, int nm, double* lx, double* ly,
+                            double* lz, int* li, int cap, int* stack, int* lf, int* lincl,
+                            int* ctl) {
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
+    const uint64_t lt = gf_lanemask_lt();
+    if (w == 0) {
+        int sn = 1, nl = 0, et = 0, res = 0;
+        if (lane == 0) stack[0] = root;
+        __builtin_amdgcn_wave_barrier();
+        while (sn > 0) {
+            const int take = stack_take(sn, FSCAP);
+            sn -= take;
+            const int node = lane < take ? stack[sn + lane] : -1;
+            __builtin_amdgcn_wave_barrier();
+            const NodeHit h = visit_node(a, node, bx, by, bz, R2, msph, nm);
+            sn += push_children(h, stack, sn);
+            if (__ballot(h.capped != 0) != 0) {
+                res = -2;
+                break;
+            }
+            const uint64_t lm = __ballot(h.cnt > 0);
+            const int einc = gf_wave_incl_scan(h.cnt);
+            const int ebatch = __shfl(einc, 63, 64);
+            const int nlb = __popcll(lm);
+            if (nl + nlb > LMAX || et + ebatch > EMAX) {
+                res = -1;
+                break;
+            }
+            if (h.cnt > 0) {
+                const int o = nl + __popcll(lm & lt);
+                lf[o] = h.first;
+                lincl[o] = et + einc;
+            }
+            nl += nlb;
+            et += ebatch;
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (lane == 0) {
+            ctl[4] = res;
+            ctl[5] = nl;
+            ctl[6] = et;
+        }
+    }
+    __syncthreads();
+    const int res = ctl[4], nl = ctl[5], et = ctl[6];
+    if (res < 0) {
+        if (threadIdx.x == 0) ctl[7] = 0;         // count not known
+        __syncthreads();
+        return res;
+    }
+    int nc = 0;
+    for (int e0 = 0; e0 < et; e0 += FT) {
+        const int e = e0 + tid;
+        bool tk = false;
+        double px = 0, py = 0, pz = 0;
+        int pj = 0;
+        if (e < et) {
+            int lo = 0, hi = nl - 1;                 // first range with lincl > e
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (lincl[mid] > e) hi = mid; else lo = mid + 1;
+            }
+            pj = lf[lo] + (e - (lo > 0 ? lincl[lo - 1] : 0));
+            const double4 q = a.p4[pj];
+            px = q.x; py = q.y; pz = q.z;
+            const double dx = px - bx, dy = py - by, dz = pz - bz;
+            tk = dx * dx + dy * dy + dz * dz <= R2 && (nm == 0 || point_in_any(px, py, pz, msph, nm));
+        }
+        const uint64_t m = __ballot(tk);
+        if (lane == 0) ctl[8 + w] = __popcll(m);
+        __syncthreads();
+        int base = nc;
+        for (int q = 0; q < w; ++q) base += ctl[8 + q];
+        const int tot = ctl[8] + ctl[9] + ctl[10] + ctl[11];
+        const int o = base + __popcll(m & lt);
+        if (tk && o < cap) {
+            lx[o] = px; ly[o] = py; lz[o] = pz;
+            li[o] = pj;
+        }
+        nc += tot;
+        __syncthreads();
+        if (nc > cap) {
+            // the list is full: estimate the sphere's count from the entries seen
+            const long long seen = min(et, e0 + FT);
+            nc = int(min(1ll << 30, (long long)nc * et / seen));
+            break;
+        }
+    }
+    if (threadIdx.x == 0) ctl[7] = nc;
+    return nc > cap ? -1 : nc;
+}
+
+// Sphere (b, R) around the search spheres of the lanes with `mine`, and the
+// largest of their radii (wave reductions; every lane gets the result).
+struct Sphere {
+    double bx, by, bz, R, rmax;
+};
+
+__device__ __forceinline__ Sphere wave_sphere(bool mine, double xi, double yi, double zi,
+                                              double ri) {
+#pragma clang fp contract(off)
+    double x0 = mine ? xi : 1e300, y0 = mine ? yi : 1e300, z0 = mine ? zi : 1e300;
+    double x1 = mine ? xi : -1e300, y1 = mine ? yi : -1e300, z1 = mine ? zi : -1e300;
+    x0 = gf_wave_min_d(x0); y0 = gf_wave_min_d(y0); z0 = gf_wave_min_d(z0);
+    x1 = gf_wave_max_d(x1); y1 = gf_wave_max_d(y1); z1 = gf_wave_max_d(z1);
+    Sphere S;
+    S.bx = 0.5 * (x0 + x1); S.by = 0.5 * (y0 + y1); S.bz = 0.5 * (z0 + z1);
+    double reach = 0;
+    if (mine) {
+        const double dx = xi - S.bx, dy = yi - S.by, dz = zi - S.bz;
+        reach = sqrt(dx * dx + dy * dy + dz * dz) + ri;
+    }
+    S.R = gf_wave_max_d(reach) * (1.0 + 1e-12) + 1e-300;
+    S.rmax = gf_wave_max_d(mine ? ri : 0.0);
+    return S;
+}
+
+// Exact kq nearest of member t (at (xi, yi, zi), radius ri) among the list
+// (lx, ly, lz, li)[0, nc) (li: tree positions), by one wave.  Keys are
+// d^2 / r^2 in 32-bit fixed point (monotone in the FP64 d^2); radix rounds
+// of 8 bits narrow the bucket holding the kq-th key until it holds at most
+// 64 keys, which are then ranked exactly by (d^2, input index), one per lane
+// (input indices are read only when two of them tie in d^2).  Returns 1 when
+// resolved, 0 when the radius was too small (grown), -1 on more than 64
+// equal keys (left for the heap pass).
+__device__ int select_member(const KnnArgs& a, int t, double xi, double yi, double zi, double ri,
+                             int npass, const double* lx, const double* ly, const double* lz,
+                             const int* li, int nc, int* hist, double* tie_d, int* tie_i,
+                             int* tie_p) {
 #pragma clang fp contract(off)
     const int lane = __lane_id();
     const int kq = a.kq;
-    const double xi = a.tx[t], yi = a.ty[t], zi = a.tz[t], ri = a.r[t];
     const double r2i = ri * ri;
+    const double ks = KEY_SCALE / r2i;
     const int ncl = (nc + 63) >> 6;
     uint32_t key[CPL];
     int cnt = 0;
@@ -467,7 +609,7 @@ __device__ int select_member(const KnnArgs& a, int t, const double* lx, const do
             if (j < nc) {
                 const double dx = xi - lx[j], dy = yi - ly[j], dz = zi - lz[j];
                 const double d2 = dx * dx + dy * dy + dz * dz;
-                if (d2 <= r2i) kk = __float_as_uint(float(d2));
+                if (d2 <= r2i) kk = uint32_t(fmin(d2 * ks, KEY_SCALE));
             }
         }
         key[c] = kk;
@@ -476,7 +618,7 @@ __device__ int select_member(const KnnArgs& a, int t, const double* lx, const do
     const int total = gf_wave_sum_i(cnt);
     if (total < kq) {
         if (lane == 0) {
-            a.passes[t] += 1;
+            a.passes[t] = npass + 1;
             const double want = double(kq) / double(max(total, 1));
             double rn = ri * fmin(2.0, fmax(1.2, 1.1 * cbrt(want)));
             a.r_lo[t] = ri;
@@ -487,18 +629,19 @@ __device__ int select_member(const KnnArgs& a, int t, const double* lx, const do
         }
         return 0;
     }
-    uint32_t prefix = 0;
-    int need = kq;
-    for (int rd = 0; rd < 4; ++rd) {
-        const int shift = 24 - 8 * rd;
+    // radix rounds: `prefix` holds the bits above `shift` of the bucket of
+    // the kq-th key, `need` its rank inside that bucket, `bcount` its size
+    uint64_t prefix = 0;
+    int need = kq, shift = 32, bcount = total;
+    while (bcount > 64 && shift > 0) {
+        shift -= 8;
 #pragma unroll
         for (int q = 0; q < 4; ++q) hist[4 * lane + q] = 0;
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
-            const uint32_t kk = key[c];
-            if (c < ncl && kk != KEY_NONE &&
-                (rd == 0 || (kk >> (shift + 8)) == (prefix >> (shift + 8))))
+            const uint64_t kk = key[c];
+            if (c < ncl && kk != KEY_NONE && (kk >> (shift + 8)) == (prefix >> (shift + 8)))
                 atomicAdd(&hist[(kk >> shift) & 0xFF], 1);
         }
         __builtin_amdgcn_wave_barrier();
@@ -509,7 +652,7 @@ __device__ int select_member(const KnnArgs& a, int t, const double* lx, const do
         const int incl = gf_wave_incl_scan(sum);
         const int excl = incl - sum;
         const bool here = excl < need && need <= incl;
-        int bsel = 0, nneed = 0;
+        int bsel = 0, nneed = 0, bc = 0;
         if (here) {
             int run = excl;
 #pragma unroll
@@ -517,6 +660,7 @@ __device__ int select_member(const KnnArgs& a, int t, const double* lx, const do
                 if (nneed == 0 && run + h[q] >= need) {
                     bsel = 4 * lane + q;
                     nneed = need - run;
+                    bc = h[q];
                 }
                 run += h[q];
             }
@@ -524,85 +668,112 @@ __device__ int select_member(const KnnArgs& a, int t, const double* lx, const do
         const int src = __ffsll((unsigned long long)__ballot(here)) - 1;
         bsel = __shfl(bsel, src, 64);
         need = __shfl(nneed, src, 64);
-        prefix |= uint32_t(bsel) << shift;
+        bcount = __shfl(bc, src, 64);
+        prefix |= uint64_t(bsel) << shift;
         __builtin_amdgcn_wave_barrier();
     }
-    const uint32_t T = prefix;
-    int eqc = 0;
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) eqc += (c < ncl) && key[c] == T;
-    const int eq = gf_wave_sum_i(eqc);
-    uint64_t tie_take = ~0ull;
-    if (eq > need) {
-        if (eq > 64) return -1;
-        const int te = gf_wave_incl_scan(eqc) - eqc;
-        int o = te;
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-            if (c < ncl && key[c] == T) {
-                const int j = lane + 64 * c;
-                const double dx = xi - lx[j], dy = yi - ly[j], dz = zi - lz[j];
-                tie_d[o] = dx * dx + dy * dy + dz * dz;
-                tie_i[o] = li[j];
-                ++o;
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        tie_take = 0;
-        o = te;
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-            if (c < ncl && key[c] == T) {
-                const double dm = tie_d[o];
-                const int im = tie_i[o];
-                int rank = 0;
-                for (int q = 0; q < eq; ++q) rank += key_less(tie_d[q], tie_i[q], dm, im);
-                if (rank < need) tie_take |= 1ull << c;
-                ++o;
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-    }
-    int selc = 0;
-#pragma unroll
-    for (int c = 0; c < CPL; ++c)
-        selc += (c < ncl) && (key[c] < T || (key[c] == T && ((tie_take >> c) & 1)));
-    int o = gf_wave_incl_scan(selc) - selc;
-    double dmax = 0;
+    if (bcount > 64) return -1;
+    const uint64_t pfx = prefix >> shift;
+    // the boundary bucket, one entry per lane, ranked exactly
+    int eqc = 0, below = 0;
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-        if ((c < ncl) && (key[c] < T || (key[c] == T && ((tie_take >> c) & 1)))) {
-            const int j = lane + 64 * c;
-            const int id = li[j];
-            a.nb[size_t(o) * a.n + t] = a.tree_pos_out ? a.inv[id] : id;
-            ++o;
-            if (key[c] == T) {
-                const double dx = xi - lx[j], dy = yi - ly[j], dz = zi - lz[j];
-                dmax = fmax(dmax, dx * dx + dy * dy + dz * dz);
-            }
+        if (c < ncl && key[c] != KEY_NONE) {
+            const uint64_t kd = uint64_t(key[c]) >> shift;
+            eqc += kd == pfx;
+            below += kd < pfx;
         }
     }
-    dmax = gf_wave_max_d(dmax);
+    int o = gf_wave_incl_scan(eqc) - eqc;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        if (c < ncl && key[c] != KEY_NONE && (uint64_t(key[c]) >> shift) == pfx) {
+            const int j = lane + 64 * c;
+            const double dx = xi - lx[j], dy = yi - ly[j], dz = zi - lz[j];
+            tie_d[o] = dx * dx + dy * dy + dz * dz;
+            tie_p[o] = li[j];
+            ++o;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    const bool bl = lane < bcount;
+    double dm = 0;
+    int pm = 0, rank = 0, same = 0;
+    if (bl) {
+        dm = tie_d[lane];
+        pm = tie_p[lane];
+        int q = 0;
+        for (; q + 4 <= bcount; q += 4) {
+            const double d0 = tie_d[q], d1 = tie_d[q + 1], d2 = tie_d[q + 2], d3 = tie_d[q + 3];
+            rank += (d0 < dm) + (d1 < dm) + (d2 < dm) + (d3 < dm);
+            same += (d0 == dm) + (d1 == dm) + (d2 == dm) + (d3 == dm);
+        }
+        for (; q < bcount; ++q) {
+            const double dq = tie_d[q];
+            rank += dq < dm;
+            same += dq == dm;
+        }
+    }
+    if (__ballot(same > 1) != 0) {
+        // equal distances: the input index decides (sph.py:165-166 order)
+        const int im = bl ? int(a.idx[pm]) : 0;
+        if (bl) tie_i[lane] = im;
+        __builtin_amdgcn_wave_barrier();
+        if (bl && same > 1)
+            for (int q = 0; q < bcount; ++q) rank += tie_d[q] == dm && tie_i[q] < im;
+        __builtin_amdgcn_wave_barrier();
+    }
+    const bool take = bl && rank < need;
+    const double dk2 = gf_wave_max_d(take ? dm : 0.0);
+    // row: the keys below the bucket, then the bucket's selected entries
+    const int nbelow = gf_wave_sum_i(below);
+    o = gf_wave_incl_scan(below) - below;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        if (c < ncl && key[c] != KEY_NONE && (uint64_t(key[c]) >> shift) < pfx) {
+            const int pj = li[lane + 64 * c];
+            a.nb[size_t(o) * a.n + t] = a.tree_pos_out ? pj : int(a.idx[pj]);
+            ++o;
+        }
+    }
+    if (take) {
+        const int ob = nbelow + __popcll(__ballot(take) & gf_lanemask_lt());
+        a.nb[size_t(ob) * a.n + t] = a.tree_pos_out ? pm : int(a.idx[pm]);
+    }
     if (lane == 0) {
-        a.dk[t] = sqrt(dmax);
+        a.dk[t] = sqrt(dk2);
         a.done[t] = 1;
-        a.passes[t] += 1;
+        a.passes[t] = npass + 1;
     }
     return 1;
 }
 
+// One pass of the neighbour solve for the groups of this pass (all groups,
+// or the previous pass's retry list).  A block of 4 waves takes one group.
+// Its unresolved members are served by ranges of members (contiguous in
+// Morton order, so spatially compact), starting with the whole group: the
+// block gathers the sphere around the range's search spheres once, and each
+// wave selects for every fourth member of the range.  A range whose sphere
+// holds more than the list is halved; a single member whose own sphere does
+// not fit shrinks its radius and takes another pass.  Groups it cannot
+// finish (a depth-capped leaf, more than 64 equal keys, a closed radius
+// bracket) go to the heap pass's queue.
 __global__ __launch_bounds__(FT) void k_knn_fast(KnnArgs a) {
 #pragma clang fp contract(off)
     __shared__ double cx[CMAX], cy[CMAX], cz[CMAX];
     __shared__ int ci[CMAX];
     __shared__ int stack[4][FSCAP];
-    __shared__ int hist[4][256];
+    __shared__ __align__(16) int hist[4][256];
     __shared__ double tie_d[4][64];
     __shared__ int tie_i[4][64];
-    __shared__ int ctl[8];
+    __shared__ int tie_p[4][64];
+    __shared__ int ctl[16];
+    __shared__ unsigned long long rstack[16];
+    __shared__ int rmode[16];                   // 0: bounding sphere, 1: union of member spheres
     const int lane = __lane_id(), w = threadIdx.x >> 6;
-    const int ngroups = a.counters[1];
-    if (threadIdx.x < 8) ctl[threadIdx.x] = 0;
+    const int ngroups = a.retry_in ? a.n_retry_in : a.counters[1];
+    const int cap = min(CMAX, a.list_cap);
+    if (threadIdx.x < 16) ctl[threadIdx.x] = 0;
     __syncthreads();
 
     while (true) {
@@ -610,104 +781,190 @@ __global__ __launch_bounds__(FT) void k_knn_fast(KnnArgs a) {
         __syncthreads();
         const int gi = ctl[0];
         if (gi >= ngroups) break;
-        const int G = a.groups[gi];
+        const int gp = a.retry_in ? a.retry_in[gi] : gi;
+        const int G = a.groups[gp];
         const int b0 = a.start[G], np = a.count[G];
         if (np > 64) {
-            if (threadIdx.x == 0) a.ovf[atomicAdd(&a.state[3], 1)] = gi;
+            if (threadIdx.x == 0) a.ovf[atomicAdd(&a.state[3], 1)] = gp;
             __syncthreads();
             continue;
         }
-        // wave 0: the group sphere; group mode gathers once here
-        if (w == 0) {
-            const int t = b0 + lane;
-            const bool mine = lane < np && !a.done[t];
-            int mode = 0;                       // 0 nothing to do, 1 group, 2 member
-            int nc = 0;
-            if (__ballot(mine) != 0) {
-                double xi = 0, yi = 0, zi = 0, ri = 0;
-                if (mine) {
-                    xi = a.tx[t]; yi = a.ty[t]; zi = a.tz[t]; ri = a.r[t];
-                }
-                double x0 = mine ? xi : 1e300, y0 = mine ? yi : 1e300, z0 = mine ? zi : 1e300;
-                double x1 = mine ? xi : -1e300, y1 = mine ? yi : -1e300, z1 = mine ? zi : -1e300;
-                x0 = gf_wave_min_d(x0); y0 = gf_wave_min_d(y0); z0 = gf_wave_min_d(z0);
-                x1 = gf_wave_max_d(x1); y1 = gf_wave_max_d(y1); z1 = gf_wave_max_d(z1);
-                const double bx = 0.5 * (x0 + x1), by = 0.5 * (y0 + y1), bz = 0.5 * (z0 + z1);
-                double reach = 0;
-                if (mine) {
-                    const double dx = xi - bx, dy = yi - by, dz = zi - bz;
-                    reach = sqrt(dx * dx + dy * dy + dz * dz) + ri;
-                }
-                const double R = gf_wave_max_d(reach) * (1.0 + 1e-12) + 1e-300;
-                const double rmax = gf_wave_max_d(ri);
-                mode = 2;
-                if (R <= 2.0 * rmax) {
-                    const int root = containing_node(a, G, bx, by, bz, R);
-                    nc = gather_sphere(a, root, bx, by, bz, R * R, cx, cy, cz, ci, CMAX, stack[0],
-                                       hist[0]);
-                    mode = nc >= 0 ? 1 : (nc == -2 ? 3 : 2);   // a full list: member mode
-                }
+        // members' data, lane m = member m (every wave holds the same)
+        const int tm = b0 + lane;
+        const bool inm = lane < np;
+        double xm = 0, ym = 0, zm = 0, rm = 0;
+        bool mine = false;
+        int pm_ = 0;
+        if (inm) {
+            mine = !a.done[tm];
+            const double4 q = a.p4[tm];
+            xm = q.x; ym = q.y; zm = q.z; rm = a.r[tm];
+            pm_ = a.passes[tm];
+        }
+        const uint64_t todo = __ballot(mine);
+        // ranges still to serve: a stack of bit masks of members in LDS,
+        // depth first (the group splits into at most 8 chunks, a chunk into
+        // halves: at most 8 + 3 entries)
+        int rs = 0;
+        if (todo != 0) {
+            if (threadIdx.x == 0) {
+                rstack[0] = todo;
+                rmode[0] = 0;
             }
-            if (lane == 0) {
-                ctl[1] = mode;
-                ctl[2] = nc;
-            }
+            rs = 1;
         }
         __syncthreads();
-        const int mode = ctl[1], ncg = ctl[2];
-        int handoff = mode == 3;
-        if (mode == 1 || mode == 2) {
-            for (int mm = w; mm < np; mm += 4) {
-                const int t = b0 + mm;
-                if (a.done[t]) continue;
-                double* lx = cx;
-                double* ly = cy;
-                double* lz = cz;
-                int* li = ci;
-                int nc = ncg;
-                if (mode == 2) {
-                    // the member's own sphere, into this wave's quarter
-                    lx = cx + w * CQ; ly = cy + w * CQ; lz = cz + w * CQ; li = ci + w * CQ;
-                    const double xi = a.tx[t], yi = a.ty[t], zi = a.tz[t];
-                    const double ri = a.r[t] * (1.0 + 1e-12) + 1e-300;
-                    const int root = containing_node(a, a.leaf_of[t], xi, yi, zi, ri);
-                    nc = gather_sphere(a, root, xi, yi, zi, ri * ri, lx, ly, lz, li, CQ, stack[w],
-                                       hist[w]);
-                    if (nc == -2) {
-                        handoff = 1;
-                        continue;
-                    }
-                    if (nc < 0) {
-                        // more candidates than the list holds (-nc of them at
-                        // least): shrink the radius towards 1.3 kq inside, or
-                        // bisect inside the bracket; a bracket narrower than 5 %
-                        // (the count jumps past the list) goes to the heap pass
-                        const double rr = a.r[t], lo = a.r_lo[t];
-                        const bool closed = lo > 0 && rr <= lo * 1.05;
-                        if (lane == 0) {
-                            a.passes[t] += 1;
-                            a.r_hi[t] = rr;
-                            double rn = rr * fmin(0.9, fmax(0.3, cbrt(1.3 * a.kq / double(-nc))));
-                            if (rn <= lo) rn = sqrt(lo * rr);
-                            if (!closed) a.r[t] = rn;
-                            atomicAdd(&a.state[0], 1);
+        int handoff = 0, again = 0;
+        while (rs > 0) {
+            --rs;
+            const uint64_t rg = rstack[rs];
+            const int um = rmode[rs];
+            const int nr = __popcll(rg);
+            const bool in_r = (rg >> lane) & 1;
+            const Sphere S = wave_sphere(in_r, xm, ym, zm, rm);
+            int nc = -1;
+            if (um == 1 || nr == 1 || S.R <= GROUP_REACH * S.rmax) {
+                // member spheres of the range (widened like R) in the LDS of
+                // the histograms, free until the selections
+                double4* msph = reinterpret_cast<double4*>(&hist[0][0]);
+                const int nm = (um == 1 && nr > 1) ? nr : 0;
+                if (nm > 0 && w == 0 && in_r) {
+                    const double rr = rm * (1.0 + 1e-12) + 1e-300;
+                    msph[__popcll(rg & gf_lanemask_lt())] = make_double4(xm, ym, zm, rr * rr);
+                }
+                __syncthreads();
+                const int root = containing_node(
+                    a, nr == 1 ? a.leaf_of[b0 + __ffsll((unsigned long long)rg) - 1] : G, S.bx,
+                    S.by, S.bz, S.R);
+                nc = gather_group(a, root, S.bx, S.by, S.bz, S.R * S.R, msph, nm, cx, cy, cz, ci,
+                                  cap, stack[0], stack[1], stack[2], ctl);
+            }
+            if (nc == -2) {
+                handoff = 1;
+            } else if (nc >= 0) {
+                // each wave selects for every fourth member of the range
+                int rank_in_r = 0;
+                for (int mm = 0; mm < np; ++mm) {
+                    if (((rg >> mm) & 1) == 0) continue;
+                    if ((rank_in_r++ & 3) != w) continue;
+                    const double xi = __shfl(xm, mm, 64), yi = __shfl(ym, mm, 64);
+                    const double zi = __shfl(zm, mm, 64), ri = __shfl(rm, mm, 64);
+                    const int npass = __shfl(pm_, mm, 64);
+                    const int res = select_member(a, b0 + mm, xi, yi, zi, ri, npass, cx, cy, cz,
+                                                  ci, nc, hist[w], tie_d[w], tie_i[w], tie_p[w]);
+                    if (res < 0) handoff = 1;
+                    if (res == 0) again = 1;
+                }
+            } else if (nr > 1 && um == 0) {
+                // the bounding sphere is too large: gather the union of the
+                // members' own spheres instead
+                __syncthreads();              // every wave has read rstack[rs]
+                if (threadIdx.x == 0) {
+                    rstack[rs] = rg;
+                    rmode[rs] = 1;
+                }
+                rs += 1;
+            } else if (nr > 1) {
+                // split the range into C chunks of consecutive members (unions
+                // too): as many as its count asks for (2..8) at the group
+                // level, halves below
+                int C = 2;
+                if (rg == todo && ctl[7] > 0) {
+                    const int want = (2 * ctl[7] + cap - 1) / cap;
+                    while (C < want && C < 8) C <<= 1;
+                }
+                C = min(C, nr);
+                __syncthreads();              // every wave has read rstack[rs] and ctl[7]
+                if (threadIdx.x == 0) {
+                    uint64_t rest = rg;
+                    for (int c = 0; c < C; ++c) {
+                        // chunk c takes the next (nr - taken) / (C - c) members
+                        const int take = (__popcll(rest) + (C - c) - 1) / (C - c);
+                        uint64_t part = 0;
+                        for (int q = 0; q < take; ++q) {
+                            const uint64_t bit = rest & (~rest + 1);
+                            part |= bit;
+                            rest ^= bit;
                         }
-                        if (closed) handoff = 1;
-                        continue;
+                        rstack[rs + C - 1 - c] = part;    // first chunk on top
+                        rmode[rs + C - 1 - c] = 1;
                     }
                 }
-                if (select_member(a, t, lx, ly, lz, li, nc, hist[w], tie_d[w], tie_i[w]) < 0)
-                    handoff = 1;
+                rs += C;
+            } else {
+                // one member whose sphere holds more than the list (ctl[7]
+                // particles at least, 0 = unknown): shrink the radius towards
+                // 1.3 kq inside, or bisect inside the bracket; a bracket
+                // narrower than 5 % (the count jumps past the list) goes to
+                // the heap pass
+                const int mm = __ffsll((unsigned long long)rg) - 1;
+                const int t = b0 + mm;
+                const double ri = __shfl(rm, mm, 64);
+                const int npass = __shfl(pm_, mm, 64);
+                const int seen = ctl[7];
+                if (threadIdx.x == 0) {
+                    const double lo = a.r_lo[t];
+                    const bool closed = lo > 0 && ri <= lo * 1.05;
+                    a.passes[t] = npass + 1;
+                    a.r_hi[t] = ri;
+                    double rn = seen > 0 ? ri * fmin(0.9, fmax(0.3, cbrt(1.3 * a.kq / double(seen))))
+                                         : 0.5 * ri;
+                    if (rn <= lo) rn = sqrt(lo * ri);
+                    if (!closed) {
+                        a.r[t] = rn;
+                        atomicAdd(&a.state[0], 1);
+                        ctl[3] |= 2;
+                    } else {
+                        ctl[3] |= 1;
+                    }
+                }
             }
+            // the list is reused by the next range
+            __syncthreads();
         }
         if (lane == 0 && handoff) atomicOr(&ctl[3], 1);
+        if (lane == 0 && again) atomicOr(&ctl[3], 2);
         __syncthreads();
         if (threadIdx.x == 0) {
-            if (ctl[3]) a.ovf[atomicAdd(&a.state[3], 1)] = gi;
+            if (ctl[3] & 1) a.ovf[atomicAdd(&a.state[3], 1)] = gp;
+            if (ctl[3] & 2) request_retry(a, gp);
             ctl[3] = 0;
         }
         __syncthreads();
     }
+}
+
+// Packed gas-tree nodes and particle positions for the neighbour walks.
+__global__ __launch_bounds__(TB) void k_knode_pack(const int* __restrict__ counters,
+                                                   const double4* __restrict__ box,
+                                                   const int8_t* __restrict__ leaf,
+                                                   const int* __restrict__ start,
+                                                   const int* __restrict__ count,
+                                                   const int* __restrict__ child8,
+                                                   KNode* __restrict__ kn) {
+    const int nn = counters[0];
+    for (int id = blockIdx.x * TB + threadIdx.x; id < nn; id += gridDim.x * TB) {
+        KNode k;
+        k.box = box[id];
+        if (leaf[id] || count[id] <= KNN_RANGE) {
+            k.lo = make_int4(KNODE_RANGE, start[id], count[id],
+                             leaf[id] && count[id] > GF_LEAF_SIZE ? 1 : 0);
+            k.hi = make_int4(-1, -1, -1, -1);
+        } else {
+            const int4* cs = reinterpret_cast<const int4*>(child8 + 8 * size_t(id));
+            k.lo = cs[0];
+            k.hi = cs[1];
+        }
+        kn[id] = k;
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_p4_pack(int n, const double* __restrict__ tx,
+                                                const double* __restrict__ ty,
+                                                const double* __restrict__ tz,
+                                                double4* __restrict__ p4) {
+    const int t = blockIdx.x * TB + threadIdx.x;
+    if (t < n) p4[t] = make_double4(tx[t], ty[t], tz[t], 0.0);
 }
 
 // First search radius: the caller's, or from the number density of the
@@ -721,9 +978,11 @@ __global__ __launch_bounds__(TB) void k_knn_init(int n, int kq, const double* __
                                                  double* __restrict__ r, double* __restrict__ r_lo,
                                                  double* __restrict__ r_hi,
                                                  int8_t* __restrict__ done,
-                                                 int* __restrict__ passes) {
+                                                 int* __restrict__ passes,
+                                                 int* __restrict__ gmark) {
     const int t = blockIdx.x * TB + threadIdx.x;
     if (t >= n) return;
+    gmark[t] = 0;                   // groups <= particles
     double rr;
     if (r_init) {
         rr = r_init[idx[t]];
@@ -1047,7 +1306,8 @@ int sph_reserve(GfSph& S, int n, int k) {
             dalloc(S.r, cn) || dalloc(S.dk, cn) || dalloc(S.h, cn) || dalloc(S.rho, cn) ||
             dalloc(S.done, cn) || dalloc(S.passes, cn) ||
             dalloc(S.inv, cn) || dalloc(S.leaf_of, cn) ||
-            dalloc(S.r_lo, cn) || dalloc(S.r_hi, cn) || dalloc(S.rec, 16 * size_t(cn)))
+            dalloc(S.r_lo, cn) || dalloc(S.r_hi, cn) || dalloc(S.rec, 16 * size_t(cn)) ||
+            dalloc(S.p4, 4 * size_t(cn)) || dalloc(S.gmark, cn))
             return -1;
         S.cap_n = cn;
         S.cap_k = 0;
@@ -1058,7 +1318,9 @@ int sph_reserve(GfSph& S, int n, int k) {
     }
     const int ng = gf_tree_node_bound(S.cap_n);
     if (ng > S.cap_groups) {
-        if (dalloc(S.ovf, ng) || dalloc(S.ovf2, ng)) return -1;
+        if (dalloc(S.ovf, ng) || dalloc(S.ovf2, ng) || dalloc(S.retry, 2 * size_t(ng)) ||
+            dalloc(S.knode, 8 * size_t(ng)))
+            return -1;
         S.cap_groups = ng;
     }
     return 0;
@@ -1069,7 +1331,7 @@ int sph_reserve(GfSph& S, int n, int k) {
 void gf_sph_free(GfSph& S) {
     gf_tree_free(S.tree);
     dfree(S.vx); dfree(S.vy); dfree(S.vz); dfree(S.u); dfree(S.r); dfree(S.dk); dfree(S.h);
-    dfree(S.rho); dfree(S.done); dfree(S.passes); dfree(S.inv); dfree(S.leaf_of); dfree(S.r_lo); dfree(S.r_hi); dfree(S.rec);
+    dfree(S.rho); dfree(S.done); dfree(S.passes); dfree(S.inv); dfree(S.leaf_of); dfree(S.r_lo); dfree(S.r_hi); dfree(S.rec); dfree(S.p4); dfree(S.gmark); dfree(S.retry); dfree(S.knode);
     dfree(S.nb); dfree(S.state); dfree(S.ovf); dfree(S.ovf2); dfree(S.gstack);
     S = GfSph();
 }
@@ -1104,9 +1366,20 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
                        T.counters, T.start, T.count, T.leaf, S.leaf_of);
     hipLaunchKernelGGL(k_knn_init, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, kq, r_init, T.idx,
                        S.leaf_of, T.parent, T.count, T.box, S.r, S.r_lo, S.r_hi, S.done,
-                       S.passes);
+                       S.passes, S.gmark);
+    hipLaunchKernelGGL(k_knode_pack, dim3(std::min(2048, gf_div_up(T.cap_nodes, TB))), dim3(TB), 0,
+                       s, T.counters, T.box, T.leaf, T.start, T.count, T.child8,
+                       reinterpret_cast<KNode*>(S.knode));
+    hipLaunchKernelGGL(k_p4_pack, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, T.tx, T.ty, T.tz,
+                       reinterpret_cast<double4*>(S.p4));
     GF_LAUNCH_CHECK();
     KnnArgs a;
+    a.kn = reinterpret_cast<const KNode*>(S.knode);
+    a.p4 = reinterpret_cast<const double4*>(S.p4);
+    a.gmark = S.gmark;
+    a.retry_in = nullptr;
+    a.n_retry_in = 0;
+    a.list_cap = gf_test_limits().knn_list > 0 ? gf_test_limits().knn_list : CMAX;
     a.start = T.start; a.count = T.count; a.leaf = T.leaf; a.child8 = T.child8; a.box = T.box;
     a.groups = T.groups; a.counters = T.counters; a.tx = T.tx; a.ty = T.ty; a.tz = T.tz;
     a.idx = T.idx; a.inv = S.inv; a.parent_ = T.parent; a.leaf_of = S.leaf_of; a.r = S.r;
@@ -1118,11 +1391,14 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
     int pass = 0;
     S.stat_queued = 0;
     S.stat_queued2 = 0;
+    int* rbuf[2] = {S.retry, S.retry + S.cap_groups};
     while (true) {
         if (pass >= max_passes) {
             gf_set_error("kNN: particles still unresolved after %d passes", max_passes);
             return -1;
         }
+        a.retry_out = rbuf[pass & 1];
+        a.pass_tag = pass + 1;
         hipLaunchKernelGGL(k_knn_reset, dim3(1), dim3(64), 0, s, S.state);
         if (!sorted)
             hipLaunchKernelGGL(k_knn_fast, dim3(FAST_BLOCKS), dim3(FT), 0, s, a);
@@ -1145,6 +1421,9 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
         S.stat_queued += st[3];
         S.stat_queued2 += st[6];
         if (st[0] == 0) break;
+        // the next fast pass visits only the groups that asked for it
+        a.retry_in = a.retry_out;
+        a.n_retry_in = st[7];
     }
     S.last_passes = pass;
     return 0;

@@ -65,6 +65,12 @@ int gf_device_info(char* buf, int len);
  * FP64.  Tree, centres of mass and acceptance tests are FP64 in both. */
 int gf_set_precision(int precision);
 int gf_get_precision(void);
+/* Testing hook, no counterpart in the reference: lowers the capacity of
+ * the gravity walk's LDS stack (walk_stack entries, 1..1024) and of the
+ * neighbour solve's candidate lists (knn_list, 1..1024) so that tests can
+ * drive the global-memory and heap fallbacks on small inputs; 0 restores
+ * the default.  Results do not depend on these capacities. */
+int gf_set_test_limits(int walk_stack, int knn_list);
 /* Interactions (sum over evaluated particles of their list length) of the
  * last nbody_accel_active call (the count nbody.cpp:397 loops over). */
 long long gf_last_interactions(void);

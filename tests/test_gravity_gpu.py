@@ -230,3 +230,23 @@ def test_clustered_duplicates(eng):
     acc = run(eng, pos, mass, 0.5, 0.05, "fp64")
     ref, _ = O.accel(pos, mass, 0.5, 0.05, mode=1)
     assert relerr(acc, ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("limit", [1, 64, 300])
+def test_global_stack_walk_equals_lds_walk(eng, limit):
+    """With the LDS stack cut to `limit` entries, groups overflow and are
+    walked again from global-memory stacks: the same nodes in the same order,
+    hence bit-identical accelerations and interaction counts."""
+    from galaxy_formation_amd import _lib
+    L = _lib.load()
+    c = ics.config("cloud10k")
+    ref = {p: (run(eng, c.pos, c.mass, 0.5, c.eps, p), eng.last_interactions())
+           for p in ("fp32", "fp64")}
+    _lib.check(L.gf_set_test_limits(limit, 0))
+    try:
+        for p in ("fp32", "fp64"):
+            acc = run(eng, c.pos, c.mass, 0.5, c.eps, p)
+            np.testing.assert_array_equal(acc, ref[p][0])
+            assert eng.last_interactions() == ref[p][1]
+    finally:
+        _lib.check(L.gf_set_test_limits(0, 0))

@@ -265,3 +265,25 @@ def test_device_neighbours_with_exact_ties():
         nb, dk, _ = _device_solve(pos, k)
         np.testing.assert_array_equal(np.sort(nb, axis=1), np.sort(onb, axis=1))
         np.testing.assert_array_equal(dk, od[:, -1])
+
+
+@pytest.mark.parametrize("k,cap", [(32, 40), (64, 70), (64, 300)])
+def test_neighbours_with_small_candidate_lists(k, cap):
+    """Candidate lists cut to `cap` entries push groups into member mode,
+    radius shrinking and the heap pass; the neighbour sets and d_k stay
+    those of brute force."""
+    from galaxy_formation_amd import _lib
+    L = _lib.load()
+    pos, *_ = disk_gas(6000, seed=13)
+    od, onb = O.knn(pos, k)
+    _lib.check(L.gf_set_test_limits(0, cap))
+    try:
+        nb, dk, passes = _device_solve(pos, k)
+        st = np.zeros(6, np.int64)
+        _lib.check(L.gf_hydro_stats(st, 6))
+    finally:
+        _lib.check(L.gf_set_test_limits(0, 0))
+    np.testing.assert_array_equal(np.sort(nb, axis=1), np.sort(onb, axis=1))
+    np.testing.assert_array_equal(dk, od[:, -1])
+    if cap < 100:
+        assert st[1] > 0          # groups handed to the heap pass
