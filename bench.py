@@ -38,7 +38,11 @@ sys.path.insert(0, ROOT)
 
 FLOP_PER_INTERACTION = 20          # Nyland, Harris & Prins (GPU Gems 3, ch. 31)
 PEAK_FP32 = 157.3e12               # MI355X vector FP32, FLOP/s
+PEAK_FP32_UNPACKED = 78.6e12       # the same without packed FP32 instructions
 PEAK_FP64 = 78.6e12
+# HBM bytes one launch of the FP32 walk moves on the 1M disk: FETCH_SIZE x 2.0
+# + WRITE_SIZE x 1.0 from rocprofv3 --pmc (profiles/k_walk_disk1m_counters.md)
+WALK_TRAFFIC_BYTES = None
 PEAK_HBM = 8.0e12
 GAMMA, ALPHA, BETA = 5.0 / 3.0, 1.0, 2.0
 PHASES = ("tree", "walk", "gas_tree", "knn", "sph")
@@ -85,13 +89,20 @@ class Bench:
         self.stream = torch.cuda.Stream(device=self.dev)
         torch.cuda.set_stream(self.stream)
         self.sp = self.stream.cuda_stream
+        # the gas pipeline runs on a second stream (default priority) beside
+        # the gravity pipeline: the two are independent until the sum
+        self.stream2 = torch.cuda.Stream(device=self.dev)
+        self.sp2 = self.stream2.cuda_stream
+        self.fork = torch.cuda.Event()
+        self.join = torch.cuda.Event()
 
     def p(self, t):
         return None if t is None else t.data_ptr()
 
-    def hydro(self, prec, pos, first, ev=None, rec=None):
+    def hydro(self, prec, pos, first, ev=None, rec=None, stream=None):
         """first: None (cell densities), ("radius", r) or ("learned", model)."""
-        L, chk, sp, o = self.L, self._lib.check, self.sp, self.out[prec]
+        stream = stream or self.stream
+        L, chk, sp, o = self.L, self._lib.check, stream.cuda_stream, self.out[prec]
         rec(2) if rec else None
         chk(L.gf_hydro_setup_dev(self.ng, self.p(pos), self.p(self.vel), self.p(self.mass),
                                  self.p(self.u), sp))
@@ -104,8 +115,8 @@ class Bench:
             chk(L.gf_hydro_cell_density_dev(self.a.k, self.p(self.cell_rho),
                                             self.p(self.cell_count), sp))
             x = model.features(self.a.k, self.dk_prev, self.cell_rho, self.cell_count,
-                               self.divv_prev, self.dtc_prev, self.D, stream=self.stream)
-            r_init = model.propose(x, self.dk_prev, out=self.r_learned, stream=self.stream)
+                               self.divv_prev, self.dtc_prev, self.D, stream=stream)
+            r_init = model.propose(x, self.dk_prev, out=self.r_learned, stream=stream)
         passes = chk(L.gf_hydro_knn_dev(self.a.k, self.p(r_init), sp))
         rec(4) if rec else None
         chk(L.gf_hydro_forces_dev(self.c.eps, GAMMA, ALPHA, BETA,
@@ -114,40 +125,70 @@ class Bench:
         rec(5) if rec else None
         return passes
 
-    def step(self, prec, first, ev=None):
-        L, chk, sp = self.L, self._lib.check, self.sp
+    def step(self, prec, first, ev=None, overlap=True):
+        """Gravity on the first stream, the gas on the second (or, without
+        overlap, after it on the first), joined at the end (events 0-1, 6 on
+        the first stream, 2-5 on the gas stream)."""
+        torch, L, chk, sp = self.torch, self.L, self._lib.check, self.sp
+        gs = self.stream2 if overlap else self.stream
 
-        def rec(i):
+        def rec(i, st):
             if ev is not None:
-                ev[i].record(self.stream)
-        rec(0)
+                ev[i].record(st)
+        rec(0, self.stream)
+        self.fork.record(self.stream)
         chk(L.gf_tree_build_dev(self.n, self.p(self.pos), self.p(self.mass), self.a.theta, sp))
-        rec(1)
+        rec(1, self.stream)
         chk(L.gf_walk_dev(self.n, None, self.p(self.acc_g[prec]), self.c.eps, prec, sp))
-        return self.hydro(prec, self.pos, first, ev, rec)
+        rec(6, self.stream)
+        gs.wait_event(self.fork)
+        with torch.cuda.stream(gs):
+            passes = self.hydro(prec, self.pos, first, ev, lambda i: rec(i, gs), stream=gs)
+        self.join.record(gs)
+        self.stream.wait_event(self.join)
+        return passes
 
-    def timed(self, prec, first, steps=None):
+    def walk_alone(self, prec, launches=5):
+        """Average duration of the walk on a built tree, nothing else running
+        (HIP events on the stream it is launched on)."""
+        torch, L, chk = self.torch, self.L, self._lib.check
+        chk(L.gf_tree_build_dev(self.n, self.p(self.pos), self.p(self.mass), self.a.theta, self.sp))
+        chk(L.gf_walk_dev(self.n, None, self.p(self.acc_g[prec]), self.c.eps, prec, self.sp))
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(self.stream)
+        for _ in range(launches):
+            chk(L.gf_walk_dev(self.n, None, self.p(self.acc_g[prec]), self.c.eps, prec, self.sp))
+        e1.record(self.stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / launches
+
+    def timed(self, prec, first, steps=None, overlap=True):
         torch = self.torch
         steps = steps or self.a.steps
         for _ in range(self.a.warmup):
-            self.step(prec, first)
+            self.step(prec, first, overlap=overlap)
         torch.cuda.synchronize()
-        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(steps)]
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(steps)]
         t0 = torch.cuda.Event(enable_timing=True)
         t1 = torch.cuda.Event(enable_timing=True)
         passes = []
         w0 = time.perf_counter()
         t0.record(self.stream)
         for k in range(steps):
-            passes.append(self.step(prec, first, evs[k]))
+            passes.append(self.step(prec, first, evs[k], overlap=overlap))
         t1.record(self.stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - w0
         self._lib.check(self.L.gf_sync(self.sp))
         res = {"ms": t0.elapsed_time(t1) / steps, "wall_ms": 1e3 * wall / steps,
                "passes": int(np.max(passes))}
-        for i, name in enumerate(PHASES):
-            res[name + "_ms"] = float(np.mean([e[i].elapsed_time(e[i + 1]) for e in evs]))
+        # (start, end) events of each phase; gravity and gas overlap in time
+        spans = {"tree": (0, 1), "walk": (1, 6), "gas_tree": (2, 3), "knn": (3, 4), "sph": (4, 5)}
+        for name in PHASES:
+            i, j = spans[name]
+            res[name + "_ms"] = float(np.mean([e[i].elapsed_time(e[j]) for e in evs]))
         inter = np.zeros(1, np.int64)
         self._lib.check(self.L.gf_interactions_dev(self._lib.ptr(inter)))
         res["interactions"] = int(inter[0])
@@ -208,14 +249,23 @@ def main():
     res["fast_fp32"] = B.timed(0, ("learned", model))
     h_fast = B.out[0]["h"].cpu().numpy()
     rho_fast = B.out[0]["rho"].cpu().numpy()
-    # the neighbour solve from the other first radii (FP32 path, 3 steps each)
-    ab = {"learned": {k: res["fast_fp32"][k] for k in ("knn_ms", "passes",
-                                                        "mean_passes_per_particle")}}
+    # the same steps with gravity and gas one after the other: phase times
+    # without sharing the GPU
+    serial = {"plain_fp64": B.timed(1, None, overlap=False),
+              "fast_fp32": B.timed(0, ("learned", model), overlap=False)}
+    for name in serial:
+        res[name]["serial"] = {k: serial[name][k] for k in ("ms",) + tuple(x + "_ms" for x in PHASES)}
+    # the walk kernel alone, for its roofline
+    walk_ms = B.walk_alone(0)
+    # the neighbour solve from the other first radii (FP32 path, 3 steps each,
+    # serial so that the solve has the GPU to itself)
+    ab = {"learned": {k: serial["fast_fp32"][k] for k in ("knn_ms", "passes",
+                                                          "mean_passes_per_particle")}}
     import math
     for name, first in (("cell_density", None), ("dk_prev", ("radius", B.dk_prev)),
                         ("dk_prev_fixed_margin",
                          ("radius", (B.dk_prev * math.exp(model.margin_fixed)).contiguous()))):
-        r = B.timed(0, first, steps=3)
+        r = B.timed(0, first, steps=3, overlap=False)
         ab[name] = {k: r[k] for k in ("knn_ms", "passes", "mean_passes_per_particle")}
         ab[name]["h_equal_plain"] = bool(np.array_equal(B.out[0]["h"].cpu().numpy(), h_plain))
 
@@ -239,7 +289,7 @@ def main():
 
     fast, plain = res["fast_fp32"], res["plain_fp64"]
     inter = fast["interactions"]
-    achieved = FLOP_PER_INTERACTION * inter / (fast["walk_ms"] * 1e-3)
+    achieved = FLOP_PER_INTERACTION * inter / (walk_ms * 1e-3)
     out = {
         "metric": "force-evals/s at 1M particles, FP32 + learned-h vs plain FP64 GPU",
         "value": n / (fast["ms"] * 1e-3),
@@ -257,10 +307,19 @@ def main():
                                "gravity + SPH (kNN, h, density, forces)",
                    "n": n, "n_gas": ng, "theta": a.theta, "eps": c.eps, "k": a.k,
                    "first_radius_fast": "learned proposal"},
-        "roofline": {"bound": "valu-fp32", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+        # compute-bound ("mfma" names the compute roof): the walk evaluates
+        # on the FP32 vector units, whose published peak is also 157.3
+        # TFLOP/s (packed FP32; 78.6 without packing, which this build does
+        # not use); no MFMA, pair terms are not a contraction
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                     "traffic": WALK_TRAFFIC_BYTES,
                      "kernel": "k_walk<false, false> (FP32 group walk + list evaluation)",
-                     "interactions": inter, "flop_per_interaction": FLOP_PER_INTERACTION},
+                     "launch_ms": walk_ms, "interactions": inter,
+                     "flop_per_interaction": FLOP_PER_INTERACTION,
+                     "unpacked_fp32_peak": PEAK_FP32_UNPACKED / 1e12,
+                     "frac_of_unpacked": achieved / PEAK_FP32_UNPACKED},
+        "roofline_sph": sph_roofline(ng, a.k, fast["serial"]["sph_ms"]),
         "plain_fp64": {"value": n / (plain["ms"] * 1e-3), **plain},
         "fast_fp32": fast,
         "speedup_fast_vs_plain": plain["ms"] / fast["ms"],
@@ -271,6 +330,18 @@ def main():
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(B)
     print(json.dumps(out))
+
+
+def sph_roofline(ng, k, ms):
+    """The density and force passes against HBM: algorithmic bytes per gas
+    particle 4k + 72 (density) and 4k + 120 (force): the int32 neighbour
+    column entries, the particle's own state and outputs; neighbour records
+    counted once per particle, as if cached."""
+    b = ng * ((4 * k + 72) + (4 * k + 120))
+    achieved = b / (ms * 1e-3)
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM, "traffic": None, "bytes_algorithmic": b,
+            "kernels": "k_pack + k_density + k_force", "ms": ms}
 
 
 def cpu_baseline(B):

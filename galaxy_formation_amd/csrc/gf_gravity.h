@@ -11,6 +11,9 @@ struct GfWalkScratch {
     int* ovf_count = nullptr;
     int* gstack = nullptr;                 // global stacks for those groups
     int cap_groups = 0;
+    void* gnode = nullptr;                 // packed nodes (128 bytes each)
+    void* p4 = nullptr;                    // packed particles (x, y, z, m), tree order
+    int cap_gnode = 0, cap_p4 = 0;
 };
 
 int gf_walk_reserve(GfWalkScratch& W, int max_groups);

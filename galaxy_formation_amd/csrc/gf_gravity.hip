@@ -34,7 +34,21 @@ constexpr int GSCAP = 512 * (GF_KEY_LEVELS + 2);    // global stack entries per 
 constexpr int WALK_BLOCKS = 8192;
 constexpr int GST_BLOCKS = 256;
 
+// A tree node packed for the walk (128 bytes): everything one visit reads,
+// so a batch of 64 nodes costs one round of loads.
+struct GNode {
+    double4 com;          // centre of mass, mass (FP64)
+    double open_r;        // opening radius (FP64)
+    int start, count;     // particle range in tree order
+    int leaf, pad0, pad1, pad2;
+    int4 c_lo, c_hi;      // children in octant order, -1 = empty
+    int4 pad3, pad4;
+};
+static_assert(sizeof(GNode) == 128, "GNode layout");
+
 struct WalkArgs {
+    const GNode* gn;
+    const double4* p4;         // (x, y, z, m) per tree position
     const int* start;
     const int* count;
     const int8_t* leaf;
@@ -168,8 +182,9 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
             bool act = false;
             if (in) {
                 act = a.active ? a.active[a.idx[i]] != 0 : true;
-                if (a.tm[i] > 0) {
-                    const double x = a.tx[i], y = a.ty[i], z = a.tz[i];
+                const double4 q = a.p4[i];
+                if (q.w > 0) {
+                    const double x = q.x, y = q.y, z = q.z;
                     x0 = fmin(x0, x); x1 = fmax(x1, x);
                     y0 = fmin(y0, y); y1 = fmax(y1, y);
                     z0 = fmin(z0, z); z1 = fmax(z1, z);
@@ -185,8 +200,9 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
         double br2 = 0;
         for (int c0 = 0; c0 < np; c0 += 64) {
             const int i = b + c0 + lane;
-            if (c0 + lane < np && a.tm[i] > 0) {
-                const double dx = a.tx[i] - bx, dy = a.ty[i] - by, dz = a.tz[i] - bz;
+            const double4 q = c0 + lane < np ? a.p4[i] : make_double4(0, 0, 0, 0);
+            if (c0 + lane < np && q.w > 0) {
+                const double dx = q.x - bx, dy = q.y - by, dz = q.z - bz;
                 br2 = fmax(br2, dx * dx + dy * dy + dz * dz);
             }
         }
@@ -199,10 +215,13 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
             const bool evl = sl < S;
             const int ip = b + c0 + p;
             Acc<F64> A;
-            if constexpr (F64) {
-                A.px = a.tx[ip]; A.py = a.ty[ip]; A.pz = a.tz[ip];
-            } else {
-                A.px = float(a.tx[ip] - bx); A.py = float(a.ty[ip] - by); A.pz = float(a.tz[ip] - bz);
+            {
+                const double4 q = a.p4[ip];
+                if constexpr (F64) {
+                    A.px = q.x; A.py = q.y; A.pz = q.z;
+                } else {
+                    A.px = float(q.x - bx); A.py = float(q.y - by); A.pz = float(q.z - bz);
+                }
             }
             bool chunk_act = false;
             if (lane < P) chunk_act = a.active ? a.active[a.idx[ip]] != 0 : true;
@@ -224,19 +243,23 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                 int4 c_lo = make_int4(-1, -1, -1, -1), c_hi = c_lo;
                 double4 cm = make_double4(0, 0, 0, 0);
                 if (node >= 0) {
-                    cm = a.com[node];
+                    // the whole record in one round of loads
+                    const GNode* g = a.gn + node;
+                    cm = g->com;
+                    const double orr = g->open_r;
+                    const int gs = g->start, gc = g->count, gl = g->leaf;
+                    const int4 k_lo = g->c_lo, k_hi = g->c_hi;
                     const double dx = cm.x - bx, dy = cm.y - by, dz = cm.z - bz;
-                    const double t = a.open_r[node] + br;
+                    const double t = orr + br;
                     if (dx * dx + dy * dy + dz * dz > t * t) {
                         mono = true;
                         nent = 1;
-                    } else if (a.leaf[node]) {
-                        first = a.start[node];
-                        nent = a.count[node];
+                    } else if (gl) {
+                        first = gs;
+                        nent = gc;
                     } else {
-                        const int4* cs = reinterpret_cast<const int4*>(a.child8 + 8 * size_t(node));
-                        c_lo = cs[0];
-                        c_hi = cs[1];
+                        c_lo = k_lo;
+                        c_hi = k_hi;
                         push = (c_lo.x >= 0) + (c_lo.y >= 0) + (c_lo.z >= 0) + (c_lo.w >= 0) +
                                (c_hi.x >= 0) + (c_hi.y >= 0) + (c_hi.z >= 0) + (c_hi.w >= 0);
                     }
@@ -256,12 +279,34 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                         if (cc[k] >= 0) stack[o++] = cc[k];
                 }
                 sn += ptot;
-                // append list entries, evaluating the list whenever it is full
-                int rem = nent, src = 0;
-                while (true) {
-                    const int einc = gf_wave_incl_scan(rem);
-                    const int etot = __shfl(einc, 63, 64);
-                    if (etot == 0) break;
+                // append this batch's entries, evaluating the list whenever it
+                // is full: the accepted nodes' monopoles (written by their
+                // lanes), then the opened leaves' particles, one per lane
+                const uint64_t mmask = __ballot(mono);
+                const int nmono = __popcll(mmask);
+                if (nmono > 0) {
+                    if (ln + nmono > LCAP) {
+                        __syncthreads();
+                        if (evl) A.flush(s_list, ln, sl, S, eps2r);
+                        total_len += ln;
+                        ln = 0;
+                        __syncthreads();
+                    }
+                    if (mono) {
+                        const int slot = ln + __popcll(mmask & gf_lanemask_lt());
+                        if constexpr (F64) {
+                            s_list[slot] = cm;
+                        } else {
+                            s_list[slot] = make_float4(float(cm.x - bx), float(cm.y - by),
+                                                       float(cm.z - bz), float(cm.w));
+                        }
+                    }
+                    ln += nmono;
+                }
+                const int pc = mono ? 0 : nent;
+                const int qinc = gf_wave_incl_scan(pc);
+                const int ptot2 = __shfl(qinc, 63, 64);
+                for (int e0 = 0; e0 < ptot2;) {
                     if (ln == LCAP) {
                         __syncthreads();
                         if (evl) A.flush(s_list, ln, sl, S, eps2r);
@@ -269,31 +314,27 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                         ln = 0;
                         __syncthreads();
                     }
-                    const int space = LCAP - ln;
-                    const int my0 = einc - rem;
-                    const int canw = max(0, min(rem, space - my0));
-                    for (int k = 0; k < canw; ++k) {
-                        const int slot = ln + my0 + k;
-                        if (mono) {
-                            if constexpr (F64) {
-                                s_list[slot] = cm;
-                            } else {
-                                s_list[slot] = make_float4(float(cm.x - bx), float(cm.y - by),
-                                                           float(cm.z - bz), float(cm.w));
-                            }
+                    const int m = min(min(64, LCAP - ln), ptot2 - e0);
+                    const int e = e0 + lane;
+                    // owner lane of entry e: the number of lanes whose
+                    // inclusive count is <= e (all lanes take part)
+                    int own = 0;
+#pragma unroll
+                    for (int step = 32; step > 0; step >>= 1)
+                        if (__shfl(qinc, own + step - 1, 64) <= e) own += step;
+                    const int ofirst = __shfl(first, own, 64);
+                    const int obase = __shfl(qinc - pc, own, 64);
+                    if (lane < m) {
+                        const double4 q = a.p4[ofirst + (e - obase)];
+                        if constexpr (F64) {
+                            s_list[ln + lane] = q;
                         } else {
-                            const int j = first + src + k;
-                            if constexpr (F64) {
-                                s_list[slot] = make_double4(a.tx[j], a.ty[j], a.tz[j], a.tm[j]);
-                            } else {
-                                s_list[slot] = make_float4(float(a.tx[j] - bx), float(a.ty[j] - by),
-                                                           float(a.tz[j] - bz), float(a.tm[j]));
-                            }
+                            s_list[ln + lane] = make_float4(float(q.x - bx), float(q.y - by),
+                                                            float(q.z - bz), float(q.w));
                         }
                     }
-                    rem -= canw;
-                    src += canw;
-                    ln += min(etot, space);
+                    ln += m;
+                    e0 += m;
                 }
                 __syncthreads();
             }
@@ -337,6 +378,40 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                 atomicAdd(a.inter, (unsigned long long)total_len * (unsigned long long)n_act);
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_gnode_pack(const int* __restrict__ counters,
+                                                   const double4* __restrict__ com,
+                                                   const double* __restrict__ open_r,
+                                                   const int* __restrict__ start,
+                                                   const int* __restrict__ count,
+                                                   const int8_t* __restrict__ leaf,
+                                                   const int* __restrict__ child8,
+                                                   GNode* __restrict__ gn) {
+    const int nn = counters[0];
+    for (int id = blockIdx.x * 256 + threadIdx.x; id < nn; id += gridDim.x * 256) {
+        GNode g;
+        g.com = com[id];
+        g.open_r = open_r[id];
+        g.start = start[id];
+        g.count = count[id];
+        g.leaf = leaf[id];
+        g.pad0 = g.pad1 = g.pad2 = 0;
+        const int4* cs = reinterpret_cast<const int4*>(child8 + 8 * size_t(id));
+        g.c_lo = cs[0];
+        g.c_hi = cs[1];
+        g.pad3 = g.pad4 = make_int4(0, 0, 0, 0);
+        gn[id] = g;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gp4_pack(int n, const double* __restrict__ tx,
+                                                 const double* __restrict__ ty,
+                                                 const double* __restrict__ tz,
+                                                 const double* __restrict__ tm,
+                                                 double4* __restrict__ p4) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < n) p4[t] = make_double4(tx[t], ty[t], tz[t], tm[t]);
 }
 
 __global__ void k_walk_reset(int* work, int* ovf_count) {
@@ -429,12 +504,29 @@ int gf_walk_reserve(GfWalkScratch& W, int max_groups) {
     return 0;
 }
 
+// Packed copies of the tree for the walk (sized for cap_nodes nodes and cap_n particles).
+static int walk_pack_reserve(GfWalkScratch& W, int cap_nodes, int cap_n) {
+    if (cap_nodes > W.cap_gnode) {
+        if (W.gnode) GF_HIP(hipFree(W.gnode));
+        GF_HIP(hipMalloc(&W.gnode, sizeof(GNode) * size_t(std::max(1, cap_nodes))));
+        W.cap_gnode = cap_nodes;
+    }
+    if (cap_n > W.cap_p4) {
+        if (W.p4) GF_HIP(hipFree(W.p4));
+        GF_HIP(hipMalloc(&W.p4, sizeof(double4) * size_t(std::max(1, cap_n))));
+        W.cap_p4 = cap_n;
+    }
+    return 0;
+}
+
 void gf_walk_free(GfWalkScratch& W) {
     if (W.work) (void)hipFree(W.work);
     if (W.inter) (void)hipFree(W.inter);
     if (W.ovf_count) (void)hipFree(W.ovf_count);
     if (W.gstack) (void)hipFree(W.gstack);
     if (W.ovf) (void)hipFree(W.ovf);
+    if (W.gnode) (void)hipFree(W.gnode);
+    if (W.p4) (void)hipFree(W.p4);
     W = GfWalkScratch();
 }
 
@@ -445,8 +537,15 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
         gf_set_error("gravity walk without a built tree");
         return -1;
     }
-    if (gf_walk_reserve(W, T.cap_nodes)) return -1;
+    if (gf_walk_reserve(W, T.cap_nodes) || walk_pack_reserve(W, T.cap_nodes, T.n)) return -1;
+    hipLaunchKernelGGL(k_gnode_pack, dim3(std::min(2048, gf_div_up(T.cap_nodes, 256))), dim3(256), 0,
+                       s, T.counters, T.com, T.open_r, T.start, T.count, T.leaf, T.child8,
+                       reinterpret_cast<GNode*>(W.gnode));
+    hipLaunchKernelGGL(k_gp4_pack, dim3(gf_div_up(T.n, 256)), dim3(256), 0, s, T.n, T.tx, T.ty,
+                       T.tz, T.tm, reinterpret_cast<double4*>(W.p4));
     WalkArgs a;
+    a.gn = reinterpret_cast<const GNode*>(W.gnode);
+    a.p4 = reinterpret_cast<const double4*>(W.p4);
     a.start = T.start; a.count = T.count; a.leaf = T.leaf; a.child8 = T.child8;
     a.com = T.com; a.open_r = T.open_r; a.groups = T.groups; a.counters = T.counters;
     a.tx = T.tx; a.ty = T.ty; a.tz = T.tz; a.tm = T.tm; a.idx = T.idx;
