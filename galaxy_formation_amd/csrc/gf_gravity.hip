@@ -30,6 +30,9 @@ namespace {
 
 constexpr int SCAP = 1024;                          // LDS stack entries
 constexpr int LCAP = 256;                           // LDS list entries
+#ifndef WALK_WAVES
+#define WALK_WAVES 4
+#endif
 constexpr int GSCAP = 512 * (GF_KEY_LEVELS + 2);    // global stack entries per block
 constexpr int WALK_BLOCKS = 8192;
 constexpr int GST_BLOCKS = 256;
@@ -147,7 +150,7 @@ struct Acc<true> {
 };
 
 template <bool F64, bool GST>
-__global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
+__global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
     // group sphere and acceptance test: the operations as written (no FMA
     // contraction), so the interaction set is the one the FP64 oracle forms
 #pragma clang fp contract(off)

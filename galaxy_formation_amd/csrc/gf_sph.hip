@@ -1073,35 +1073,39 @@ struct SphRec {
     double4 a, b, c, d;
 };
 
-__global__ __launch_bounds__(TB) void k_pack(SphView v, double gamma, SphRec* __restrict__ rec) {
+__global__ __launch_bounds__(TB) void k_pack(SphView v, double gamma, SphRec* __restrict__ rec,
+                                             double4* __restrict__ a4) {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * TB + threadIdx.x;
     if (t >= v.n) return;
     const double hv = v.h[t];
     SphRec r;
     r.a = make_double4(v.x[t], v.y[t], v.z[t], v.m[t]);
+    a4[t] = r.a;
     r.b = make_double4(v.vx[t], v.vy[t], v.vz[t], sqrt(gamma * (gamma - 1) * v.u[t]));
     r.c = make_double4(hv, 1.0 / hv, 1.0 / (M_PI * (hv * hv * hv * hv)), 0.0);
     r.d = make_double4(0.0, 0.0, 0.0, 0.0);
     rec[t] = r;
 }
 
-// rho_i = sum_j m_j W(r_ij, h_i) over the neighbour table (sph.py:63-71); also
-// completes the particle's record with rho and P / rho^2.
+// rho_i = sum_j m_j W(r_ij, h_i) over the neighbour table (sph.py:63-71),
+// neighbours read from the compact (x, y, z, m) copy (four per cache line);
+// also completes the particle's record with rho and P / rho^2.
 __global__ __launch_bounds__(TB) void k_density(int n, const int* __restrict__ nb, int k,
                                                 const double* __restrict__ u, double gamma,
+                                                const double4* __restrict__ a4,
                                                 SphRec* __restrict__ rec,
                                                 double* __restrict__ rho) {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * TB + threadIdx.x;
     if (t >= n) return;
-    const double4 at = rec[t].a;
+    const double4 at = a4[t];
     const double4 ct = rec[t].c;
     const double hv = ct.x, ihv = ct.y, s3 = 1.0 / (M_PI * (hv * hv * hv));
     double s = 0.0;
     for (int jj = 0; jj < k; ++jj) {
         const int j = nb[size_t(jj) * n + t];
-        const double4 aj = rec[j].a;
+        const double4 aj = a4[j];
         const double dx = at.x - aj.x, dy = at.y - aj.y, dz = at.z - aj.z;
         s += aj.w * kernel_w(sqrt(dx * dx + dy * dy + dz * dz), ihv, s3);
     }
@@ -1446,8 +1450,9 @@ int gf_sph_forces_tree(GfSph& S, int k, double gamma, double alpha, double beta,
     SphView v{S.tree.tx, S.tree.ty, S.tree.tz, S.vx, S.vy, S.vz, S.tree.tm, S.u, S.h, n};
     const int g = gf_div_up(n, TB);
     SphRec* rec = reinterpret_cast<SphRec*>(S.rec);
-    hipLaunchKernelGGL(k_pack, dim3(g), dim3(TB), 0, s, v, gamma, rec);
-    hipLaunchKernelGGL(k_density, dim3(g), dim3(TB), 0, s, n, S.nb, k, S.u, gamma, rec, S.rho);
+    double4* a4 = reinterpret_cast<double4*>(S.p4);      // free after the solve
+    hipLaunchKernelGGL(k_pack, dim3(g), dim3(TB), 0, s, v, gamma, rec, a4);
+    hipLaunchKernelGGL(k_density, dim3(g), dim3(TB), 0, s, n, S.nb, k, S.u, gamma, a4, rec, S.rho);
     hipLaunchKernelGGL(k_force, dim3(g), dim3(TB), 0, s, n, S.nb, k, rec, alpha, beta,
                        S.tree.idx, acc, dudt, dt, divv, rho, h_out);
     GF_LAUNCH_CHECK();
@@ -1465,12 +1470,14 @@ int gf_sph_forces_table(int n, int k, const double* pos, const double* vel, cons
     SphRec* rec = reinterpret_cast<SphRec*>(
         (reinterpret_cast<uintptr_t>(soa + 6 * size_t(n)) + 255) & ~uintptr_t(255));
     double* rr = soa + 6 * size_t(n) + 16 * size_t(n) + 64;
+    double4* a4 = reinterpret_cast<double4*>(
+        (reinterpret_cast<uintptr_t>(rr + size_t(n)) + 255) & ~uintptr_t(255));
     const int g = gf_div_up(n, TB);
     hipLaunchKernelGGL(k_table_prep, dim3(g), dim3(TB), 0, s, n, k, nb, pos, vel, nbt, soa);
     SphView v{soa, soa + n, soa + 2 * size_t(n), soa + 3 * size_t(n), soa + 4 * size_t(n),
               soa + 5 * size_t(n), mass, u, h, n};
-    hipLaunchKernelGGL(k_pack, dim3(g), dim3(TB), 0, s, v, gamma, rec);
-    hipLaunchKernelGGL(k_density, dim3(g), dim3(TB), 0, s, n, nbt, k, u, gamma, rec, rr);
+    hipLaunchKernelGGL(k_pack, dim3(g), dim3(TB), 0, s, v, gamma, rec, a4);
+    hipLaunchKernelGGL(k_density, dim3(g), dim3(TB), 0, s, n, nbt, k, u, gamma, a4, rec, rr);
     hipLaunchKernelGGL(k_force, dim3(g), dim3(TB), 0, s, n, nbt, k, rec, alpha, beta, nullptr, acc,
                        dudt, dt, divv, rho, nullptr);
     GF_LAUNCH_CHECK();
@@ -1478,7 +1485,7 @@ int gf_sph_forces_table(int n, int k, const double* pos, const double* vel, cons
 }
 
 size_t gf_sph_table_scratch_bytes(int n, int k) {
-    return sizeof(int) * (size_t(k) * n + 2) + sizeof(double) * (6 + 16 + 1) * size_t(n) + 1024;
+    return sizeof(int) * (size_t(k) * n + 2) + sizeof(double) * (6 + 16 + 1 + 4) * size_t(n) + 2048;
 }
 
 int gf_sph_export_knn(GfSph& S, int kq, int64_t* nb, double* dist, hipStream_t s) {
