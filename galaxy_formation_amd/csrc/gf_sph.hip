@@ -583,24 +583,29 @@ __device__ __forceinline__ Sphere wave_sphere(bool mine, double xi, double yi, d
 
 // Exact kq nearest of member t (at (xi, yi, zi), radius ri) among the list
 // (lx, ly, lz, li)[0, nc) (li: tree positions), by one wave.  Keys are
-// d^2 / r^2 in 32-bit fixed point (monotone in the FP64 d^2); radix rounds
-// of 8 bits narrow the bucket holding the kq-th key until it holds at most
-// 64 keys, which are then ranked exactly by (d^2, input index), one per lane
-// (input indices are read only when two of them tie in d^2).  Returns 1 when
-// resolved, 0 when the radius was too small (grown), -1 on more than 64
-// equal keys (left for the heap pass).
+// d^2 / r^2 in 32-bit fixed point (monotone in the FP64 d^2).  Pass 1 forms
+// the keys and the histogram of their top 8 bits; radix rounds of 8 bits
+// narrow the bucket holding the kq-th key until it holds at most 64 keys;
+// pass 2 writes the keys below that bucket to the row and gathers the
+// bucket, whose entries are then ranked exactly by (d^2, input index), one
+// per lane (input indices are read only when two of them tie in d^2).
+// Returns 1 when resolved, 0 when the radius was too small (grown), -1 on
+// more than 64 equal keys (left for the heap pass).
 __device__ int select_member(const KnnArgs& a, int t, double xi, double yi, double zi, double ri,
                              int npass, const double* lx, const double* ly, const double* lz,
                              const int* li, int nc, int* hist, double* tie_d, int* tie_i,
                              int* tie_p) {
 #pragma clang fp contract(off)
     const int lane = __lane_id();
+    const uint64_t lt = gf_lanemask_lt();
     const int kq = a.kq;
     const double r2i = ri * ri;
     const double ks = KEY_SCALE / r2i;
     const int ncl = (nc + 63) >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) hist[4 * lane + q] = 0;
+    __builtin_amdgcn_wave_barrier();
     uint32_t key[CPL];
-    int cnt = 0;
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
         uint32_t kk = KEY_NONE;
@@ -609,47 +614,41 @@ __device__ int select_member(const KnnArgs& a, int t, double xi, double yi, doub
             if (j < nc) {
                 const double dx = xi - lx[j], dy = yi - ly[j], dz = zi - lz[j];
                 const double d2 = dx * dx + dy * dy + dz * dz;
-                if (d2 <= r2i) kk = uint32_t(fmin(d2 * ks, KEY_SCALE));
+                if (d2 <= r2i) {
+                    kk = uint32_t(fmin(d2 * ks, KEY_SCALE));
+                    atomicAdd(&hist[kk >> 24], 1);
+                }
             }
         }
         key[c] = kk;
-        cnt += kk != KEY_NONE;
     }
-    const int total = gf_wave_sum_i(cnt);
-    if (total < kq) {
-        if (lane == 0) {
-            a.passes[t] = npass + 1;
-            const double want = double(kq) / double(max(total, 1));
-            double rn = ri * fmin(2.0, fmax(1.2, 1.1 * cbrt(want)));
-            a.r_lo[t] = ri;
-            const double hi = a.r_hi[t];
-            if (rn >= hi) rn = sqrt(ri * hi);       // bisect inside the bracket
-            a.r[t] = rn;
-            atomicAdd(&a.state[0], 1);
-        }
-        return 0;
-    }
+    __builtin_amdgcn_wave_barrier();
     // radix rounds: `prefix` holds the bits above `shift` of the bucket of
     // the kq-th key, `need` its rank inside that bucket, `bcount` its size
     uint64_t prefix = 0;
-    int need = kq, shift = 32, bcount = total;
-    while (bcount > 64 && shift > 0) {
-        shift -= 8;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) hist[4 * lane + q] = 0;
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-            const uint64_t kk = key[c];
-            if (c < ncl && kk != KEY_NONE && (kk >> (shift + 8)) == (prefix >> (shift + 8)))
-                atomicAdd(&hist[(kk >> shift) & 0xFF], 1);
-        }
-        __builtin_amdgcn_wave_barrier();
+    int need = kq, shift = 24, bcount = 0;
+    while (true) {
         int h[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) h[q] = hist[4 * lane + q];
         const int sum = h[0] + h[1] + h[2] + h[3];
         const int incl = gf_wave_incl_scan(sum);
+        if (shift == 24) {
+            const int total = __shfl(incl, 63, 64);
+            if (total < kq) {
+                if (lane == 0) {
+                    a.passes[t] = npass + 1;
+                    const double want = double(kq) / double(max(total, 1));
+                    double rn = ri * fmin(2.0, fmax(1.2, 1.1 * cbrt(want)));
+                    a.r_lo[t] = ri;
+                    const double hi = a.r_hi[t];
+                    if (rn >= hi) rn = sqrt(ri * hi);       // bisect inside the bracket
+                    a.r[t] = rn;
+                    atomicAdd(&a.state[0], 1);
+                }
+                return 0;
+            }
+        }
         const int excl = incl - sum;
         const bool here = excl < need && need <= incl;
         int bsel = 0, nneed = 0, bc = 0;
@@ -671,28 +670,47 @@ __device__ int select_member(const KnnArgs& a, int t, double xi, double yi, doub
         bcount = __shfl(bc, src, 64);
         prefix |= uint64_t(bsel) << shift;
         __builtin_amdgcn_wave_barrier();
+        if (bcount <= 64 || shift == 0) break;
+        shift -= 8;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) hist[4 * lane + q] = 0;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const uint64_t kk = key[c];
+            if (c < ncl && kk != KEY_NONE && (kk >> (shift + 8)) == (prefix >> (shift + 8)))
+                atomicAdd(&hist[(kk >> shift) & 0xFF], 1);
+        }
+        __builtin_amdgcn_wave_barrier();
     }
     if (bcount > 64) return -1;
     const uint64_t pfx = prefix >> shift;
-    // the boundary bucket, one entry per lane, ranked exactly
-    int eqc = 0, below = 0;
+    // pass 2: keys below the bucket go to the row, the bucket to the tie lists
+    // (both compacted by ballots: slot by slot, lanes in order)
+    const size_t n = size_t(a.n);
+    int ob = 0, tb = 0;
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-        if (c < ncl && key[c] != KEY_NONE) {
+        if (c < ncl) {
             const uint64_t kd = uint64_t(key[c]) >> shift;
-            eqc += kd == pfx;
-            below += kd < pfx;
-        }
-    }
-    int o = gf_wave_incl_scan(eqc) - eqc;
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-        if (c < ncl && key[c] != KEY_NONE && (uint64_t(key[c]) >> shift) == pfx) {
+            const bool valid = key[c] != KEY_NONE;
+            const bool isb = valid && kd < pfx;
+            const bool ist = valid && kd == pfx;
+            const uint64_t mb = __ballot(isb);
+            const uint64_t mt = __ballot(ist);
             const int j = lane + 64 * c;
-            const double dx = xi - lx[j], dy = yi - ly[j], dz = zi - lz[j];
-            tie_d[o] = dx * dx + dy * dy + dz * dz;
-            tie_p[o] = li[j];
-            ++o;
+            if (isb) {
+                const int pj = li[j];
+                a.nb[size_t(ob + __popcll(mb & lt)) * n + t] = a.tree_pos_out ? pj : int(a.idx[pj]);
+            }
+            if (ist) {
+                const int o = tb + __popcll(mt & lt);
+                const double dx = xi - lx[j], dy = yi - ly[j], dz = zi - lz[j];
+                tie_d[o] = dx * dx + dy * dy + dz * dz;
+                tie_p[o] = li[j];
+            }
+            ob += __popcll(mb);
+            tb += __popcll(mt);
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -725,20 +743,9 @@ __device__ int select_member(const KnnArgs& a, int t, double xi, double yi, doub
     }
     const bool take = bl && rank < need;
     const double dk2 = gf_wave_max_d(take ? dm : 0.0);
-    // row: the keys below the bucket, then the bucket's selected entries
-    const int nbelow = gf_wave_sum_i(below);
-    o = gf_wave_incl_scan(below) - below;
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-        if (c < ncl && key[c] != KEY_NONE && (uint64_t(key[c]) >> shift) < pfx) {
-            const int pj = li[lane + 64 * c];
-            a.nb[size_t(o) * a.n + t] = a.tree_pos_out ? pj : int(a.idx[pj]);
-            ++o;
-        }
-    }
     if (take) {
-        const int ob = nbelow + __popcll(__ballot(take) & gf_lanemask_lt());
-        a.nb[size_t(ob) * a.n + t] = a.tree_pos_out ? pm : int(a.idx[pm]);
+        const int o = ob + __popcll(__ballot(take) & lt);
+        a.nb[size_t(o) * n + t] = a.tree_pos_out ? pm : int(a.idx[pm]);
     }
     if (lane == 0) {
         a.dk[t] = sqrt(dk2);
@@ -758,7 +765,7 @@ __device__ int select_member(const KnnArgs& a, int t, double xi, double yi, doub
 // not fit shrinks its radius and takes another pass.  Groups it cannot
 // finish (a depth-capped leaf, more than 64 equal keys, a closed radius
 // bracket) go to the heap pass's queue.
-__global__ __launch_bounds__(FT) void k_knn_fast(KnnArgs a) {
+__global__ __launch_bounds__(FT, 3) void k_knn_fast(KnnArgs a) {
 #pragma clang fp contract(off)
     __shared__ double cx[CMAX], cy[CMAX], cz[CMAX];
     __shared__ int ci[CMAX];
