@@ -70,13 +70,15 @@ struct KnnArgs {
     int tree_pos_out, sorted;
     int* state;     // [0] unresolved, [1] cursor of the fast pass, [2] cursor of the
                     // heap pass, [3] queue 1 length, [4] errors, [5] cursor of the
-                    // global-stack heap pass, [6] queue 2 length
+                    // global-stack heap pass, [6] queue 2 length, [7] retry list
+                    // length; [8] groups of this pass's retry list; [9], [10]
+                    // queue 1 and 2 lengths summed over passes; [11] passes that
+                    // had work; [12] errors summed over passes
     int* ovf;       // queue 1: groups the fast pass hands to the heap pass
     int* ovf2;      // queue 2: groups the heap pass hands to its global-stack variant
     int* gstack;
     int use_queue;  // the heap pass walks queue 1 (else every group)
-    const int* retry_in;    // groups of this pass (null = all)
-    int n_retry_in;
+    const int* retry_in;    // groups of this pass (null = all; count in state[8])
     int* retry_out;         // groups that need another pass (count in state[7])
     int* gmark;             // pass tag of the last retry request per group
     int pass_tag;
@@ -114,6 +116,7 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
     const int scap = GST ? KGSCAP : KSCAP;
     const int* list = GST ? a.ovf2 : (a.use_queue ? a.ovf : nullptr);
     const int ngroups = GST ? a.state[6] : (a.use_queue ? a.state[3] : a.counters[1]);
+    if (ngroups == 0) return;          // nothing queued (the common case)
     int* cursor = a.state + (GST ? 5 : 2);
     const int kq = a.kq;
 
@@ -339,8 +342,21 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
     }
 }
 
-__global__ void k_knn_reset(int* state) {
-    if (threadIdx.x < 8) state[threadIdx.x] = 0;   // [7]: retry list length
+// Between passes: this pass visits the previous pass's retry list (all
+// groups on the first pass); the per-pass counters restart; totals carry on.
+__global__ void k_knn_reset(int* state, int first) {
+    if (threadIdx.x != 0) return;
+    if (first) {
+        state[9] = state[10] = state[11] = state[12] = 0;
+    } else {
+        state[9] += state[3];
+        state[10] += state[6];
+        state[12] += state[4];
+    }
+    const int work = first ? 1 : state[7];
+    state[8] = first ? 0 : state[7];
+    state[11] += work > 0;
+    for (int i = 0; i < 8; ++i) state[i] = 0;
 }
 
 // ----------------------------------------------------- fast neighbour pass
@@ -778,7 +794,7 @@ __global__ __launch_bounds__(FT, 3) void k_knn_fast(KnnArgs a) {
     __shared__ unsigned long long rstack[16];
     __shared__ int rmode[16];                   // 0: bounding sphere, 1: union of member spheres
     const int lane = __lane_id(), w = threadIdx.x >> 6;
-    const int ngroups = a.retry_in ? a.n_retry_in : a.counters[1];
+    const int ngroups = a.retry_in ? a.state[8] : a.counters[1];
     const int cap = min(CMAX, a.list_cap);
     if (threadIdx.x < 16) ctl[threadIdx.x] = 0;
     __syncthreads();
@@ -1389,7 +1405,6 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
     a.p4 = reinterpret_cast<const double4*>(S.p4);
     a.gmark = S.gmark;
     a.retry_in = nullptr;
-    a.n_retry_in = 0;
     a.list_cap = gf_test_limits().knn_list > 0 ? gf_test_limits().knn_list : CMAX;
     a.start = T.start; a.count = T.count; a.leaf = T.leaf; a.child8 = T.child8; a.box = T.box;
     a.groups = T.groups; a.counters = T.counters; a.tx = T.tx; a.ty = T.ty; a.tz = T.tz;
@@ -1399,43 +1414,47 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
     a.state = S.state; a.ovf = S.ovf; a.ovf2 = S.ovf2; a.gstack = S.gstack;
     // the fast pass writes unordered rows; sorted rows come from the heap pass
     a.use_queue = sorted ? 0 : 1;
-    int pass = 0;
+    // passes are enqueued two at a time and read their work from the device
+    // (a pass after convergence finds nothing to do), so the host waits once
+    // per two passes
     S.stat_queued = 0;
     S.stat_queued2 = 0;
     int* rbuf[2] = {S.retry, S.retry + S.cap_groups};
+    int pass = 0;
+    int st[16];
     while (true) {
         if (pass >= max_passes) {
             gf_set_error("kNN: particles still unresolved after %d passes", max_passes);
             return -1;
         }
-        a.retry_out = rbuf[pass & 1];
-        a.pass_tag = pass + 1;
-        hipLaunchKernelGGL(k_knn_reset, dim3(1), dim3(64), 0, s, S.state);
-        if (!sorted)
-            hipLaunchKernelGGL(k_knn_fast, dim3(FAST_BLOCKS), dim3(FT), 0, s, a);
-        if (kq <= 64) {
-            hipLaunchKernelGGL((k_knn_pass<64, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
-            hipLaunchKernelGGL((k_knn_pass<64, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);
-        } else {
-            hipLaunchKernelGGL((k_knn_pass<128, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
-            hipLaunchKernelGGL((k_knn_pass<128, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);
+        for (int q = 0; q < 2 && pass < max_passes; ++q, ++pass) {
+            // the previous pass's retry list is this pass's work
+            a.retry_in = pass == 0 ? nullptr : rbuf[(pass - 1) & 1];
+            a.retry_out = rbuf[pass & 1];
+            a.pass_tag = pass + 1;
+            hipLaunchKernelGGL(k_knn_reset, dim3(1), dim3(64), 0, s, S.state, pass == 0 ? 1 : 0);
+            if (!sorted)
+                hipLaunchKernelGGL(k_knn_fast, dim3(FAST_BLOCKS), dim3(FT), 0, s, a);
+            if (kq <= 64) {
+                hipLaunchKernelGGL((k_knn_pass<64, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
+                hipLaunchKernelGGL((k_knn_pass<64, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);
+            } else {
+                hipLaunchKernelGGL((k_knn_pass<128, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
+                hipLaunchKernelGGL((k_knn_pass<128, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);
+            }
+            GF_LAUNCH_CHECK();
         }
-        GF_LAUNCH_CHECK();
-        ++pass;
-        int st[8];
         GF_HIP(hipMemcpyAsync(st, S.state, sizeof(st), hipMemcpyDeviceToHost, s));
         GF_HIP(hipStreamSynchronize(s));
-        if (st[4] != 0) {
-            gf_set_error("kNN: %d groups overflowed the global stack", st[4]);
+        if (st[4] != 0 || st[12] != 0) {
+            gf_set_error("kNN: %d groups overflowed the global stack", st[4] + st[12]);
             return -1;
         }
-        S.stat_queued += st[3];
-        S.stat_queued2 += st[6];
         if (st[0] == 0) break;
-        // the next fast pass visits only the groups that asked for it
-        a.retry_in = a.retry_out;
-        a.n_retry_in = st[7];
     }
+    S.stat_queued = st[9] + st[3];
+    S.stat_queued2 = st[10] + st[6];
+    pass = st[11];
     S.last_passes = pass;
     return 0;
 }

@@ -57,7 +57,7 @@ def main(argv=None):
     ap.add_argument("--out", default=DEFAULT_OUT)
     ap.add_argument("--samples", type=int, default=2_000_000)
     ap.add_argument("--train-steps", type=int, default=3000)
-    ap.add_argument("--quantile", type=float, default=0.97)
+    ap.add_argument("--quantile", type=float, default=0.999)
     a = ap.parse_args(argv)
     import torch
 
