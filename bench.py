@@ -41,8 +41,9 @@ PEAK_FP32 = 157.3e12               # MI355X vector FP32, FLOP/s
 PEAK_FP32_UNPACKED = 78.6e12       # the same without packed FP32 instructions
 PEAK_FP64 = 78.6e12
 # HBM bytes one launch of the FP32 walk moves on the 1M disk: FETCH_SIZE x 2.0
-# + WRITE_SIZE x 1.0 from rocprofv3 --pmc (profiles/k_walk_disk1m_counters.md)
-WALK_TRAFFIC_BYTES = None
+# + WRITE_SIZE x 1.0 from rocprofv3 --pmc (profiles/k_walk_disk1m_trace_and_bytes.md);
+# counters cannot be read inside a run, so the measured figure is carried here
+WALK_TRAFFIC_BYTES = 3440326848
 PEAK_HBM = 8.0e12
 GAMMA, ALPHA, BETA = 5.0 / 3.0, 1.0, 2.0
 PHASES = ("tree", "walk", "gas_tree", "knn", "sph")
@@ -287,6 +288,8 @@ def main():
     res["fast_fp32"]["h_equal_plain"] = bool(np.array_equal(h_fast, h_plain))
     res["fast_fp32"]["rho_max_rel_err_vs_plain"] = float(np.max(np.abs(rho_fast - rho_plain) / rho_plain))
 
+    accuracy = accuracy_table(B)
+
     fast, plain = res["fast_fp32"], res["plain_fp64"]
     inter = fast["interactions"]
     achieved = FLOP_PER_INTERACTION * inter / (walk_ms * 1e-3)
@@ -313,7 +316,7 @@ def main():
         # not use); no MFMA, pair terms are not a contraction
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": WALK_TRAFFIC_BYTES,
+                     "traffic": WALK_TRAFFIC_BYTES if a.workload == "disk1m" else None,
                      "kernel": "k_walk<false, false> (FP32 group walk + list evaluation)",
                      "launch_ms": walk_ms, "interactions": inter,
                      "flop_per_interaction": FLOP_PER_INTERACTION,
@@ -326,10 +329,55 @@ def main():
         "first_radius_ab": ab,
         "learned_margin": model.margin, "fixed_margin": model.margin_fixed,
         "previous_step_D": D,
+        "accuracy": accuracy,
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(B)
     print(json.dumps(out))
+
+
+def accuracy_table(B):
+    """RMS relative acceleration error on the seeded 2,000-particle subsets of
+    the reference fixtures (tests/golden: written by the reference
+    cpp/nbody.cpp built from source, FP32 Barnes-Hut at the same theta), for
+    the FP32 and FP64 walks and for the reference itself, against the FP64
+    direct sum; and the RMS difference of each walk from the reference."""
+    torch, L, chk = B.torch, B.L, B._lib.check
+    from galaxy_formation_amd import ics
+    rows = []
+    for name, wl, theta in (("bh_cloud200k_t04", "cloud200k", 0.4),
+                            ("bh_cloud200k_t05", "cloud200k", 0.5),
+                            ("bh_cloud200k_t07", "cloud200k", 0.7),
+                            ("bh_disk1m_t05", "disk1m", 0.5)):
+        path = os.path.join(ROOT, "tests", "golden", name + ".npz")
+        if not os.path.exists(path):
+            continue
+        with np.load(path, allow_pickle=False) as f:
+            sub, ref = f["sub"], f["acc_sub"]
+        c = ics.config(wl)
+        pos = torch.from_numpy(np.ascontiguousarray(c.pos)).to(B.dev)
+        mass = torch.from_numpy(np.ascontiguousarray(c.mass)).to(B.dev)
+        subt = torch.from_numpy(sub.astype(np.int64)).to(B.dev)
+        exact = torch.zeros((len(sub), 3), dtype=torch.float64, device=B.dev)
+        chk(L.gf_direct_dev(c.n, B.p(pos), B.p(mass), len(sub), B.p(subt), B.p(exact), c.eps, B.sp))
+        acc = torch.zeros((c.n, 3), dtype=torch.float64, device=B.dev)
+        out = {"workload": wl, "n": c.n, "theta": theta}
+        torch.cuda.synchronize()
+        ex = exact.cpu().numpy()
+
+        def rms(a, b):
+            e = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+            return float(np.sqrt(np.mean(e ** 2)))
+        out["reference_vs_direct"] = rms(ref, ex)
+        for label, prec in (("fp32", 0), ("fp64", 1)):
+            chk(L.gf_gravity_dev(c.n, B.p(pos), B.p(mass), B.p(acc), theta, c.eps, None, 1, prec,
+                                 B.sp))
+            torch.cuda.synchronize()
+            a = acc.cpu().numpy()[sub]
+            out[label + "_vs_direct"] = rms(a, ex)
+            out[label + "_vs_reference"] = rms(a, ref)
+        rows.append(out)
+    return rows
 
 
 def sph_roofline(ng, k, ms):
