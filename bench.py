@@ -80,11 +80,11 @@ class Bench:
         T = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(self.dev)  # noqa: E731
         self.pos, self.vel, self.mass, self.u = T(c.pos), T(c.vel), T(c.mass), T(c.u)
         z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.dev)  # noqa: E731
-        self.acc_g = {0: z(self.n, 3), 1: z(self.n, 3)}
+        self.acc_g = {0: z(self.n, 3), 1: z(self.n, 3), 2: z(self.n, 3)}
         self.cell_rho, self.cell_count, self.r_learned = z(self.ng), z(self.ng), z(self.ng)
         self.dk_prev, self.divv_prev, self.dtc_prev, self.D = z(self.ng), z(self.ng), z(self.ng), 0.0
         self.out = {p: dict(h=z(self.ng), rho=z(self.ng), acc=z(self.ng, 3), dudt=z(self.ng),
-                            dt=z(self.ng), divv=z(self.ng)) for p in (0, 1)}
+                            dt=z(self.ng), divv=z(self.ng)) for p in (0, 1, 2)}
         # a stream of its own (default priority): the library then runs on the
         # stream the events are recorded on (handle 0 would select its own)
         self.stream = torch.cuda.Stream(device=self.dev)
@@ -250,14 +250,19 @@ def main():
     res["fast_fp32"] = B.timed(0, ("learned", model))
     h_fast = B.out[0]["h"].cpu().numpy()
     rho_fast = B.out[0]["rho"].cpu().numpy()
+    res["fast_fp32_quad"] = B.timed(2, ("learned", model))
     # the same steps with gravity and gas one after the other: phase times
     # without sharing the GPU
     serial = {"plain_fp64": B.timed(1, None, overlap=False),
-              "fast_fp32": B.timed(0, ("learned", model), overlap=False)}
+              "fast_fp32": B.timed(0, ("learned", model), overlap=False),
+              "fast_fp32_quad": B.timed(2, ("learned", model), overlap=False)}
     for name in serial:
         res[name]["serial"] = {k: serial[name][k] for k in ("ms",) + tuple(x + "_ms" for x in PHASES)}
     # the walk kernel alone, for its roofline
     walk_ms = B.walk_alone(0)
+    walk_quad_ms = B.walk_alone(2)
+    node_inter = np.zeros(1, np.int64)
+    B._lib.check(B.L.gf_node_interactions_dev(B._lib.ptr(node_inter)))
     # the neighbour solve from the other first radii (FP32 path, 3 steps each,
     # serial so that the solve has the GPU to itself)
     ab = {"learned": {k: serial["fast_fp32"][k] for k in ("knn_ms", "passes",
@@ -278,7 +283,7 @@ def main():
                                    c.eps, B.sp))
     torch.cuda.synchronize()
     ex = exact.cpu().numpy()
-    for name, prec in (("plain_fp64", 1), ("fast_fp32", 0)):
+    for name, prec in (("plain_fp64", 1), ("fast_fp32", 0), ("fast_fp32_quad", 2)):
         ap = B.acc_g[prec].cpu().numpy()[sub]
         e = np.linalg.norm(ap - ex, axis=1) / np.linalg.norm(ex, axis=1)
         res[name]["grav_rms_rel_err_vs_direct"] = float(np.sqrt(np.mean(e ** 2)))
@@ -323,9 +328,18 @@ def main():
                      "unpacked_fp32_peak": PEAK_FP32_UNPACKED / 1e12,
                      "frac_of_unpacked": achieved / PEAK_FP32_UNPACKED},
         "roofline_sph": sph_roofline(ng, a.k, fast["serial"]["sph_ms"]),
+        # the quadrupole walk: node entries cost 52 flop (20 + Q.d, d.Q.d and
+        # the two extra terms), particle entries 20
+        "roofline_quad": {"bound": "mfma", "kernel": "k_walk<false, false, true>",
+                          "launch_ms": walk_quad_ms, "node_interactions": int(node_inter[0]),
+                          "achieved": (20 * inter + 32 * int(node_inter[0])) / (walk_quad_ms * 1e-3) / 1e12,
+                          "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                          "frac": (20 * inter + 32 * int(node_inter[0])) / (walk_quad_ms * 1e-3) / PEAK_FP32},
         "plain_fp64": {"value": n / (plain["ms"] * 1e-3), **plain},
         "fast_fp32": fast,
         "speedup_fast_vs_plain": plain["ms"] / fast["ms"],
+        "fast_fp32_quad": res["fast_fp32_quad"],
+        "speedup_fast_quad_vs_plain": plain["ms"] / res["fast_fp32_quad"]["ms"],
         "first_radius_ab": ab,
         "learned_margin": model.margin, "fixed_margin": model.margin_fixed,
         "previous_step_D": D,
@@ -369,7 +383,7 @@ def accuracy_table(B):
             e = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
             return float(np.sqrt(np.mean(e ** 2)))
         out["reference_vs_direct"] = rms(ref, ex)
-        for label, prec in (("fp32", 0), ("fp64", 1)):
+        for label, prec in (("fp32", 0), ("fp64", 1), ("fp32_quad", 2)):
             chk(L.gf_gravity_dev(c.n, B.p(pos), B.p(mass), B.p(acc), theta, c.eps, None, 1, prec,
                                  B.sp))
             torch.cuda.synchronize()

@@ -42,6 +42,7 @@ SIGNATURES = {
     "gf_walk_dev": (_ci, [_ci, _vp, _vp, _cd, _ci, _vp]),
     "gf_direct_dev": (_ci, [_ci, _vp, _vp, _ci, _vp, _vp, _cd, _vp]),
     "gf_interactions_dev": (_ci, [_vp]),
+    "gf_node_interactions_dev": (_ci, [_vp]),
     "gf_sync": (_ci, [_vp]),
     # SPH
     "gf_sph_kernel": (_ci, [_ci, _dbl, _dbl, _vp, _vp]),

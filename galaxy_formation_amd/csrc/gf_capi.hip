@@ -85,7 +85,7 @@ static int gravity_impl(int n, const double* pos, const double* mass, double* ac
     }
     if (n == 0) return 0;
     if (gf_walk_reserve(g_ctx.walk, T.cap_nodes)) return -1;
-    GF_HIP(hipMemsetAsync(g_ctx.walk.inter, 0, sizeof(unsigned long long), s));
+    GF_HIP(hipMemsetAsync(g_ctx.walk.inter, 0, 2 * sizeof(unsigned long long), s));
     return gf_walk(T, g_ctx.walk, active, acc, eps, precision, s);
 }
 
@@ -125,8 +125,9 @@ int gf_set_test_limits(int walk_stack, int knn_list) {
 }
 
 int gf_set_precision(int p) {
-    if (p != GF_FP32 && p != GF_FP64) {
-        gf_set_error("precision must be 0 (FP32 interactions) or 1 (FP64)");
+    if (p != GF_FP32 && p != GF_FP64 && p != GF_FP32_QUAD) {
+        gf_set_error("precision must be 0 (FP32 interactions), 1 (FP64) or 2 (FP32 with "
+                     "node quadrupoles)");
         return -1;
     }
     g_ctx.precision = p;
@@ -241,7 +242,7 @@ int gf_tree_export(int cap, int* start, int* count, int* next, int* leaf, int* d
 int gf_gravity_dev(int n, const double* pos, const double* mass, double* acc, double theta,
                    double eps, const uint8_t* active, int rebuild, int precision, void* stream) {
     if (gf_ctx_init()) return -1;
-    if (n < 0 || (precision != GF_FP32 && precision != GF_FP64)) {
+    if (n < 0 || (precision != GF_FP32 && precision != GF_FP64 && precision != GF_FP32_QUAD)) {
         gf_set_error("gf_gravity_dev: bad n or precision");
         return -1;
     }
@@ -265,14 +266,14 @@ int gf_walk_dev(int n, const uint8_t* active, double* acc, double eps, int preci
         gf_set_error("gf_walk_dev: no tree built for n = %d", n);
         return -1;
     }
-    if (precision != GF_FP32 && precision != GF_FP64) {
+    if (precision != GF_FP32 && precision != GF_FP64 && precision != GF_FP32_QUAD) {
         gf_set_error("gf_walk_dev: bad precision");
         return -1;
     }
     if (n == 0) return 0;
     hipStream_t s = gf_stream(stream);
     if (gf_walk_reserve(g_ctx.walk, g_ctx.tree.cap_nodes)) return -1;
-    GF_HIP(hipMemsetAsync(g_ctx.walk.inter, 0, sizeof(unsigned long long), s));
+    GF_HIP(hipMemsetAsync(g_ctx.walk.inter, 0, 2 * sizeof(unsigned long long), s));
     return gf_walk(g_ctx.tree, g_ctx.walk, active, acc, eps, precision, s);
 }
 
@@ -287,6 +288,15 @@ int gf_interactions_dev(long long* out) {
     if (gf_ctx_init()) return -1;
     unsigned long long v = 0;
     if (g_ctx.walk.inter) GF_HIP(hipMemcpy(&v, g_ctx.walk.inter, sizeof(v), hipMemcpyDeviceToHost));
+    *out = (long long)v;
+    return 0;
+}
+
+int gf_node_interactions_dev(long long* out) {
+    if (gf_ctx_init()) return -1;
+    unsigned long long v = 0;
+    if (g_ctx.walk.inter_nodes)
+        GF_HIP(hipMemcpy(&v, g_ctx.walk.inter_nodes, sizeof(v), hipMemcpyDeviceToHost));
     *out = (long long)v;
     return 0;
 }

@@ -2,11 +2,13 @@
 #pragma once
 #include "gf_tree.h"
 
-enum GfPrecision { GF_FP32 = 0, GF_FP64 = 1 };
+// FP32 monopoles, FP64 monopoles, FP32 monopoles + node quadrupoles
+enum GfPrecision { GF_FP32 = 0, GF_FP64 = 1, GF_FP32_QUAD = 2 };
 
 struct GfWalkScratch {
     int* work = nullptr;                   // 1 + 8 ints: group cursor(s)
-    unsigned long long* inter = nullptr;   // interaction counter
+    unsigned long long* inter = nullptr;   // interaction counter ([0] all, [1] node entries)
+    unsigned long long* inter_nodes = nullptr;
     int* ovf = nullptr;                    // groups whose LDS stack overflowed
     int* ovf_count = nullptr;
     int* gstack = nullptr;                 // global stacks for those groups

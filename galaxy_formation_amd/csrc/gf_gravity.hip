@@ -30,6 +30,8 @@ namespace {
 
 constexpr int SCAP = 1024;                          // LDS stack entries
 constexpr int LCAP = 256;                           // LDS list entries
+constexpr int LCAP_QUAD = 192;                      // particle list with quadrupoles
+constexpr int QCAP = 64;                            // node list with quadrupoles (48 B each)
 #ifndef WALK_WAVES
 #define WALK_WAVES 4
 #endif
@@ -45,7 +47,8 @@ struct GNode {
     int start, count;     // particle range in tree order
     int leaf, pad0, pad1, pad2;
     int4 c_lo, c_hi;      // children in octant order, -1 = empty
-    int4 pad3, pad4;
+    float q[6];           // traceless quadrupole (xx, yy, zz, xy, xz, yz), FP32
+    float qpad0, qpad1;
 };
 static_assert(sizeof(GNode) == 128, "GNode layout");
 
@@ -74,6 +77,7 @@ struct WalkArgs {
     int* ovf_count;            // [0] queued groups, [1] global-stack overflows (error)
     int* gstack;
     int lds_scap;              // LDS stack entries used by the first launch (<= SCAP)
+    unsigned long long* inter_nodes;   // the node (monopole or quadrupole) share of inter
 };
 
 __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v) {
@@ -123,6 +127,34 @@ struct Acc<false> {
         two_sum_add(hy, ly, ay);
         two_sum_add(hz, lz, az);
     }
+    // Node entries with quadrupoles: e = (x, y, z, m) relative to the group
+    // centre, qa = (xx, yy, zz, xy), qb = (xz, yz, -, -).  With d = node -
+    // particle and r^2 = |d|^2 + eps^2, the acceleration of the potential
+    // -m/r - (1/2) d.Q.d / r^5 is  a = (m/r^3 + (5/2) d.Q.d / r^7) d - Q.d / r^5.
+    __device__ void flush_quad(const float4* E, const float4* QA, const float4* QB, int ln, int sl,
+                               int S, float eps2) {
+        float ax = 0, ay = 0, az = 0;
+        for (int j = sl; j < ln; j += S) {
+            const float4 e = E[j], qa = QA[j], qb = QB[j];
+            const float dx = e.x - px, dy = e.y - py, dz = e.z - pz;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            const float inv2 = inv * inv;
+            const float inv3 = inv * inv2;
+            const float inv5 = inv3 * inv2;
+            const float qx = fmaf(qa.x, dx, fmaf(qa.w, dy, qb.x * dz));
+            const float qy = fmaf(qa.w, dx, fmaf(qa.y, dy, qb.y * dz));
+            const float qz = fmaf(qb.x, dx, fmaf(qb.y, dy, qa.z * dz));
+            const float dqd = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
+            const float f = fmaf(2.5f * dqd, inv5 * inv2, e.w * inv3);
+            ax = fmaf(f, dx, fmaf(-inv5, qx, ax));
+            ay = fmaf(f, dy, fmaf(-inv5, qy, ay));
+            az = fmaf(f, dz, fmaf(-inv5, qz, az));
+        }
+        two_sum_add(hx, lx, ax);
+        two_sum_add(hy, ly, ay);
+        two_sum_add(hz, lz, az);
+    }
     __device__ double3 total() const {
         return make_double3(double(hx) + double(lx), double(hy) + double(ly),
                             double(hz) + double(lz));
@@ -149,16 +181,22 @@ struct Acc<true> {
     __device__ double3 total() const { return make_double3(ax, ay, az); }
 };
 
-template <bool F64, bool GST>
+// QUAD (FP32 only): accepted nodes carry their quadrupole and go to a list of
+// their own; particles keep the monopole list.
+template <bool F64, bool GST, bool QUAD>
 __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
     // group sphere and acceptance test: the operations as written (no FMA
     // contraction), so the interaction set is the one the FP64 oracle forms
 #pragma clang fp contract(off)
+    static_assert(!(F64 && QUAD), "quadrupoles are evaluated in FP32");
     using Ent = typename std::conditional<F64, double4, float4>::type;
+    constexpr int LC = QUAD ? LCAP_QUAD : LCAP;
+    constexpr int QC = QUAD ? QCAP : 1;
     __shared__ int s_stack[GST ? 1 : SCAP];
-    __shared__ Ent s_list[LCAP];
+    __shared__ Ent s_list[LC];
+    __shared__ float4 s_qe[QC], s_qa[QC], s_qb[QC];
     // the slice reduction reuses the list memory once the list is evaluated
-    static_assert(sizeof(Ent) * LCAP >= 3 * 64 * sizeof(double), "reduction space");
+    static_assert(sizeof(Ent) * LC >= 3 * 64 * sizeof(double), "reduction space");
     double(*s_red)[64] = reinterpret_cast<double(*)[64]>(s_list);
     const int lane = __lane_id();
     int* stack = GST ? a.gstack + size_t(blockIdx.x) * GSCAP : s_stack;
@@ -231,8 +269,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
             const int n_act = __popcll(__ballot(chunk_act));
             if (n_act == 0) continue;
 
-            int sn = 1, ln = 0;
-            long long total_len = 0;
+            int sn = 1, ln = 0, nq = 0;
+            long long total_len = 0, total_nodes = 0;
             bool overflow = false;
             if (lane == 0) stack[0] = 0;
             __syncthreads();
@@ -288,36 +326,56 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 const uint64_t mmask = __ballot(mono);
                 const int nmono = __popcll(mmask);
                 if (nmono > 0) {
-                    if (ln + nmono > LCAP) {
-                        __syncthreads();
-                        if (evl) A.flush(s_list, ln, sl, S, eps2r);
-                        total_len += ln;
-                        ln = 0;
-                        __syncthreads();
-                    }
-                    if (mono) {
-                        const int slot = ln + __popcll(mmask & gf_lanemask_lt());
-                        if constexpr (F64) {
-                            s_list[slot] = cm;
-                        } else {
-                            s_list[slot] = make_float4(float(cm.x - bx), float(cm.y - by),
-                                                       float(cm.z - bz), float(cm.w));
+                    total_nodes += nmono;
+                    if constexpr (QUAD) {
+                        if (nq + nmono > QC) {
+                            __syncthreads();
+                            if (evl) A.flush_quad(s_qe, s_qa, s_qb, nq, sl, S, eps2r);
+                            total_len += nq;
+                            nq = 0;
+                            __syncthreads();
                         }
+                        if (mono) {
+                            const int slot = nq + __popcll(mmask & gf_lanemask_lt());
+                            const GNode* g = a.gn + node;
+                            s_qe[slot] = make_float4(float(cm.x - bx), float(cm.y - by),
+                                                     float(cm.z - bz), float(cm.w));
+                            s_qa[slot] = make_float4(g->q[0], g->q[1], g->q[2], g->q[3]);
+                            s_qb[slot] = make_float4(g->q[4], g->q[5], 0.0f, 0.0f);
+                        }
+                        nq += nmono;
+                    } else {
+                        if (ln + nmono > LC) {
+                            __syncthreads();
+                            if (evl) A.flush(s_list, ln, sl, S, eps2r);
+                            total_len += ln;
+                            ln = 0;
+                            __syncthreads();
+                        }
+                        if (mono) {
+                            const int slot = ln + __popcll(mmask & gf_lanemask_lt());
+                            if constexpr (F64) {
+                                s_list[slot] = cm;
+                            } else {
+                                s_list[slot] = make_float4(float(cm.x - bx), float(cm.y - by),
+                                                           float(cm.z - bz), float(cm.w));
+                            }
+                        }
+                        ln += nmono;
                     }
-                    ln += nmono;
                 }
                 const int pc = mono ? 0 : nent;
                 const int qinc = gf_wave_incl_scan(pc);
                 const int ptot2 = __shfl(qinc, 63, 64);
                 for (int e0 = 0; e0 < ptot2;) {
-                    if (ln == LCAP) {
+                    if (ln == LC) {
                         __syncthreads();
                         if (evl) A.flush(s_list, ln, sl, S, eps2r);
                         total_len += ln;
                         ln = 0;
                         __syncthreads();
                     }
-                    const int m = min(min(64, LCAP - ln), ptot2 - e0);
+                    const int m = min(min(64, LC - ln), ptot2 - e0);
                     const int e = e0 + lane;
                     // owner lane of entry e: the number of lanes whose
                     // inclusive count is <= e (all lanes take part)
@@ -356,6 +414,14 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 total_len += ln;
                 ln = 0;
             }
+            if constexpr (QUAD) {
+                if (nq > 0) {
+                    __syncthreads();
+                    if (evl) A.flush_quad(s_qe, s_qa, s_qb, nq, sl, S, eps2r);
+                    total_len += nq;
+                    nq = 0;
+                }
+            }
             // reduce the slices of each particle in FP64
             double3 v = make_double3(0, 0, 0);
             if (evl) v = A.total();
@@ -377,8 +443,10 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 a.acc[o + 2] = sz;
             }
             __syncthreads();
-            if (lane == 0)
+            if (lane == 0) {
                 atomicAdd(a.inter, (unsigned long long)total_len * (unsigned long long)n_act);
+                atomicAdd(a.inter_nodes, (unsigned long long)total_nodes * (unsigned long long)n_act);
+            }
         }
     }
 }
@@ -390,6 +458,7 @@ __global__ __launch_bounds__(256) void k_gnode_pack(const int* __restrict__ coun
                                                    const int* __restrict__ count,
                                                    const int8_t* __restrict__ leaf,
                                                    const int* __restrict__ child8,
+                                                   const double* __restrict__ quad,
                                                    GNode* __restrict__ gn) {
     const int nn = counters[0];
     for (int id = blockIdx.x * 256 + threadIdx.x; id < nn; id += gridDim.x * 256) {
@@ -403,7 +472,8 @@ __global__ __launch_bounds__(256) void k_gnode_pack(const int* __restrict__ coun
         const int4* cs = reinterpret_cast<const int4*>(child8 + 8 * size_t(id));
         g.c_lo = cs[0];
         g.c_hi = cs[1];
-        g.pad3 = g.pad4 = make_int4(0, 0, 0, 0);
+        for (int k = 0; k < 6; ++k) g.q[k] = quad ? float(quad[6 * size_t(id) + k]) : 0.0f;
+        g.qpad0 = g.qpad1 = 0.0f;
         gn[id] = g;
     }
 }
@@ -494,10 +564,11 @@ int direct_splits(int n, int nt) {
 int gf_walk_reserve(GfWalkScratch& W, int max_groups) {
     if (!W.work) {
         GF_HIP(hipMalloc(&W.work, 16 * sizeof(int)));
-        GF_HIP(hipMalloc(&W.inter, sizeof(unsigned long long)));
+        GF_HIP(hipMalloc(&W.inter, 2 * sizeof(unsigned long long)));
+        W.inter_nodes = W.inter + 1;
         GF_HIP(hipMalloc(&W.ovf_count, 4 * sizeof(int)));
         GF_HIP(hipMalloc(&W.gstack, sizeof(int) * size_t(GST_BLOCKS) * GSCAP));
-        GF_HIP(hipMemset(W.inter, 0, sizeof(unsigned long long)));
+        GF_HIP(hipMemset(W.inter, 0, 2 * sizeof(unsigned long long)));
     }
     if (max_groups > W.cap_groups) {
         if (W.ovf) GF_HIP(hipFree(W.ovf));
@@ -543,7 +614,7 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
     if (gf_walk_reserve(W, T.cap_nodes) || walk_pack_reserve(W, T.cap_nodes, T.n)) return -1;
     hipLaunchKernelGGL(k_gnode_pack, dim3(std::min(2048, gf_div_up(T.cap_nodes, 256))), dim3(256), 0,
                        s, T.counters, T.com, T.open_r, T.start, T.count, T.leaf, T.child8,
-                       reinterpret_cast<GNode*>(W.gnode));
+                       T.quad, reinterpret_cast<GNode*>(W.gnode));
     hipLaunchKernelGGL(k_gp4_pack, dim3(gf_div_up(T.n, 256)), dim3(256), 0, s, T.n, T.tx, T.ty,
                        T.tz, T.tm, reinterpret_cast<double4*>(W.p4));
     WalkArgs a;
@@ -555,14 +626,18 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
     a.active = active; a.acc = acc; a.eps2 = eps * eps;
     a.work = W.work; a.inter = W.inter; a.ovf = W.ovf; a.ovf_count = W.ovf_count;
     a.gstack = W.gstack;
+    a.inter_nodes = W.inter_nodes;
     a.lds_scap = gf_test_limits().walk_stack > 0 ? std::min(SCAP, gf_test_limits().walk_stack) : SCAP;
     hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, W.work, W.ovf_count);
     if (precision == GF_FP64) {
-        hipLaunchKernelGGL((k_walk<true, false>), dim3(WALK_BLOCKS), dim3(64), 0, s, a);
-        hipLaunchKernelGGL((k_walk<true, true>), dim3(GST_BLOCKS), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_walk<true, false, false>), dim3(WALK_BLOCKS), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_walk<true, true, false>), dim3(GST_BLOCKS), dim3(64), 0, s, a);
+    } else if (precision == GF_FP32_QUAD) {
+        hipLaunchKernelGGL((k_walk<false, false, true>), dim3(WALK_BLOCKS), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_walk<false, true, true>), dim3(GST_BLOCKS), dim3(64), 0, s, a);
     } else {
-        hipLaunchKernelGGL((k_walk<false, false>), dim3(WALK_BLOCKS), dim3(64), 0, s, a);
-        hipLaunchKernelGGL((k_walk<false, true>), dim3(GST_BLOCKS), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_walk<false, false, false>), dim3(WALK_BLOCKS), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_walk<false, true, false>), dim3(GST_BLOCKS), dim3(64), 0, s, a);
     }
     GF_LAUNCH_CHECK();
     return 0;

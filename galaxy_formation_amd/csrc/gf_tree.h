@@ -31,6 +31,8 @@ struct GfTree {
     double *open_r = nullptr;                      // opening radius (FP64)
     double4 *box = nullptr;                        // geometric centre and half-width (FP64)
     double *ndisp = nullptr;                       // largest displacement below the node
+    double *quad = nullptr;                        // traceless quadrupole about the centre of
+                                                   // mass (xx, yy, zz, xy, xz, yz), FP64
     int *gflag = nullptr;                          // nodes + 1, then group offsets
     int *groups = nullptr;                         // group nodes, DFS order
 

@@ -296,18 +296,32 @@ __global__ __launch_bounds__(TB) void k_gather(const double* __restrict__ pos,
 
 // nbody.cpp:98-115 (build) and :297-303 (refit) in FP64: open_r =
 // 2 (half + dm) / theta + |com - centre| (dm = 0 right after a build).
-__device__ __forceinline__ void finalize_node(int id, double M, double sx, double sy, double sz,
-                                              double dm, double theta_d,
-                                              const double4* __restrict__ box, double4* com,
-                                              double* open_r, double* ndisp) {
+__device__ __forceinline__ double4 finalize_node(int id, double M, double sx, double sy,
+                                                 double sz, double dm, double theta_d,
+                                                 const double4* __restrict__ box, double4* com,
+                                                 double* open_r, double* ndisp) {
 #pragma clang fp contract(off)
     const double inv = M > 0 ? 1.0 / M : 0.0;
     const double cx = sx * inv, cy = sy * inv, cz = sz * inv;
     const double4 bx = box[id];
     const double dx = cx - bx.x, dy = cy - bx.y, dz = cz - bx.z;
-    com[id] = make_double4(cx, cy, cz, M);
+    const double4 c = make_double4(cx, cy, cz, M);
+    com[id] = c;
     open_r[id] = 2 * (bx.w + dm) / theta_d + sqrt(dx * dx + dy * dy + dz * dz);
     ndisp[id] = dm;
+    return c;
+}
+
+// Adds m (3 d d^T - |d|^2 I) to the traceless quadrupole q (xx, yy, zz, xy, xz, yz).
+__device__ __forceinline__ void quad_add(double* q, double m, double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    q[0] += m * (3 * dx * dx - r2);
+    q[1] += m * (3 * dy * dy - r2);
+    q[2] += m * (3 * dz * dz - r2);
+    q[3] += m * (3 * dx * dy);
+    q[4] += m * (3 * dx * dz);
+    q[5] += m * (3 * dy * dz);
 }
 
 // Nodes listed by depth (deepest level first is the bottom-up order).
@@ -376,7 +390,8 @@ __global__ __launch_bounds__(TB) void k_level_com(
     const int* __restrict__ child8, const double4* __restrict__ box,
     const double* __restrict__ tx, const double* __restrict__ ty, const double* __restrict__ tz,
     const double* __restrict__ tm, const double* __restrict__ tdisp, double4* __restrict__ com,
-    double* __restrict__ open_r, double* __restrict__ ndisp, double theta_d) {
+    double* __restrict__ open_r, double* __restrict__ ndisp, double theta_d,
+    double* __restrict__ quad) {
 #pragma clang fp contract(off)
     const int b0 = doff[d], b1 = doff[d + 1];
     for (int k = b0 + blockIdx.x * TB + threadIdx.x; k < b1; k += gridDim.x * TB) {
@@ -405,7 +420,26 @@ __global__ __launch_bounds__(TB) void k_level_com(
                 dm = fmax(dm, ndisp[c]);
             }
         }
-        finalize_node(id, M, sx, sy, sz, dm, theta_d, box, com, open_r, ndisp);
+        const double4 c = finalize_node(id, M, sx, sy, sz, dm, theta_d, box, com, open_r, ndisp);
+        if (quad) {
+            // quadrupole about the centre of mass: from the particles of a
+            // leaf, or the children's tensors moved by the parallel-axis rule
+            double q[6] = {0, 0, 0, 0, 0, 0};
+            if (leaf[id]) {
+                const int b = start[id], e = b + count[id];
+                for (int i = b; i < e; ++i) quad_add(q, tm[i], tx[i] - c.x, ty[i] - c.y, tz[i] - c.z);
+            } else {
+                const int* slots = child8 + 8 * size_t(id);
+                for (int o = 0; o < 8; ++o) {
+                    const int ch = slots[o];
+                    if (ch < 0) continue;
+                    const double4 cm = com[ch];
+                    for (int k = 0; k < 6; ++k) q[k] += quad[6 * size_t(ch) + k];
+                    quad_add(q, cm.w, cm.x - c.x, cm.y - c.y, cm.z - c.z);
+                }
+            }
+            for (int k = 0; k < 6; ++k) quad[6 * size_t(id) + k] = q[k];
+        }
     }
 }
 
@@ -488,6 +522,7 @@ int gf_tree_reserve(GfTree& T, int n) {
             dalloc(T.parent, nb) || dalloc(T.leaf, nb) || dalloc(T.depth, nb) ||
             dalloc(T.child8, 8 * size_t(nb)) || dalloc(T.dlist, nb) || dalloc(T.com, nb) ||
             dalloc(T.open_r, nb) || dalloc(T.box, nb) || dalloc(T.ndisp, nb) ||
+            dalloc(T.quad, 6 * size_t(nb)) ||
             dalloc(T.gflag, size_t(nb) + 1) || dalloc(T.groups, nb))
             return -1;
         T.cap_nodes = nb;
@@ -501,7 +536,7 @@ void gf_tree_free(GfTree& T) {
     dfree(T.chain); dfree(T.tx); dfree(T.ty); dfree(T.tz); dfree(T.tm); dfree(T.tdisp);
     dfree(T.xb); dfree(T.start); dfree(T.count); dfree(T.next); dfree(T.parent); dfree(T.leaf);
     dfree(T.depth); dfree(T.child8); dfree(T.dlist); dfree(T.dmeta); dfree(T.com); dfree(T.open_r);
-    dfree(T.box); dfree(T.ndisp); dfree(T.gflag); dfree(T.groups); dfree(T.counters);
+    dfree(T.box); dfree(T.ndisp); dfree(T.quad); dfree(T.gflag); dfree(T.groups); dfree(T.counters);
     dfree(T.root); dfree(T.bbox_part); dfree(T.scan_tmp); dfree(T.sort_tmp);
     T = GfTree();
 }
@@ -515,7 +550,7 @@ static int com_pass(GfTree& T, hipStream_t s) {
         const int g = d >= 6 ? 1024 : std::max(1, 1 << std::max(0, 3 * d - 8));
         hipLaunchKernelGGL(k_level_com, dim3(g), dim3(TB), 0, s, d, doff, T.dlist, T.start,
                            T.count, T.leaf, T.child8, T.box, T.tx, T.ty, T.tz, T.tm, T.tdisp,
-                           T.com, T.open_r, T.ndisp, double(T.theta));
+                           T.com, T.open_r, T.ndisp, double(T.theta), T.quad);
     }
     GF_LAUNCH_CHECK();
     return 0;

@@ -8,7 +8,7 @@ import numpy as np
 
 from . import _lib
 
-PRECISIONS = {"fp32": 0, "fp64": 1}
+PRECISIONS = {"fp32": 0, "fp64": 1, "fp32q": 2}
 
 
 class Engine:

@@ -62,7 +62,10 @@ int gf_version(void);
 int gf_device_info(char* buf, int len);
 /* Interaction precision of nbody_accel_active: 0 = FP32 pair arithmetic on
  * group-relative coordinates with compensated accumulation (default), 1 =
- * FP64.  Tree, centres of mass and acceptance tests are FP64 in both. */
+ * FP64, 2 = as 0 with each accepted node's traceless quadrupole moment added
+ * to its monopole (no counterpart in the reference, which evaluates
+ * monopoles: cpp/nbody.cpp:386-408).  Tree, centres of mass, quadrupoles and
+ * acceptance tests are FP64 in all three; the interaction set is the same. */
 int gf_set_precision(int precision);
 int gf_get_precision(void);
 /* Testing hook, no counterpart in the reference: lowers the capacity of
@@ -104,6 +107,8 @@ int gf_direct_dev(int n, const double* pos, const double* mass, int nt, const in
                   double* acc, double eps, void* stream);
 /* Interaction count of the last gravity call (synchronises). */
 int gf_interactions_dev(long long* out);
+/* The part of that count that are node (not particle) entries. */
+int gf_node_interactions_dev(long long* out);
 /* Waits for the stream and reports kernel-side errors. */
 int gf_sync(void* stream);
 

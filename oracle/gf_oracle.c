@@ -22,8 +22,13 @@
  *   mode 0 ("ref"):  the reference arithmetic: FP32 node payload, FP32 MAC and
  *                    FP32 list evaluation in absolute coordinates.
  *   mode 1 ("fp64"): FP64 centres of mass, FP64 MAC and FP64 evaluation.  This
- *                    is the interaction set the GPU walk uses for both of its
+ *                    is the interaction set the GPU walk uses for all of its
  *                    evaluation precisions.
+ *   mode 2 ("quad"): mode 1 with each accepted node's traceless quadrupole
+ *                    moment Q = sum m (3 d d^T - |d|^2 I) about its centre of
+ *                    mass (summed here directly over the node's particles)
+ *                    added to its monopole.  No counterpart in the reference;
+ *                    pinned by closed forms (tests/test_oracle.py).
  *
  * Pinning (tests/): mode 0 against the reference library built from source
  * (oracle/_ref) and its golden fixtures; both modes against the FP64 direct sum
@@ -49,6 +54,7 @@ typedef struct {
     /* FP64 geometric box and FP64 mode payload */
     double bcx, bcy, bcz, bhalf;
     double dcx, dcy, dcz, dm, dopen_r;
+    double dq[6];   /* mode 2: quadrupole (xx, yy, zz, xy, xz, yz) */
 } ONode;
 
 typedef struct {
@@ -290,6 +296,7 @@ static void tree_refit(int n, const double *pos, const double *mass) {
 typedef struct {
     float *x, *y, *z, *m;
     double *dx, *dy, *dz, *dm;
+    double *q;      /* 6 per entry (mode 2; zero for particles) */
     int n, cap;
 } OList;
 
@@ -302,10 +309,34 @@ static void list_push(OList *L, double x, double y, double z, double m, float fx
         L->z = (float *)xrealloc(L->z, fs); L->m = (float *)xrealloc(L->m, fs);
         L->dx = (double *)xrealloc(L->dx, ds); L->dy = (double *)xrealloc(L->dy, ds);
         L->dz = (double *)xrealloc(L->dz, ds); L->dm = (double *)xrealloc(L->dm, ds);
+        L->q = (double *)xrealloc(L->q, 6 * ds);
     }
     L->x[L->n] = fx; L->y[L->n] = fy; L->z[L->n] = fz; L->m[L->n] = fm;
     L->dx[L->n] = x; L->dy[L->n] = y; L->dz[L->n] = z; L->dm[L->n] = m;
+    for (int k = 0; k < 6; ++k) L->q[6 * L->n + k] = 0.0;
     L->n++;
+}
+
+/* Mode 2: the quadrupole of every node, directly over its particles (tree
+ * order) about its FP64 centre of mass. */
+static void compute_quads(void) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int k = 0; k < T.nn; ++k) {
+        ONode *q = &T.nodes[k];
+        double Q[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = q->start; i < q->start + q->count; ++i) {
+            double m = T.dpm[i];
+            double dx = T.dpx[i] - q->dcx, dy = T.dpy[i] - q->dcy, dz = T.dpz[i] - q->dcz;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            Q[0] += m * (3 * dx * dx - r2);
+            Q[1] += m * (3 * dy * dy - r2);
+            Q[2] += m * (3 * dz * dz - r2);
+            Q[3] += m * (3 * dx * dy);
+            Q[4] += m * (3 * dx * dz);
+            Q[5] += m * (3 * dy * dz);
+        }
+        for (int c = 0; c < 6; ++c) q->dq[c] = Q[c];
+    }
 }
 
 /* nbody.cpp:323-411 (compute_accel): group walk + list evaluation. */
@@ -401,6 +432,8 @@ static long long walk_eval(double *acc, double eps, const uint8_t *active, int m
                     double t = q->dopen_r + br;
                     if (dx * dx + dy * dy + dz * dz > t * t) {
                         list_push(&L, q->dcx, q->dcy, q->dcz, q->dm, 0, 0, 0, 0);
+                        if (mode == 2)
+                            for (int c = 0; c < 6; ++c) L.q[6 * (L.n - 1) + c] = q->dq[c];
                         i = q->next;
                     } else if (q->leaf) {
                         for (int j = q->start; j < q->start + q->count; ++j)
@@ -419,6 +452,17 @@ static long long walk_eval(double *acc, double eps, const uint8_t *active, int m
                         double r2 = ex * ex + ey * ey + ez * ez + deps2;
                         double inv = 1.0 / sqrt(r2);
                         double f = L.dm[j] * inv * inv * inv;
+                        if (mode == 2) {
+                            /* potential -m/r - (1/2) d.Q.d / r^5 of the node at d */
+                            const double *Q = L.q + 6 * j;
+                            double qx = Q[0] * ex + Q[3] * ey + Q[4] * ez;
+                            double qy = Q[3] * ex + Q[1] * ey + Q[5] * ez;
+                            double qz = Q[4] * ex + Q[5] * ey + Q[2] * ez;
+                            double dqd = ex * qx + ey * qy + ez * qz;
+                            double inv2 = inv * inv, inv5 = inv * inv2 * inv2;
+                            f += 2.5 * dqd * inv5 * inv2;
+                            sx -= inv5 * qx; sy -= inv5 * qy; sz -= inv5 * qz;
+                        }
                         sx += f * ex; sy += f * ey; sz += f * ez;
                     }
                     const int k = T.idx[p];
@@ -428,7 +472,7 @@ static long long walk_eval(double *acc, double eps, const uint8_t *active, int m
             }
         }
         free(L.x); free(L.y); free(L.z); free(L.m);
-        free(L.dx); free(L.dy); free(L.dz); free(L.dm);
+        free(L.dx); free(L.dy); free(L.dz); free(L.dm); free(L.q);
     }
     return inter;
 }
@@ -445,7 +489,8 @@ int orc_num_threads(void) {
 
 /* nbody.cpp:422-425 semantics: only active rows of acc are written; rebuild=0
  * refits the tree kept from the previous call (a change of n forces a build).
- * mode 0 = reference FP32 arithmetic, 1 = FP64.  Returns the interaction count
+ * mode 0 = reference FP32 arithmetic, 1 = FP64, 2 = FP64 with node
+ * quadrupoles.  Returns the interaction count
  * (sum over evaluated particles of their list length). */
 long long orc_accel_active(int n, const double *pos, const double *mass, double *acc, double theta,
                            double eps, const uint8_t *active, int rebuild, int mode) {
@@ -453,6 +498,7 @@ long long orc_accel_active(int n, const double *pos, const double *mass, double 
         tree_build(n, pos, mass, (float)theta);
     else
         tree_refit(n, pos, mass);
+    if (mode == 2) compute_quads();
     return walk_eval(acc, eps, active, mode);
 }
 

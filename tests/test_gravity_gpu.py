@@ -250,3 +250,27 @@ def test_global_stack_walk_equals_lds_walk(eng, limit):
             assert eng.last_interactions() == ref[p][1]
     finally:
         _lib.check(L.gf_set_test_limits(0, 0))
+
+
+@pytest.mark.parametrize("theta", [0.4, 0.5, 0.7])
+def test_fp32_quadrupole_close_to_oracle(eng, theta):
+    """Mode 2 (FP32 evaluation with node quadrupoles) against the oracle's
+    FP64 mode 2: the same interaction set, FP32 rounding apart."""
+    c = ics.config("cloud10k")
+    acc = run(eng, c.pos, c.mass, theta, c.eps, "fp32q")
+    inter = eng.last_interactions()
+    ref, oi = O.accel(c.pos, c.mass, theta, c.eps, mode=2)
+    assert inter == oi
+    e = relerr(acc, ref)
+    assert np.sqrt(np.mean(e ** 2)) < 2e-6 and e.max() < 5e-5
+
+
+def test_quadrupoles_reduce_the_error_vs_direct_sum(eng):
+    """The node quadrupoles cut the Barnes-Hut truncation error at the same
+    opening angle (and the same interaction count)."""
+    c = ics.config("cloud10k")
+    exact = eng.gravity_direct(c.pos, c.mass, c.eps)
+    for theta in (0.5, 0.7):
+        e1 = relerr(run(eng, c.pos, c.mass, theta, c.eps, "fp32"), exact)
+        e2 = relerr(run(eng, c.pos, c.mass, theta, c.eps, "fp32q"), exact)
+        assert np.sqrt(np.mean(e2 ** 2)) < 0.4 * np.sqrt(np.mean(e1 ** 2))

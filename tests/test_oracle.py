@@ -359,3 +359,47 @@ def test_divv_of_linear_flows():
     assert np.abs(divv).max() < 1e-13
     np.testing.assert_allclose(dt, dt0, rtol=1e-12)
     np.testing.assert_allclose(acc, acc0, rtol=1e-9, atol=1e-12 * np.abs(acc0).max())
+
+
+def test_quadrupole_far_field():
+    """Mode 2 (node quadrupoles): eight equal masses at the corners of a small
+    box (a leaf accepted as one node) seen from a distant group.  Against the
+    direct sum, the monopole walk's error is the cluster's quadrupole field;
+    the quadrupole walk removes it (the remainder of a box is of order
+    (a/D)^4, the first term the expansion drops)."""
+    rng = np.random.default_rng(3)
+    a, b, c, L, R = 0.03, 0.02, 0.01, 1.0, 1.0
+    corners = np.array([[sx * a, sy * b, sz * c] for sx in (-1, 1) for sy in (-1, 1)
+                        for sz in (-1, 1)]) + L
+    target = np.array([[-R, -R, -R]])
+    fill = target + rng.normal(scale=1e-3, size=(60, 3))
+    pos = np.ascontiguousarray(np.concatenate([corners, target, fill]))
+    mass = np.concatenate([np.full(8, 0.5), [1e-6], np.full(60, 1e-6)])
+    eps = 1e-4
+    d = O.direct(pos, mass, eps)
+    a1, i1 = O.accel(pos, mass, 0.9, eps, mode=1)
+    a2, i2 = O.accel(pos, mass, 0.9, eps, mode=2)
+    assert i1 == i2
+    e1 = np.linalg.norm(a1[8:] - d[8:], axis=1)
+    e2 = np.linalg.norm(a2[8:] - d[8:], axis=1)
+    g = 4.0 / np.sum((target[0] - L) ** 2)            # the cluster's monopole field
+    assert e1.min() > 1e-5 * g            # quadrupole part, (a/D)^2 ~ 1e-4: the node was accepted
+    assert (e2 < 0.03 * e1).all()
+    # the sign of the correction: the cluster is elongated along x, so its
+    # quadrupole field differs from a point mass's; mode 2 moves towards d
+    assert (np.linalg.norm(a2[8:] - a1[8:], axis=1) > 0.9 * e1).all()
+
+
+def test_quadrupole_mode_on_a_cloud():
+    """On the rotating cloud, mode 2 has the same interaction set as mode 1
+    and is several times closer to the direct sum; theta -> 0 is the direct
+    sum in every mode."""
+    c = ics.make_cloud(3000, 3000, seed=8)
+    d = O.direct(c.pos, c.mass, c.eps)
+    a1, i1 = O.accel(c.pos, c.mass, 0.5, c.eps, mode=1)
+    a2, i2 = O.accel(c.pos, c.mass, 0.5, c.eps, mode=2)
+    assert i1 == i2
+    rms = lambda x: np.sqrt(np.mean(relerr(x, d) ** 2))  # noqa: E731
+    assert rms(a2) < 0.3 * rms(a1)
+    a0, _ = O.accel(c.pos, c.mass, 1e-6, c.eps, mode=2)
+    assert relerr(a0, d).max() < 1e-12
