@@ -612,9 +612,11 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
         return -1;
     }
     if (gf_walk_reserve(W, T.cap_nodes) || walk_pack_reserve(W, T.cap_nodes, T.n)) return -1;
+    if (precision == GF_FP32_QUAD && gf_tree_quadrupoles(T, s)) return -1;
     hipLaunchKernelGGL(k_gnode_pack, dim3(std::min(2048, gf_div_up(T.cap_nodes, 256))), dim3(256), 0,
                        s, T.counters, T.com, T.open_r, T.start, T.count, T.leaf, T.child8,
-                       T.quad, reinterpret_cast<GNode*>(W.gnode));
+                       precision == GF_FP32_QUAD ? T.quad : nullptr,
+                       reinterpret_cast<GNode*>(W.gnode));
     hipLaunchKernelGGL(k_gp4_pack, dim3(gf_div_up(T.n, 256)), dim3(256), 0, s, T.n, T.tx, T.ty,
                        T.tz, T.tm, reinterpret_cast<double4*>(W.p4));
     WalkArgs a;

@@ -33,6 +33,7 @@ struct GfTree {
     double *ndisp = nullptr;                       // largest displacement below the node
     double *quad = nullptr;                        // traceless quadrupole about the centre of
                                                    // mass (xx, yy, zz, xy, xz, yz), FP64
+    mutable int quad_ok = 0;                       // quad matches the held centres of mass
     int *gflag = nullptr;                          // nodes + 1, then group offsets
     int *groups = nullptr;                         // group nodes, DFS order
 
@@ -55,6 +56,10 @@ void gf_tree_free(GfTree& T);
 // with_com = false skips the centres of mass (a tree used for neighbour search).
 int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float theta,
                   hipStream_t s, bool with_com = true);
+// Quadrupole moments of the held tree (computed once per build or refit, when
+// a walk asks for them).
+int gf_tree_quadrupoles(const GfTree& T, hipStream_t s);
+
 // Refit (cpp/nbody.cpp:262-305): same cells, new masses, centres of mass and
 // opening radii enlarged by the largest displacement since the build.
 int gf_tree_refit(GfTree& T, const double* pos, const double* mass, hipStream_t s);

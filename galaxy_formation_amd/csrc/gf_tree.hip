@@ -390,8 +390,7 @@ __global__ __launch_bounds__(TB) void k_level_com(
     const int* __restrict__ child8, const double4* __restrict__ box,
     const double* __restrict__ tx, const double* __restrict__ ty, const double* __restrict__ tz,
     const double* __restrict__ tm, const double* __restrict__ tdisp, double4* __restrict__ com,
-    double* __restrict__ open_r, double* __restrict__ ndisp, double theta_d,
-    double* __restrict__ quad) {
+    double* __restrict__ open_r, double* __restrict__ ndisp, double theta_d) {
 #pragma clang fp contract(off)
     const int b0 = doff[d], b1 = doff[d + 1];
     for (int k = b0 + blockIdx.x * TB + threadIdx.x; k < b1; k += gridDim.x * TB) {
@@ -420,26 +419,39 @@ __global__ __launch_bounds__(TB) void k_level_com(
                 dm = fmax(dm, ndisp[c]);
             }
         }
-        const double4 c = finalize_node(id, M, sx, sy, sz, dm, theta_d, box, com, open_r, ndisp);
-        if (quad) {
-            // quadrupole about the centre of mass: from the particles of a
-            // leaf, or the children's tensors moved by the parallel-axis rule
-            double q[6] = {0, 0, 0, 0, 0, 0};
-            if (leaf[id]) {
-                const int b = start[id], e = b + count[id];
-                for (int i = b; i < e; ++i) quad_add(q, tm[i], tx[i] - c.x, ty[i] - c.y, tz[i] - c.z);
-            } else {
-                const int* slots = child8 + 8 * size_t(id);
-                for (int o = 0; o < 8; ++o) {
-                    const int ch = slots[o];
-                    if (ch < 0) continue;
-                    const double4 cm = com[ch];
-                    for (int k = 0; k < 6; ++k) q[k] += quad[6 * size_t(ch) + k];
-                    quad_add(q, cm.w, cm.x - c.x, cm.y - c.y, cm.z - c.z);
-                }
+        finalize_node(id, M, sx, sy, sz, dm, theta_d, box, com, open_r, ndisp);
+    }
+}
+
+// Traceless quadrupoles about the centres of mass, bottom-up by depth like
+// the centres of mass: from the particles of a leaf, or the children's
+// tensors moved by the parallel-axis rule (FP64).
+__global__ __launch_bounds__(TB) void k_level_quad(
+    int d, const int* __restrict__ doff, const int* __restrict__ dlist,
+    const int* __restrict__ start, const int* __restrict__ count, const int8_t* __restrict__ leaf,
+    const int* __restrict__ child8, const double* __restrict__ tx, const double* __restrict__ ty,
+    const double* __restrict__ tz, const double* __restrict__ tm, const double4* __restrict__ com,
+    double* __restrict__ quad) {
+#pragma clang fp contract(off)
+    const int b0 = doff[d], b1 = doff[d + 1];
+    for (int k = b0 + blockIdx.x * TB + threadIdx.x; k < b1; k += gridDim.x * TB) {
+        const int id = dlist[k];
+        const double4 c = com[id];
+        double q[6] = {0, 0, 0, 0, 0, 0};
+        if (leaf[id]) {
+            const int b = start[id], e = b + count[id];
+            for (int i = b; i < e; ++i) quad_add(q, tm[i], tx[i] - c.x, ty[i] - c.y, tz[i] - c.z);
+        } else {
+            const int* slots = child8 + 8 * size_t(id);
+            for (int o = 0; o < 8; ++o) {
+                const int ch = slots[o];
+                if (ch < 0) continue;
+                const double4 cm = com[ch];
+                for (int j = 0; j < 6; ++j) q[j] += quad[6 * size_t(ch) + j];
+                quad_add(q, cm.w, cm.x - c.x, cm.y - c.y, cm.z - c.z);
             }
-            for (int k = 0; k < 6; ++k) quad[6 * size_t(id) + k] = q[k];
         }
+        for (int j = 0; j < 6; ++j) quad[6 * size_t(id) + j] = q[j];
     }
 }
 
@@ -550,9 +562,23 @@ static int com_pass(GfTree& T, hipStream_t s) {
         const int g = d >= 6 ? 1024 : std::max(1, 1 << std::max(0, 3 * d - 8));
         hipLaunchKernelGGL(k_level_com, dim3(g), dim3(TB), 0, s, d, doff, T.dlist, T.start,
                            T.count, T.leaf, T.child8, T.box, T.tx, T.ty, T.tz, T.tm, T.tdisp,
-                           T.com, T.open_r, T.ndisp, double(T.theta), T.quad);
+                           T.com, T.open_r, T.ndisp, double(T.theta));
     }
     GF_LAUNCH_CHECK();
+    T.quad_ok = 0;
+    return 0;
+}
+
+int gf_tree_quadrupoles(const GfTree& T, hipStream_t s) {
+    if (T.quad_ok || T.n == 0) return 0;
+    const int* doff = T.dmeta + 32;
+    for (int d = GF_KEY_LEVELS; d >= 0; --d) {
+        const int g = d >= 6 ? 1024 : std::max(1, 1 << std::max(0, 3 * d - 8));
+        hipLaunchKernelGGL(k_level_quad, dim3(g), dim3(TB), 0, s, d, doff, T.dlist, T.start,
+                           T.count, T.leaf, T.child8, T.tx, T.ty, T.tz, T.tm, T.com, T.quad);
+    }
+    GF_LAUNCH_CHECK();
+    T.quad_ok = 1;
     return 0;
 }
 
@@ -562,6 +588,7 @@ int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float
     T.n = n;
     T.theta = theta;
     T.built = 0;
+    T.quad_ok = 0;
     if (n == 0) return 0;
     const int g = gf_div_up(n, TB);
     const int nbb = std::min(1024, g);
