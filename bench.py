@@ -58,6 +58,9 @@ def parse():
     p.add_argument("--theta", type=float, default=0.5)
     p.add_argument("--k", type=int, default=64)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--steploop", action="store_true",
+                   help="the hierarchical-leapfrog step loop instead of the force evaluation")
+    p.add_argument("--substeps", type=int, default=224)
     p.add_argument("--model", default=os.path.join(ROOT, "galaxy_formation_amd", "data",
                                                   "learned_h_k64.npz"))
     return p.parse_args()
@@ -204,12 +207,73 @@ class Bench:
         return res
 
 
+def steploop(a):
+    """The step loop: a hierarchical leapfrog with 6 rungs over `substeps`
+    sub-steps (a multiple of 32: whole long steps), FP32 + learned first
+    radius against plain FP64, with the energy and angular-momentum drift
+    (stepper.RungStepper).  Prints one JSON line."""
+    import torch
+
+    from galaxy_formation_amd import ics
+    from galaxy_formation_amd.learned_h import LearnedH
+    from galaxy_formation_amd.stepper import RungStepper
+    assert a.substeps % 32 == 0, "whole long steps of 32 sub-steps"
+    c = ics.config(a.workload)
+    model = LearnedH().load(a.model)
+    out = {"metric": "hierarchical leapfrog sub-steps/s", "unit": "sub-steps/s",
+           "workload": f"{a.workload}: {c.n} particles ({int((c.kind == 0).sum())} gas), 6 rungs",
+           "substeps": a.substeps, "theta": a.theta, "k": a.k}
+    for name, prec, first in (("fast_fp32", "fp32", "learned"), ("plain_fp64", "fp64", "plain")):
+        log = lambda m: print(f"[steploop {a.workload} {name}] {m}", file=sys.stderr,  # noqa: E731
+                              flush=True)
+        log("start: forces of all particles, energy")
+        S = RungStepper(c, theta=a.theta, k=a.k, precision=prec, first=first, model=model)
+        S.start()
+        torch.cuda.synchronize()
+        log(f"timed: {a.substeps} sub-steps")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        w0 = time.perf_counter()
+        e0.record()
+        for _ in range(a.substeps):
+            S.substep()
+        e1.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        log(f"done in {wall:.2f} s; final energy")
+        E1 = S.synchronised_energy()
+        E0 = S.E0
+        tot0, tot1 = sum(E0[:3]), sum(E1[:3])
+        af = np.array(S.active_fraction)
+        out[name] = {
+            "ms_per_substep": e0.elapsed_time(e1) / a.substeps,
+            "wall_ms_per_substep": 1e3 * wall / a.substeps,
+            "dt_min_last_long_step": S.dt_min,
+            "active_fraction": {"mean": float(af.mean()), "min": float(af.min()),
+                                "max": float(af.max())},
+            "gravity_evals_per_s": float(af.sum() * c.n / (e0.elapsed_time(e1) * 1e-3)),
+            "knn_pass_launches_mean": float(np.mean(S.passes)),
+            "energy_start": {"kinetic": E0[0], "thermal": E0[1], "potential": E0[2]},
+            "energy_end": {"kinetic": E1[0], "thermal": E1[1], "potential": E1[2]},
+            "energy_drift": (tot1 - tot0) / abs(tot0),
+            "angular_momentum_drift": (E1[3] - E0[3]) / E0[3],
+            "angular_momentum_drift_max_over_long_steps":
+                float(max(abs(x - E0[3]) for x in S.L_sync) / E0[3]) if S.L_sync else None,
+        }
+        del S
+        torch.cuda.empty_cache()
+    out["speedup_fast_vs_plain"] = out["plain_fp64"]["ms_per_substep"] / out["fast_fp32"]["ms_per_substep"]
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the multi-GPU "
               "decomposition is described, not built", file=sys.stderr)
         sys.exit(2)
+    if a.steploop:
+        steploop(a)
+        return
     B = Bench(a)
     torch, n, ng, c = B.torch, B.n, B.ng, B.c
     # a plain copy of known size before the timed region: the reference point

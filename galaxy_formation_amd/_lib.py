@@ -41,6 +41,7 @@ SIGNATURES = {
     "gf_tree_build_dev": (_ci, [_ci, _vp, _vp, _cd, _vp]),
     "gf_walk_dev": (_ci, [_ci, _vp, _vp, _cd, _ci, _vp]),
     "gf_direct_dev": (_ci, [_ci, _vp, _vp, _ci, _vp, _vp, _cd, _vp]),
+    "gf_potential_dev": (_ci, [_ci, _vp, _vp, _vp, _cd, _vp]),
     "gf_interactions_dev": (_ci, [_vp]),
     "gf_node_interactions_dev": (_ci, [_vp]),
     "gf_sync": (_ci, [_vp]),

@@ -284,6 +284,17 @@ int gf_direct_dev(int n, const double* pos, const double* mass, int nt, const in
     return gf_direct(n, pos, mass, nt, tgt, acc, eps, g_ctx.d_partial, gf_stream(stream));
 }
 
+int gf_potential_dev(int n, const double* pos, const double* mass, double* phi, double eps,
+                     void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (n < 0 || (n > 0 && (!pos || !mass || !phi))) {
+        gf_set_error("gf_potential_dev: bad arguments");
+        return -1;
+    }
+    if (STAGE(g_ctx.d_partial, sizeof(double) * gf_potential_partial_doubles(n))) return -1;
+    return gf_potential(n, pos, mass, phi, eps, g_ctx.d_partial, gf_stream(stream));
+}
+
 int gf_interactions_dev(long long* out) {
     if (gf_ctx_init()) return -1;
     unsigned long long v = 0;

@@ -34,3 +34,10 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
 size_t gf_direct_partial_doubles(int n, int nt);
 int gf_direct(int n, const double* pos, const double* mass, int nt, const int64_t* tgt,
               double* acc, double eps, double* partial, hipStream_t s);
+
+// Softened FP64 potential of all n particles by direct summation (the
+// energy of a run): phi_i = -sum_{j != i} m_j / sqrt(r_ij^2 + eps^2).
+// partial: gf_potential_partial_doubles(n) doubles of scratch.
+size_t gf_potential_partial_doubles(int n);
+int gf_potential(int n, const double* pos, const double* mass, double* phi, double eps,
+                 double* partial, hipStream_t s);

@@ -553,6 +553,56 @@ __global__ __launch_bounds__(DT) void k_direct_reduce(int nt, int nsplit,
     acc[3 * size_t(t) + 2] = az;
 }
 
+// Softened potential phi_i = -sum_{j != i} m_j / sqrt(r_ij^2 + eps^2), FP64,
+// for the targets t0 .. t0 + nt - 1 of the n particles (the energy of a run;
+// no counterpart in the reference).  partial[split][t - t0].
+__global__ __launch_bounds__(DT) void k_potential(int n, int t0, int nt,
+                                                  const double* __restrict__ pos,
+                                                  const double* __restrict__ mass, double eps2,
+                                                  int per_split, double* __restrict__ partial) {
+    __shared__ double4 src[DT];
+    const int tt = blockIdx.x * DT + threadIdx.x;
+    const int t = t0 + tt;
+    const int split = blockIdx.y;
+    const bool ok = tt < nt;
+    const int ti = ok ? t : 0;
+    const double x = pos[3 * size_t(ti)], y = pos[3 * size_t(ti) + 1], z = pos[3 * size_t(ti) + 2];
+    double phi = 0;
+    const int j0 = split * per_split, j1 = min(n, j0 + per_split);
+    for (int jb = j0; jb < j1; jb += DT) {
+        const int j = jb + threadIdx.x;
+        __syncthreads();
+        src[threadIdx.x] = j < j1 ? make_double4(pos[3 * size_t(j)], pos[3 * size_t(j) + 1],
+                                                 pos[3 * size_t(j) + 2], mass[j])
+                                  : make_double4(0, 0, 0, 0);
+        __syncthreads();
+        const int m = min(DT, j1 - jb);
+        for (int k = 0; k < m; ++k) {
+            const double4 q = src[k];
+            const double dx = q.x - x, dy = q.y - y, dz = q.z - z;
+            const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+            double inv = __builtin_amdgcn_rsq(r2);
+            inv = inv * fma(-0.5 * r2, inv * inv, 1.5);
+            phi -= (jb + k != t) ? q.w * inv : 0.0;
+        }
+    }
+    if (ok) partial[size_t(split) * nt + tt] = phi;
+}
+
+__global__ __launch_bounds__(DT) void k_potential_reduce(int t0, int nt, int nsplit,
+                                                         const double* __restrict__ partial,
+                                                         double* __restrict__ phi) {
+    const int tt = blockIdx.x * DT + threadIdx.x;
+    if (tt >= nt) return;
+    double s = 0;
+    for (int k = 0; k < nsplit; ++k) s += partial[size_t(k) * nt + tt];
+    phi[t0 + tt] = s;
+}
+
+// Targets per potential launch: 2^18 targets of 4M sources is ~1e12 pair
+// terms, well under a second, so no launch holds the device for long.
+constexpr int POT_CHUNK = 1 << 18;
+
 int direct_splits(int n, int nt) {
     const int tb = gf_div_up(nt, DT);
     const int want = std::max(1, 2048 / std::max(1, tb));
@@ -647,6 +697,31 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
 
 size_t gf_direct_partial_doubles(int n, int nt) {
     return 3 * size_t(std::max(nt, 1)) * size_t(direct_splits(n, nt));
+}
+
+int gf_potential(int n, const double* pos, const double* mass, double* phi, double eps,
+                 double* partial, hipStream_t s) {
+    if (n <= 0) return 0;
+    for (int t0 = 0; t0 < n; t0 += POT_CHUNK) {
+        const int nt = std::min(POT_CHUNK, n - t0);
+        const int ns = direct_splits(n, nt);
+        const int per = gf_div_up(gf_div_up(n, ns), DT) * DT;
+        const int nsplit = gf_div_up(n, per);
+        hipLaunchKernelGGL(k_potential, dim3(gf_div_up(nt, DT), nsplit), dim3(DT), 0, s, n, t0, nt,
+                           pos, mass, eps * eps, per, partial);
+        hipLaunchKernelGGL(k_potential_reduce, dim3(gf_div_up(nt, DT)), dim3(DT), 0, s, t0, nt,
+                           nsplit, partial, phi);
+        GF_LAUNCH_CHECK();
+    }
+    return 0;
+}
+
+size_t gf_potential_partial_doubles(int n) {
+    // the largest launch: a full chunk, or the last (shorter, more splits)
+    n = std::max(n, 1);
+    const int full = std::min(POT_CHUNK, n), last = n - (n - 1) / POT_CHUNK * POT_CHUNK;
+    return std::max(size_t(full) * size_t(direct_splits(n, full)),
+                    size_t(last) * size_t(direct_splits(n, last)));
 }
 
 int gf_direct(int n, const double* pos, const double* mass, int nt, const int64_t* tgt,

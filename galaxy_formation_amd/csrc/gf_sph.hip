@@ -73,7 +73,9 @@ struct KnnArgs {
                     // global-stack heap pass, [6] queue 2 length, [7] retry list
                     // length; [8] groups of this pass's retry list; [9], [10]
                     // queue 1 and 2 lengths summed over passes; [11] passes that
-                    // had work; [12] errors summed over passes
+                    // had work; [12] errors summed over passes; [13] internal
+                    // bounds errors of the fast pass (bit 0 candidate range,
+                    // bit 1 member-range stack)
     int* ovf;       // queue 1: groups the fast pass hands to the heap pass
     int* ovf2;      // queue 2: groups the heap pass hands to its global-stack variant
     int* gstack;
@@ -347,7 +349,7 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
 __global__ void k_knn_reset(int* state, int first) {
     if (threadIdx.x != 0) return;
     if (first) {
-        state[9] = state[10] = state[11] = state[12] = 0;
+        state[9] = state[10] = state[11] = state[12] = state[13] = 0;
     } else {
         state[9] += state[3];
         state[10] += state[6];
@@ -543,10 +545,15 @@ __device__ int gather_group(const KnnArgs& a, int root, double bx, double by, do
                 if (lincl[mid] > e) hi = mid; else lo = mid + 1;
             }
             pj = lf[lo] + (e - (lo > 0 ? lincl[lo - 1] : 0));
-            const double4 q = a.p4[pj];
-            px = q.x; py = q.y; pz = q.z;
-            const double dx = px - bx, dy = py - by, dz = pz - bz;
-            tk = dx * dx + dy * dy + dz * dz <= R2 && (nm == 0 || point_in_any(px, py, pz, msph, nm));
+            if (pj >= 0 && pj < a.n) {
+                const double4 q = a.p4[pj];
+                px = q.x; py = q.y; pz = q.z;
+                const double dx = px - bx, dy = py - by, dz = pz - bz;
+                tk = dx * dx + dy * dy + dz * dz <= R2 &&
+                     (nm == 0 || point_in_any(px, py, pz, msph, nm));
+            } else {
+                atomicOr(&a.state[13], 1);    // inconsistent range list: an error, no access
+            }
         }
         const uint64_t m = __ballot(tk);
         if (lane == 0) ctl[8 + w] = __popcll(m);
@@ -569,6 +576,9 @@ __device__ int gather_group(const KnnArgs& a, int root, double bx, double by, do
         }
     }
     if (threadIdx.x == 0) ctl[7] = nc;
+    // every wave reads ctl[7] next (the chunk count of a split): the write
+    // must land first
+    __syncthreads();
     return nc > cap ? -1 : nc;
 }
 
@@ -893,27 +903,34 @@ __global__ __launch_bounds__(FT, 3) void k_knn_fast(KnnArgs a) {
                 // level, halves below
                 int C = 2;
                 if (rg == todo && ctl[7] > 0) {
-                    const int want = (2 * ctl[7] + cap - 1) / cap;
+                    const long long want = (2ll * ctl[7] + cap - 1) / cap;
                     while (C < want && C < 8) C <<= 1;
                 }
                 C = min(C, nr);
                 __syncthreads();              // every wave has read rstack[rs] and ctl[7]
-                if (threadIdx.x == 0) {
-                    uint64_t rest = rg;
-                    for (int c = 0; c < C; ++c) {
-                        // chunk c takes the next (nr - taken) / (C - c) members
-                        const int take = (__popcll(rest) + (C - c) - 1) / (C - c);
-                        uint64_t part = 0;
-                        for (int q = 0; q < take; ++q) {
-                            const uint64_t bit = rest & (~rest + 1);
-                            part |= bit;
-                            rest ^= bit;
+                if (rs + C > 16) {
+                    // cannot happen (at most 8 + 3 entries); an error, not a write
+                    if (threadIdx.x == 0) atomicOr(&a.state[13], 2);
+                    handoff = 1;
+                    rs = 0;
+                } else {
+                    if (threadIdx.x == 0) {
+                        uint64_t rest = rg;
+                        for (int c = 0; c < C; ++c) {
+                            // chunk c takes the next (nr - taken) / (C - c) members
+                            const int take = (__popcll(rest) + (C - c) - 1) / (C - c);
+                            uint64_t part = 0;
+                            for (int q = 0; q < take; ++q) {
+                                const uint64_t bit = rest & (~rest + 1);
+                                part |= bit;
+                                rest ^= bit;
+                            }
+                            rstack[rs + C - 1 - c] = part;    // first chunk on top
+                            rmode[rs + C - 1 - c] = 1;
                         }
-                        rstack[rs + C - 1 - c] = part;    // first chunk on top
-                        rmode[rs + C - 1 - c] = 1;
                     }
+                    rs += C;
                 }
-                rs += C;
             } else {
                 // one member whose sphere holds more than the list (ctl[7]
                 // particles at least, 0 = unknown): shrink the radius towards
@@ -1448,6 +1465,11 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
         GF_HIP(hipStreamSynchronize(s));
         if (st[4] != 0 || st[12] != 0) {
             gf_set_error("kNN: %d groups overflowed the global stack", st[4] + st[12]);
+            return -1;
+        }
+        if (st[13] != 0) {
+            gf_set_error("kNN: internal error %d (candidate range or member-range stack out of "
+                         "bounds)", st[13]);
             return -1;
         }
         if (st[0] == 0) break;

@@ -37,7 +37,7 @@ class ForceEvaluator:
         self.dev = torch.device(device)
         self.n, self.ng = n, n_gas
         self.eps, self.theta, self.k = eps, theta, k
-        self.prec = {"fp32": 0, "fp64": 1}[precision]
+        self.prec = {"fp32": 0, "fp64": 1, "fp32q": 2}[precision]
         self.stream = stream
         z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.dev)  # noqa: E731
         self.acc_grav = z(n, 3)
@@ -191,3 +191,190 @@ def training_pairs(records, model, k, max_gap=4):
             raws.append(raw)
             targets.append((torch.log(dk1) - torch.log(dk0)).float())
     return torch.cat(xs), torch.cat(raws), torch.cat(targets)
+
+
+class RungStepper:
+    """Device-resident hierarchical (block) leapfrog with `rungs` time-step
+    levels, the pattern of simulation.py:114-149 (a long step and sub-steps
+    for the fast particles) widened to 2^(rungs-1) sub-steps per long step.
+
+    The smallest step dt_min is the 1st percentile of the particles' admissible
+    steps dt_ind = min(0.3 sqrt(eps/|a|), dt_Courant) (simulation.py:68-72),
+    chosen again at every long-step boundary (as simulation.py:117 chooses
+    its long step each cycle); rung r steps by dt_min 2^(rungs-1-r), the
+    largest step not above dt_ind (rung rungs-1 also takes the particles
+    below dt_min).  Every sub-step
+    drifts all particles by dt_min; the particles at a boundary of their step
+    are active: gravity is evaluated for them only, on a tree rebuilt every
+    sub-step (refit = True refits it between long steps instead, as the
+    reference does, rebuild = 0 of nbody_accel_active; a refit widens every
+    opening radius by the largest drift since the build, which on the GPU
+    costs more walk than a new tree costs build), they take
+    the closing and opening half kicks (v, and u of the gas with du/dt), and
+    may change rung (to a longer step only where it is also a boundary).  The
+    gas hydro (neighbours, h, density, forces) is evaluated for all gas at
+    every sub-step; only the active particles use it.  Cooling, star formation
+    and feedback are not part of this force path.
+
+    Energy and angular momentum are measured where every particle is at a
+    boundary (every 2^(rungs-1) sub-steps), on the synchronised velocities
+    and energies (the opening half kicks taken back): kinetic + thermal +
+    potential, the potential by the FP64 direct sum (gf_potential_dev)."""
+
+    def __init__(self, particles, theta=0.5, k=64, precision="fp32", rungs=6, first="plain",
+                 model=None, device="cuda:0", refit=False):
+        import torch
+        self.torch = torch
+        dev = torch.device(device)
+        self.dev = dev
+        T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+        p = particles
+        self.ng = int((p.kind == 0).sum())
+        assert (np.flatnonzero(p.kind == 0) == np.arange(self.ng)).all(), "gas first"
+        self.n = p.n
+        self.eps = p.eps
+        self.pos, self.vel, self.mass, self.u = T(p.pos), T(p.vel), T(p.mass), T(p.u)
+        self.u_floor = 1e4 * U_PER_K
+        self.R = rungs
+        self.refit = refit
+        self.first, self.model = first, model
+        self.F = ForceEvaluator(p.n, self.ng, p.eps, theta, k, precision, device)
+        self.L = self.F.L
+        self.phi = torch.zeros(p.n, dtype=torch.float64, device=dev)
+        self.rung = torch.zeros(p.n, dtype=torch.int64, device=dev)
+        self.active_counts = []   # device scalars, read at the end
+        self.passes = []
+        self.L_sync = []
+        self.t_hydro = 0.0        # time of the last hydro evaluation
+        self.prev = None          # (d_k, div v, dt_Courant) of that evaluation
+
+    # -- forces ----------------------------------------------------------------
+
+    def _forces(self, active, rebuild, t):
+        torch, F = self.torch, self.F
+        act8 = None if active is None else active.to(torch.uint8).contiguous()
+        F.gravity(self.pos, self.mass, active=act8, rebuild=rebuild)
+        kw = {}
+        if self.first == "learned" and self.prev is not None:
+            kw = dict(first="learned", dk_prev=self.prev[0], divv_prev=self.prev[1],
+                      dtc_prev=self.prev[2], dt=max(t - self.t_hydro, 1e-30), model=self.model)
+        elif self.first == "prev" and self.prev is not None:
+            kw = dict(first="prev", dk_prev=self.prev[0])
+        else:
+            kw = dict(first="plain")
+        if self.ng:
+            self.passes.append(F.hydro(self.pos[:self.ng], self.vel[:self.ng],
+                                       self.mass[:self.ng], self.u[:self.ng], **kw))
+            self.prev = (F.dk.clone(), F.divv.clone(), F.dt_c.clone())
+            self.t_hydro = t
+        F.acc.copy_(F.acc_grav)
+        if self.ng:
+            F.acc[:self.ng] += F.acc_sph
+        amag = torch.linalg.vector_norm(F.acc, dim=1)
+        F.dt_ind = 0.3 * torch.sqrt(self.eps / torch.clamp(amag, min=1e-30))
+        if self.ng:
+            F.dt_ind[:self.ng] = torch.minimum(F.dt_ind[:self.ng], F.dt_c)
+
+    def _rung_of(self, dt):
+        torch = self.torch
+        r = torch.ceil(torch.log2(torch.clamp(self.dt_max / dt, min=1.0)))
+        return torch.clamp(r, 0, self.R - 1).to(torch.int64)
+
+    def _kick(self, sel, dt_r):
+        """Half kick of the selected particles by dt_r / 2 (per particle); no
+        host synchronisation (masks become zero weights)."""
+        torch, F = self.torch, self.F
+        h = torch.where(sel, 0.5 * dt_r, torch.zeros_like(dt_r))
+        self.vel.addcmul_(h[:, None], F.acc)
+        if self.ng:
+            gsel = sel[:self.ng]
+            ug = self.u[:self.ng]
+            ug.copy_(torch.where(gsel, torch.clamp(ug + h[:self.ng] * F.dudt, min=self.u_floor), ug))
+
+    # -- diagnostics -----------------------------------------------------------
+
+    def energy(self):
+        """(kinetic, thermal, potential, |L|) of the synchronised state."""
+        torch, _lib = self.torch, _lib_module()
+        m = self.mass
+        ke = 0.5 * torch.sum(m * torch.sum(self.vel * self.vel, dim=1))
+        te = torch.sum(m[:self.ng] * self.u[:self.ng]) if self.ng else torch.zeros((), dtype=m.dtype)
+        _lib.check(self.L.gf_potential_dev(self.n, self.pos.data_ptr(), m.data_ptr(),
+                                           self.phi.data_ptr(), self.eps, None))
+        torch.cuda.synchronize()
+        pe = 0.5 * torch.sum(m * self.phi)
+        Lv = torch.sum(m[:, None] * torch.linalg.cross(self.pos, self.vel), dim=0)
+        return float(ke), float(te), float(pe), float(torch.linalg.vector_norm(Lv))
+
+    @property
+    def active_fraction(self):
+        return [float(c.item()) / self.n for c in self.active_counts]
+
+    def angular_momentum(self):
+        torch = self.torch
+        Lv = torch.sum(self.mass[:, None] * torch.linalg.cross(self.pos, self.vel), dim=0)
+        return float(torch.linalg.vector_norm(Lv))
+
+    # -- integration -------------------------------------------------------------
+
+    def start(self):
+        """Forces of every particle, rungs, opening half kicks."""
+        torch = self.torch
+        self.t = 0.0
+        self._forces(None, True, 0.0)
+        self.dt_min = float(torch.quantile(self.F.dt_ind.float(), 0.01).item())
+        self.dt_max = self.dt_min * 2 ** (self.R - 1)
+        self.rung = self._rung_of(self.F.dt_ind)
+        self.E0 = self.energy()
+        all_ = torch.ones(self.n, dtype=torch.bool, device=self.dev)
+        self._kick(all_, self.dt_max / 2.0 ** self.rung.double())
+        self.k = 0
+
+    def substep(self):
+        torch = self.torch
+        R = self.R
+        self.k += 1
+        self.t += self.dt_min
+        self.pos += self.dt_min * self.vel
+        period = 2 ** (R - 1 - self.rung)                      # sub-steps per step of each rung
+        active = (self.k % period) == 0
+        self.active_counts.append(active.sum())
+        long_step = self.k % 2 ** (R - 1) == 0
+        self._forces(active, long_step or not self.refit, self.t)
+        dt_old = self.dt_max / 2.0 ** self.rung.double()
+        self._kick(active, dt_old)                             # closing half kicks
+        if long_step:
+            # every particle is at a boundary: a new ladder of steps
+            self.L_sync.append(self.angular_momentum())
+            self.dt_min = float(torch.quantile(self.F.dt_ind.float(), 0.01).item())
+            self.dt_max = self.dt_min * 2 ** (R - 1)
+            self.rung = self._rung_of(self.F.dt_ind)
+        else:
+            new = self._rung_of(self.F.dt_ind)
+            # a longer step only where its own boundary is now
+            longer = new < self.rung
+            ok = (self.k % (2 ** (R - 1 - new))) == 0
+            new = torch.where(longer & ~ok, self.rung, new)
+            self.rung = torch.where(active, new, self.rung)
+        self._kick(active, self.dt_max / 2.0 ** self.rung.double())   # opening half kicks
+
+    def synchronised_energy(self):
+        """Energy and |L| at a long-step boundary (every particle just took its
+        opening half kick with the forces of this sub-step): the kicks are
+        taken back on copies, measured, and the state restored."""
+        torch = self.torch
+        assert self.k % 2 ** (self.R - 1) == 0, "not at a long-step boundary"
+        v0, u0 = self.vel.clone(), self.u.clone()
+        h = 0.5 * self.dt_max / 2.0 ** self.rung.double()
+        self.vel -= h[:, None] * self.F.acc
+        if self.ng:
+            self.u[:self.ng] = self.u[:self.ng] - h[:self.ng] * self.F.dudt
+        try:
+            return self.energy()
+        finally:
+            self.vel.copy_(v0)
+            self.u.copy_(u0)
+
+
+def _lib_module():
+    return _lib

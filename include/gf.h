@@ -105,6 +105,11 @@ int gf_walk_dev(int n, const uint8_t* active, double* acc, double eps, int preci
 /* Device-pointer FP64 direct sum for nt targets. */
 int gf_direct_dev(int n, const double* pos, const double* mass, int nt, const int64_t* tgt,
                   double* acc, double eps, void* stream);
+/* No counterpart in the reference (which reports no energies): the softened
+ * FP64 potential phi_i = -sum_{j != i} m_j / sqrt(r_ij^2 + eps^2) of all n
+ * particles by direct summation, for the energy of a run (device arrays). */
+int gf_potential_dev(int n, const double* pos, const double* mass, double* phi, double eps,
+                     void* stream);
 /* Interaction count of the last gravity call (synchronises). */
 int gf_interactions_dev(long long* out);
 /* The part of that count that are node (not particle) entries. */
