@@ -62,8 +62,8 @@ def test_direct_potential_matches_numpy(n):
     _lib.check(L.gf_potential_dev(n, dpos.data_ptr(), dm.data_ptr(), phi.data_ptr(), eps, None))
     torch.cuda.synchronize()
     got = phi.cpu().numpy()
-    tg = np.unique(np.concatenate([rng.integers(0, n, 64), [0, n - 1],
-                                   [t for t in (262143, 262144, 262145) if t < n]]))
+    edge = [t for t in (0, n - 1, 262143, 262144, 262145) if t < n]
+    tg = np.unique(np.concatenate([rng.integers(0, n, 64), np.array(edge, dtype=np.int64)]))
     for t in tg:
         d = pos - pos[t]
         r = np.sqrt((d * d).sum(1) + eps * eps)
