@@ -409,9 +409,98 @@ def main():
         "previous_step_D": D,
         "accuracy": accuracy,
     }
+    out["gas_physics"] = gas_physics(B)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(B)
     print(json.dumps(out))
+
+
+def gas_physics(B, launches=20, n_new=1000, k_fb=16):
+    """The sub-grid steps beside the force path, on the workload's gas, each
+    timed alone with HIP events on its stream:
+      * thermodynamics of every gas row (simulation.py:97-104): one pass,
+        48 B per row (g, u, du/dt, rho, no_cool_until read; u written);
+      * supernova feedback of n_new new stars (star_formation.py:52-68): the
+        k_fb-nearest query against all gas, its tree build included (the
+        reference builds a cKDTree per call), and the heating; scipy's
+        cKDTree build + query on the host beside it, and whether the two
+        neighbour lists agree."""
+    import torch
+    from scipy.spatial import cKDTree
+
+    from galaxy_formation_amd import ics
+    from galaxy_formation_amd.star_formation import E_SN_PER_MSUN
+    from galaxy_formation_amd.units import MSUN_G, MYR
+    L, chk, sp, ng, n = B.L, B._lib.check, B.sp, B.ng, B.n
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    g = torch.arange(ng, dtype=torch.int64, device=B.dev)
+    u = B.u.clone()
+    ncu = torch.full((n,), -np.inf, dtype=torch.float64, device=B.dev)
+    dudt, rho = B.out[1]["dudt"], B.out[1]["rho"]
+
+    def thermo():
+        chk(L.gf_thermo_dev(ng, g.data_ptr(), n, u.data_ptr(), dudt.data_ptr(), rho.data_ptr(),
+                            ncu.data_ptr(), 0.0, 1e-4, ics.U_PER_K, 1e4 * ics.U_PER_K, sp))
+    thermo()
+    torch.cuda.synchronize()
+    e0.record(B.stream)
+    for _ in range(launches):
+        thermo()
+    e1.record(B.stream)
+    torch.cuda.synchronize()
+    t_thermo = e0.elapsed_time(e1) / launches
+    chk(L.gf_gas_errors_dev(sp))
+    gbs = 48.0 * ng / (t_thermo * 1e-3) / 1e9
+
+    rng = np.random.default_rng(0)
+    new = np.sort(rng.choice(ng, n_new, replace=False))
+    new_t = torch.from_numpy(new).to(B.dev)
+    q = B.pos[new_t].contiguous()
+    m_new = B.mass[new_t].contiguous()
+    nb = torch.empty(n_new * k_fb, dtype=torch.int64, device=B.dev)
+    dist = torch.empty(n_new * k_fb, dtype=torch.float64, device=B.dev)
+
+    def query():
+        chk(L.gf_knn_query_dev(ng, B.pos.data_ptr(), n_new, q.data_ptr(), k_fb, nb.data_ptr(),
+                               dist.data_ptr(), sp))
+    query()
+    reps = 5
+    torch.cuda.synchronize()
+    e0.record(B.stream)
+    for _ in range(reps):
+        query()
+    e1.record(B.stream)
+    torch.cuda.synchronize()
+    t_query = e0.elapsed_time(e1) / reps
+    e_per_g = 0.4 * E_SN_PER_MSUN / MSUN_G
+
+    def heat():
+        chk(L.gf_feedback_heat_dev(n_new, k_fb, nb.data_ptr(), m_new.data_ptr(), e_per_g,
+                                   30.0 / MYR, n, B.mass.data_ptr(), u.data_ptr(), ncu.data_ptr(),
+                                   sp))
+    heat()
+    torch.cuda.synchronize()
+    e0.record(B.stream)
+    for _ in range(reps):
+        heat()
+    e1.record(B.stream)
+    torch.cuda.synchronize()
+    t_heat = e0.elapsed_time(e1) / reps
+
+    pg = B.pos[:ng].cpu().numpy()
+    qn = q.cpu().numpy()
+    w0 = time.perf_counter()
+    _, nb_ref = cKDTree(pg).query(qn, k=k_fb, workers=-1)
+    t_cpu = 1e3 * (time.perf_counter() - w0)
+    same = bool(np.array_equal(nb.cpu().numpy().reshape(n_new, k_fb), nb_ref))
+    return {
+        "thermodynamics": {"rows": ng, "ms": t_thermo, "bytes_per_row": 48,
+                           "achieved_GBps": gbs, "frac_of_8TBps": gbs / 8000.0},
+        "feedback": {"new_stars": n_new, "k": k_fb, "gas": ng,
+                     "knn_query_ms_incl_tree_build": t_query, "heat_ms": t_heat,
+                     "cpu_ckdtree_build_and_query_ms": t_cpu, "cpu_threads": os.cpu_count(),
+                     "neighbours_equal_ckdtree": same},
+    }
 
 
 def accuracy_table(B):

@@ -71,6 +71,16 @@ SIGNATURES = {
     "gf_mlp_grad_host": (_ci, [_ci, np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS"), _f32,
                                _f32, _cf, _f32, _vp]),
     "gf_hydro_neighbours": (_ci, [_i64, _vp]),
+    # gas physics (cooling, thermodynamics, star formation, feedback)
+    "gf_gas_constants": (_ci, [_dbl]),
+    "gf_cooling_function_dev": (_ci, [_ci, _vp, _vp, _vp]),
+    "gf_cool_dev": (_ci, [_ci, _vp, _vp, _vp, _cd, _vp, _cd, _vp, _vp]),
+    "gf_thermo_dev": (_ci, [_ci, _vp, _ci, _vp, _vp, _vp, _vp, _cd, _cd, _cd, _cd, _vp]),
+    "gf_gas_errors_dev": (_ci, [_vp]),
+    "gf_sf_candidates_dev": (_ci, [_ci, _vp, _ci, _vp, _vp, _cd, _cd, _cd, _cd, _vp, _vp, _vp]),
+    "gf_sf_select_dev": (_ci, [_ci, _vp, _vp, _vp, _vp, _vp]),
+    "gf_knn_query_dev": (_ci, [_ci, _vp, _ci, _vp, _ci, _vp, _vp, _vp]),
+    "gf_feedback_heat_dev": (_ci, [_ci, _ci, _vp, _vp, _cd, _cd, _ci, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -7,6 +7,8 @@
 
 #define GF_ABI_VERSION 1
 
+#include "gf_gas.h"
+
 struct GfCtx {
     int inited = 0;
     int device = 0;
@@ -39,6 +41,7 @@ struct GfCtx {
     int* d_flag = nullptr; size_t d_flag_cap = 0;
     float* d_mlp = nullptr; size_t d_mlp_cap = 0;
     float* d_mlp_part = nullptr; size_t d_mlp_part_cap = 0;
+    GfGas gas;                      // gas-physics scratch (cooling, star formation, feedback)
 };
 
 extern GfCtx g_ctx;

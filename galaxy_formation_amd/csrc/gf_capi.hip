@@ -673,4 +673,100 @@ int gf_mlp_grad_host(int n, const uint16_t* x, const float* target, const float*
     return 0;
 }
 
+
+// ------------------------------------------------------------ gas physics
+// cooling.py, simulation.py:97-104, star_formation.py (gf_gas.hip).
+
+int gf_gas_constants(double* out) {
+    if (!out) {
+        gf_set_error("gf_gas_constants: null output");
+        return -1;
+    }
+    const GfGasUnits& u = gf_gas_units();
+    const double v[8] = {u.msun, u.msun_g, u.m_h, u.x_h, u.rho_cgs, u.u_cgs, u.myr, u.sec_per_myr};
+    memcpy(out, v, sizeof(v));
+    return 0;
+}
+
+int gf_cooling_function_dev(int n, const double* T, double* lam, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (n < 0 || (n > 0 && (!T || !lam))) {
+        gf_set_error("gf_cooling_function_dev: bad arguments");
+        return -1;
+    }
+    return gf_cooling_function(n, T, lam, gf_stream(stream));
+}
+
+int gf_cool_dev(int n, const double* u, const double* T, const double* rho, double dt,
+                const uint8_t* blocked, double u_floor, double* out, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (n < 0 || (n > 0 && (!u || !T || !rho || !out))) {
+        gf_set_error("gf_cool_dev: bad arguments");
+        return -1;
+    }
+    return gf_cool(n, u, T, rho, dt, blocked, u_floor, out, gf_stream(stream));
+}
+
+int gf_thermo_dev(int ng, const int64_t* g, int n_total, double* u, const double* dudt,
+                  const double* rho, const double* no_cool_until, double t, double dt,
+                  double u_per_K, double u_floor, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (ng < 0 || (ng > 0 && (!u || !dudt || !rho)) || (!g && n_total < ng)) {
+        gf_set_error("gf_thermo_dev: bad arguments");
+        return -1;
+    }
+    return gf_thermo_launch(g_ctx.gas, ng, g, n_total, u, dudt, rho, no_cool_until, t, dt,
+                            u_per_K, u_floor, gf_stream(stream));
+}
+
+int gf_gas_errors_dev(void* stream) {
+    if (gf_ctx_init()) return -1;
+    return gf_gas_errors(g_ctx.gas, gf_stream(stream));
+}
+
+int gf_sf_candidates_dev(int ng, const int64_t* g, int n_total, const double* u,
+                         const double* rho, double u_per_K, double rho_sf, double sf_eff,
+                         double dt, int64_t* rows, double* p, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (ng < 0 || (ng > 0 && (!u || !rho || !rows || !p)) || (!g && n_total < ng)) {
+        gf_set_error("gf_sf_candidates_dev: bad arguments");
+        return -1;
+    }
+    return gf_sf_candidates(g_ctx.gas, ng, g, n_total, u, rho, u_per_K, rho_sf, sf_eff, dt, rows,
+                            p, gf_stream(stream));
+}
+
+int gf_sf_select_dev(int m, const int64_t* rows, const double* p, const double* uniform,
+                     int64_t* out, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (m < 0 || (m > 0 && (!rows || !p || !uniform || !out))) {
+        gf_set_error("gf_sf_select_dev: bad arguments");
+        return -1;
+    }
+    return gf_sf_select(g_ctx.gas, m, rows, p, uniform, out, gf_stream(stream));
+}
+
+int gf_knn_query_dev(int n_src, const double* src, int n_q, const double* q, int k, int64_t* nb,
+                     double* dist, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (n_src < 1 || n_q < 0 || !src || (n_q > 0 && (!q || !nb))) {
+        gf_set_error("gf_knn_query_dev: bad arguments");
+        return -1;
+    }
+    return gf_knn_query(g_ctx.gas, n_src, src, n_q, q, k, nb, dist, gf_stream(stream));
+}
+
+int gf_feedback_heat_dev(int n_new, int k, const int64_t* targets, const double* new_mass,
+                         double e_per_g, double until, int n_total, const double* mass, double* u,
+                         double* no_cool_until, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (n_new < 0 || n_total < 1 ||
+        (n_new > 0 && (!targets || !new_mass || !mass || !u || !no_cool_until))) {
+        gf_set_error("gf_feedback_heat_dev: bad arguments");
+        return -1;
+    }
+    return gf_feedback_heat(g_ctx.gas, n_new, k, targets, new_mass, e_per_g, until, n_total, mass,
+                            u, no_cool_until, gf_stream(stream));
+}
+
 }  // extern "C"

@@ -284,7 +284,8 @@ __global__ __launch_bounds__(TB) void k_gather(const double* __restrict__ pos,
     const int i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     const int j = int(idx[i]);
-    const double x = pos[3 * j], y = pos[3 * j + 1], z = pos[3 * j + 2], m = mass[j];
+    const double x = pos[3 * j], y = pos[3 * j + 1], z = pos[3 * j + 2];
+    const double m = mass ? mass[j] : 0.0;      // no masses: a neighbour-search tree
     tx[i] = x; ty[i] = y; tz[i] = z; tm[i] = m;
     double d = 0.0;
     if (xb) {

@@ -11,14 +11,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-KPC = 3.5                       # length unit in kpc          (units.py:18)
-MSUN = 4e10                     # mass unit in Msun           (units.py:19)
-G_KPC = 4.30091e-6              # kpc (km/s)^2 / Msun         (units.py:17)
-KMS = np.sqrt(G_KPC * MSUN / KPC)
-M_H = 1.6726e-24                # g                           (units.py:24)
-K_B = 1.381e-16                 # erg / K                     (units.py:25)
-MU_IONISED = 0.6                #                             (units.py:35)
-U_CGS = (KMS * 1e5) ** 2        # erg / g                     (units.py:31)
+from .units import G_KPC, K_B, KMS, KPC, M_H, MSUN, MU_IONISED, U_CGS  # noqa: F401
+
 GAMMA = 5.0 / 3.0
 U_PER_K = K_B / ((GAMMA - 1) * MU_IONISED * M_H) / U_CGS   # simulation.py:40
 

@@ -219,6 +219,59 @@ int gf_mlp_train_step_dev(int n, const uint16_t* x, const float* target, float* 
 int gf_mlp_grad_host(int n, const uint16_t* x, const float* target, const float* params,
                      float tau, float* grad, float* loss);
 
+/* ------------------------------------------------------------ gas physics
+ * The sub-grid steps around the force path, in FP64 with the reference's
+ * operation order.  Device arrays; gas rows are 0 .. ng-1 and, where a gas
+ * index array g (int64, NULL = identity) is given, u and no_cool_until are
+ * the simulation's arrays over all n_total particles indexed through g, as
+ * self.u[g] in galaxy_formation/simulation.py. */
+
+/* The unit constants of galaxy_formation/units.py:17-35 as this library
+ * evaluates them (host array of 8): MSUN, MSUN_G, M_H, X_H, RHO_CGS, U_CGS,
+ * MYR, SEC_PER_MYR. */
+int gf_gas_constants(double* out);
+/* cooling.cooling_function (cooling.py:24-32): Lambda(T) in erg cm^3 / s. */
+int gf_cooling_function_dev(int n, const double* T, double* lam, void* stream);
+/* cooling.cool (cooling.py:35-43): out = max(u - du, u_floor) with du the
+ * energy radiated in dt at density rho, 0 where blocked (u8, may be NULL). */
+int gf_cool_dev(int n, const double* u, const double* T, const double* rho, double dt,
+                const uint8_t* blocked, double u_floor, double* out, void* stream);
+/* Simulation._thermodynamics (simulation.py:97-104) for ng gas rows, in place
+ * on u: u[g] = max(u[g] + dt dudt, u_floor), then cooling with
+ * T = u[g] / u_per_K, blocked where no_cool_until[g] > t (no_cool_until may be
+ * NULL).  dudt and rho are per gas row.  Rows whose g is outside
+ * [0, n_total) are skipped and counted for gf_gas_errors_dev. */
+int gf_thermo_dev(int ng, const int64_t* g, int n_total, double* u, const double* dudt,
+                  const double* rho, const double* no_cool_until, double t, double dt,
+                  double u_per_K, double u_floor, void* stream);
+/* Rows skipped by gf_thermo_dev since the last call (waits; clears). */
+int gf_gas_errors_dev(void* stream);
+/* star_formation.form_stars (star_formation.py:37-47), first half: the gas
+ * rows of dense (rho > rho_sf), cold (u[g] / u_per_K < 3e4 K) gas, ascending,
+ * into rows, and p = 1 - exp(-sf_eff dt / t_ff), t_ff = sqrt(3 pi / (32 rho)).
+ * Returns the count m (waits). */
+int gf_sf_candidates_dev(int ng, const int64_t* g, int n_total, const double* u,
+                         const double* rho, double u_per_K, double rho_sf, double sf_eff,
+                         double dt, int64_t* rows, double* p, void* stream);
+/* Second half (star_formation.py:46): rows[i] for which uniform[i] < p[i],
+ * in order, into out.  The m uniform draws come from the caller's generator
+ * (numpy's rng.uniform(size=m) in the reference).  Returns the count (waits). */
+int gf_sf_select_dev(int m, const int64_t* rows, const double* p, const double* uniform,
+                     int64_t* out, void* stream);
+/* The k nearest source points of each query point: cKDTree(src).query(q, k)
+ * of star_formation.py:62.  nb (n_q, k) gets source input indices ascending in
+ * (distance, index), dist (may be NULL) the distances.  1 <= k <= 256.
+ * Waits. */
+int gf_knn_query_dev(int n_src, const double* src, int n_q, const double* q, int k, int64_t* nb,
+                     double* dist, void* stream);
+/* Supernova heating (star_formation.py:63-68): for the (n_new, k) targets
+ * (particle indices), du = E / k / (mass[t] MSUN MSUN_G) / U_CGS with
+ * E = e_per_g new_mass MSUN MSUN_G, added to u in the order np.add.at adds the
+ * flattened list; no_cool_until[targets] = until.  Waits for the index check. */
+int gf_feedback_heat_dev(int n_new, int k, const int64_t* targets, const double* new_mass,
+                         double e_per_g, double until, int n_total, const double* mass, double* u,
+                         double* no_cool_until, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
