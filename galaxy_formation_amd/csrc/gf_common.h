@@ -90,6 +90,18 @@ __device__ __forceinline__ uint64_t gf_key_prefix(uint64_t key, int d) {
 
 static inline int gf_div_up(long long a, long long b) { return int((a + b - 1) / b); }
 
+// The workgroups of a launch go to the chip's 8 XCDs round-robin (block b on
+// XCD b % 8), and every XCD has an L2 of its own.  This logical block index
+// gives XCD x one contiguous range of the blocks instead (a bijection of
+// 0 .. gridDim.x - 1), so that work neighbouring in memory shares an L2.
+#define GF_XCDS 8
+__device__ __forceinline__ int gf_xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int x = b % GF_XCDS, i = b / GF_XCDS;
+    const int per = nb / GF_XCDS, rem = nb % GF_XCDS;
+    return x * per + min(x, rem) + i;
+}
+
 // Capacities that tests lower to drive the fallback paths (0 = default):
 // the walk's LDS stack entries and the neighbour solve's candidate list.
 struct GfTestLimits {

@@ -1131,24 +1131,53 @@ __global__ __launch_bounds__(TB) void k_pack(SphView v, double gamma, SphRec* __
 // rho_i = sum_j m_j W(r_ij, h_i) over the neighbour table (sph.py:63-71),
 // neighbours read from the compact (x, y, z, m) copy (four per cache line);
 // also completes the particle's record with rho and P / rho^2.
+// Lanes per particle of the density and force sums: lane q of a particle sums
+// its neighbours q, q + SPH_LANES, ...; the partial sums combine by
+// butterfly steps.  With four lanes per particle and the blocks of each XCD
+// in particle order, an XCD has about 16k particles in flight, whose
+// neighbours' 128-byte records (about 2 MB) fit its 4 MB L2.
+#ifndef SPH_LANES
+#define SPH_LANES 4
+#endif
+constexpr int SL = SPH_LANES;
+
+template <typename T>
+__device__ __forceinline__ T lanes_sum(T v) {
+#pragma unroll
+    for (int o = 1; o < SL; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double lanes_max(double v) {
+#pragma unroll
+    for (int o = 1; o < SL; o <<= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 __global__ __launch_bounds__(TB) void k_density(int n, const int* __restrict__ nb, int k,
                                                 const double* __restrict__ u, double gamma,
                                                 const double4* __restrict__ a4,
                                                 SphRec* __restrict__ rec,
                                                 double* __restrict__ rho) {
 #pragma clang fp contract(off)
-    const int t = blockIdx.x * TB + threadIdx.x;
-    if (t >= n) return;
-    const double4 at = a4[t];
-    const double4 ct = rec[t].c;
+    const int gt = gf_xcd_block() * TB + threadIdx.x;   // particles in order per XCD
+    const int t = gt / SL, q = gt % SL;
+    const bool ok = t < n;                    // every lane takes part in the sums
+    const int tt = ok ? t : 0;
+    const double4 at = a4[tt];
+    const double4 ct = rec[tt].c;
     const double hv = ct.x, ihv = ct.y, s3 = 1.0 / (M_PI * (hv * hv * hv));
     double s = 0.0;
-    for (int jj = 0; jj < k; ++jj) {
-        const int j = nb[size_t(jj) * n + t];
-        const double4 aj = a4[j];
-        const double dx = at.x - aj.x, dy = at.y - aj.y, dz = at.z - aj.z;
-        s += aj.w * kernel_w(sqrt(dx * dx + dy * dy + dz * dz), ihv, s3);
+    if (ok) {
+        for (int jj = q; jj < k; jj += SL) {
+            const int j = nb[size_t(jj) * n + t];
+            const double4 aj = a4[j];
+            const double dx = at.x - aj.x, dy = at.y - aj.y, dz = at.z - aj.z;
+            s += aj.w * kernel_w(sqrt(dx * dx + dy * dy + dz * dz), ihv, s3);
+        }
     }
+    s = lanes_sum(s);
+    if (!ok || q != 0) return;
     rho[t] = s;
     rec[t].c.w = s;
     const double P = (gamma - 1) * s * u[t];
@@ -1165,41 +1194,51 @@ __global__ __launch_bounds__(TB) void k_force(int n, const int* __restrict__ nb,
                                               double* __restrict__ rho_out,
                                               double* __restrict__ h_out) {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * TB + threadIdx.x;
-    if (i >= n) return;
-    const SphRec ri = rec[i];
+    const int gt = gf_xcd_block() * TB + threadIdx.x;   // particles in order per XCD
+    const int i = gt / SL, q = gt % SL;
+    const bool ok = i < n;
+    const SphRec ri = rec[ok ? i : 0];
     const double xi = ri.a.x, yi = ri.a.y, zi = ri.a.z;
     const double vxi = ri.b.x, vyi = ri.b.y, vzi = ri.b.z, ci = ri.b.w;
     const double hi_ = ri.c.x, ihi = ri.c.y, sgi = ri.c.z, rhoi = ri.c.w;
     const double pi2 = ri.d.x;
     double ax = 0, ay = 0, az = 0, du = 0, mu_max = 0, dv = 0;
-    for (int jj = 0; jj < k; ++jj) {
-        const int j = nb[size_t(jj) * n + i];
-        if (j == i) continue;
-        const double4 aj = rec[j].a, bj = rec[j].b, cj4 = rec[j].c;
-        const double pj2 = rec[j].d.x;
-        const double dx = xi - aj.x, dy = yi - aj.y, dz = zi - aj.z;
-        const double r = sqrt(dx * dx + dy * dy + dz * dz);
-        const double gi = kernel_g(r, ihi, sgi);
-        const double gj = kernel_g(r, cj4.y, cj4.z);
-        const double rhoj = cj4.w, hj = cj4.x, cj = bj.w;
-        const double vr = (vxi - bj.x) * dx + (vyi - bj.y) * dy + (vzi - bj.z) * dz;
-        double visc = 0.0;
-        if (vr < 0) {
-            const double hb = 0.5 * (hi_ + hj);
-            const double mu = hb * vr / (r * r + 0.01 * hb * hb);
-            mu_max = fmax(mu_max, -mu);
-            visc = (-alpha * 0.5 * (ci + cj) * mu + beta * mu * mu) / (0.5 * (rhoi + rhoj));
+    if (ok) {
+        for (int jj = q; jj < k; jj += SL) {
+            const int j = nb[size_t(jj) * n + i];
+            if (j == i) continue;
+            const double4 aj = rec[j].a, bj = rec[j].b, cj4 = rec[j].c;
+            const double pj2 = rec[j].d.x;
+            const double dx = xi - aj.x, dy = yi - aj.y, dz = zi - aj.z;
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            const double gi = kernel_g(r, ihi, sgi);
+            const double gj = kernel_g(r, cj4.y, cj4.z);
+            const double rhoj = cj4.w, hj = cj4.x, cj = bj.w;
+            const double vr = (vxi - bj.x) * dx + (vyi - bj.y) * dy + (vzi - bj.z) * dz;
+            double visc = 0.0;
+            if (vr < 0) {
+                const double hb = 0.5 * (hi_ + hj);
+                const double mu = hb * vr / (r * r + 0.01 * hb * hb);
+                mu_max = fmax(mu_max, -mu);
+                visc = (-alpha * 0.5 * (ci + cj) * mu + beta * mu * mu) / (0.5 * (rhoi + rhoj));
+            }
+            const double gb = 0.5 * (gi + gj);
+            const double f = pi2 * gi + pj2 * gj + visc * gb;
+            const double mj = aj.w;
+            ax -= mj * f * dx;
+            ay -= mj * f * dy;
+            az -= mj * f * dz;
+            du += mj * (pi2 * gi + 0.5 * visc * gb) * vr;
+            dv += mj * vr * gi;
         }
-        const double gb = 0.5 * (gi + gj);
-        const double f = pi2 * gi + pj2 * gj + visc * gb;
-        const double mj = aj.w;
-        ax -= mj * f * dx;
-        ay -= mj * f * dy;
-        az -= mj * f * dz;
-        du += mj * (pi2 * gi + 0.5 * visc * gb) * vr;
-        dv += mj * vr * gi;
     }
+    ax = lanes_sum(ax);
+    ay = lanes_sum(ay);
+    az = lanes_sum(az);
+    du = lanes_sum(du);
+    dv = lanes_sum(dv);
+    mu_max = lanes_max(mu_max);
+    if (!ok || q != 0) return;
     const size_t o = out_idx ? size_t(out_idx[i]) : size_t(i);
     if (acc) {
         acc[3 * o] = ax;
@@ -1500,8 +1539,9 @@ int gf_sph_forces_tree(GfSph& S, int k, double gamma, double alpha, double beta,
     SphRec* rec = reinterpret_cast<SphRec*>(S.rec);
     double4* a4 = reinterpret_cast<double4*>(S.p4);      // free after the solve
     hipLaunchKernelGGL(k_pack, dim3(g), dim3(TB), 0, s, v, gamma, rec, a4);
-    hipLaunchKernelGGL(k_density, dim3(g), dim3(TB), 0, s, n, S.nb, k, S.u, gamma, a4, rec, S.rho);
-    hipLaunchKernelGGL(k_force, dim3(g), dim3(TB), 0, s, n, S.nb, k, rec, alpha, beta,
+    const int gl = gf_div_up((long long)n * SL, TB);
+    hipLaunchKernelGGL(k_density, dim3(gl), dim3(TB), 0, s, n, S.nb, k, S.u, gamma, a4, rec, S.rho);
+    hipLaunchKernelGGL(k_force, dim3(gl), dim3(TB), 0, s, n, S.nb, k, rec, alpha, beta,
                        S.tree.idx, acc, dudt, dt, divv, rho, h_out);
     GF_LAUNCH_CHECK();
     return 0;
@@ -1525,8 +1565,9 @@ int gf_sph_forces_table(int n, int k, const double* pos, const double* vel, cons
     SphView v{soa, soa + n, soa + 2 * size_t(n), soa + 3 * size_t(n), soa + 4 * size_t(n),
               soa + 5 * size_t(n), mass, u, h, n};
     hipLaunchKernelGGL(k_pack, dim3(g), dim3(TB), 0, s, v, gamma, rec, a4);
-    hipLaunchKernelGGL(k_density, dim3(g), dim3(TB), 0, s, n, nbt, k, u, gamma, a4, rec, rr);
-    hipLaunchKernelGGL(k_force, dim3(g), dim3(TB), 0, s, n, nbt, k, rec, alpha, beta, nullptr, acc,
+    const int gl = gf_div_up((long long)n * SL, TB);
+    hipLaunchKernelGGL(k_density, dim3(gl), dim3(TB), 0, s, n, nbt, k, u, gamma, a4, rec, rr);
+    hipLaunchKernelGGL(k_force, dim3(gl), dim3(TB), 0, s, n, nbt, k, rec, alpha, beta, nullptr, acc,
                        dudt, dt, divv, rho, nullptr);
     GF_LAUNCH_CHECK();
     return 0;
