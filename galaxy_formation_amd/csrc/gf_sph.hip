@@ -373,11 +373,19 @@ __global__ void k_knn_reset(int* state, int first) {
 // the radius; one whose sphere holds more candidates than the list shrinks
 // it; both take another pass.
 constexpr int FT = 256;
-constexpr int CMAX = 1024;                   // candidate list of a group gather
-constexpr int CPL = CMAX / 64;
-constexpr int FSCAP = 1024;                  // per-wave stack
-constexpr int LMAX = FSCAP;                  // touched ranges of a group gather
-constexpr int EMAX = 4 * CMAX;               // particles in those ranges
+// Two shapes of the fast pass.  A solve that starts from the caller's radii
+// (the previous d_k, the learned proposal) holds about k candidates per
+// member: a 512-entry list, 4 blocks per CU, and the union of the member
+// spheres gathered first.  The cell-density start overshoots: a 1024-entry
+// list and the bounding sphere first (fewer shrink passes).
+struct KnnNarrow {
+    static constexpr int CMAX = 512, BLOCKS_PER_CU = 4, UNION_FIRST = 1;
+};
+struct KnnWide {
+    static constexpr int CMAX = 1024, BLOCKS_PER_CU = 3, UNION_FIRST = 0;
+};
+constexpr int CMAX_MAX = 1024;               // largest candidate list (test limits)
+constexpr int FSCAP = 1024;                  // stack of the gather walk
 constexpr int FAST_BLOCKS = 2048;
 constexpr uint32_t KEY_NONE = 0xFFFFFFFFu;
 constexpr double KEY_SCALE = 4294967294.0;   // keys d^2 / r^2 * KEY_SCALE < KEY_NONE
@@ -479,7 +487,8 @@ __device__ __forceinline__ int push_children(const NodeHit& h, int* stack, int s
 // order, so the candidate order is deterministic.  Called by all threads.
 // Returns the count, -2 for a depth-capped leaf, -1 when the sphere holds
 // more than cap particles (ctl[7] = an estimate of their number, from the
-// share of entries inside) or its ranges more than LMAX / EMAX (ctl[7] = 0).
+// share of entries inside) or its ranges more than CMAX / 4 CMAX (ctl[7] = 0).
+template <int CMAX>
 __device__ int gather_group(const KnnArgs& a, int root, double bx, double by, double bz,
                             double R2, const double4* msph, int nm, double* lx, double* ly,
                             double* lz, int* li, int cap, int* stack, int* lf, int* lincl,
@@ -506,7 +515,8 @@ __device__ int gather_group(const KnnArgs& a, int root, double bx, double by, do
             const int einc = gf_wave_incl_scan(h.cnt);
             const int ebatch = __shfl(einc, 63, 64);
             const int nlb = __popcll(lm);
-            if (nl + nlb > LMAX || et + ebatch > EMAX) {
+            // ranges touched at most CMAX, particles in them at most 4 CMAX
+            if (nl + nlb > CMAX || et + ebatch > 4 * CMAX) {
                 res = -1;
                 break;
             }
@@ -617,6 +627,7 @@ __device__ __forceinline__ Sphere wave_sphere(bool mine, double xi, double yi, d
 // per lane (input indices are read only when two of them tie in d^2).
 // Returns 1 when resolved, 0 when the radius was too small (grown), -1 on
 // more than 64 equal keys (left for the heap pass).
+template <int CMAX>
 __device__ int select_member(const KnnArgs& a, int t, double xi, double yi, double zi, double ri,
                              int npass, const double* lx, const double* ly, const double* lz,
                              const int* li, int nc, int* hist, double* tie_d, int* tie_i,
@@ -631,6 +642,7 @@ __device__ int select_member(const KnnArgs& a, int t, double xi, double yi, doub
 #pragma unroll
     for (int q = 0; q < 4; ++q) hist[4 * lane + q] = 0;
     __builtin_amdgcn_wave_barrier();
+    constexpr int CPL = CMAX / 64;
     uint32_t key[CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
@@ -791,11 +803,14 @@ __device__ int select_member(const KnnArgs& a, int t, double xi, double yi, doub
 // not fit shrinks its radius and takes another pass.  Groups it cannot
 // finish (a depth-capped leaf, more than 64 equal keys, a closed radius
 // bracket) go to the heap pass's queue.
-__global__ __launch_bounds__(FT, 3) void k_knn_fast(KnnArgs a) {
+template <typename Shape>
+__global__ __launch_bounds__(FT, Shape::BLOCKS_PER_CU) void k_knn_fast(KnnArgs a) {
 #pragma clang fp contract(off)
+    constexpr int CMAX = Shape::CMAX;
     __shared__ double cx[CMAX], cy[CMAX], cz[CMAX];
     __shared__ int ci[CMAX];
-    __shared__ int stack[4][FSCAP];
+    __shared__ int stack[FSCAP];
+    __shared__ int lf[CMAX], lincl[CMAX];
     __shared__ __align__(16) int hist[4][256];
     __shared__ double tie_d[4][64];
     __shared__ int tie_i[4][64];
@@ -842,7 +857,7 @@ __global__ __launch_bounds__(FT, 3) void k_knn_fast(KnnArgs a) {
         if (todo != 0) {
             if (threadIdx.x == 0) {
                 rstack[0] = todo;
-                rmode[0] = 0;
+                rmode[0] = Shape::UNION_FIRST;
             }
             rs = 1;
         }
@@ -869,8 +884,8 @@ __global__ __launch_bounds__(FT, 3) void k_knn_fast(KnnArgs a) {
                 const int root = containing_node(
                     a, nr == 1 ? a.leaf_of[b0 + __ffsll((unsigned long long)rg) - 1] : G, S.bx,
                     S.by, S.bz, S.R);
-                nc = gather_group(a, root, S.bx, S.by, S.bz, S.R * S.R, msph, nm, cx, cy, cz, ci,
-                                  cap, stack[0], stack[1], stack[2], ctl);
+                nc = gather_group<CMAX>(a, root, S.bx, S.by, S.bz, S.R * S.R, msph, nm, cx, cy, cz, ci,
+                                  cap, stack, lf, lincl, ctl);
             }
             if (nc == -2) {
                 handoff = 1;
@@ -883,7 +898,7 @@ __global__ __launch_bounds__(FT, 3) void k_knn_fast(KnnArgs a) {
                     const double xi = __shfl(xm, mm, 64), yi = __shfl(ym, mm, 64);
                     const double zi = __shfl(zm, mm, 64), ri = __shfl(rm, mm, 64);
                     const int npass = __shfl(pm_, mm, 64);
-                    const int res = select_member(a, b0 + mm, xi, yi, zi, ri, npass, cx, cy, cz,
+                    const int res = select_member<CMAX>(a, b0 + mm, xi, yi, zi, ri, npass, cx, cy, cz,
                                                   ci, nc, hist[w], tie_d[w], tie_i[w], tie_p[w]);
                     if (res < 0) handoff = 1;
                     if (res == 0) again = 1;
@@ -1461,7 +1476,7 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
     a.p4 = reinterpret_cast<const double4*>(S.p4);
     a.gmark = S.gmark;
     a.retry_in = nullptr;
-    a.list_cap = gf_test_limits().knn_list > 0 ? gf_test_limits().knn_list : CMAX;
+    a.list_cap = gf_test_limits().knn_list > 0 ? gf_test_limits().knn_list : CMAX_MAX;
     a.start = T.start; a.count = T.count; a.leaf = T.leaf; a.child8 = T.child8; a.box = T.box;
     a.groups = T.groups; a.counters = T.counters; a.tx = T.tx; a.ty = T.ty; a.tz = T.tz;
     a.idx = T.idx; a.inv = S.inv; a.parent_ = T.parent; a.leaf_of = S.leaf_of; a.r = S.r;
@@ -1490,7 +1505,8 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
             a.pass_tag = pass + 1;
             hipLaunchKernelGGL(k_knn_reset, dim3(1), dim3(64), 0, s, S.state, pass == 0 ? 1 : 0);
             if (!sorted)
-                hipLaunchKernelGGL(k_knn_fast, dim3(FAST_BLOCKS), dim3(FT), 0, s, a);
+                hipLaunchKernelGGL(r_init ? k_knn_fast<KnnNarrow> : k_knn_fast<KnnWide>,
+                                   dim3(FAST_BLOCKS), dim3(FT), 0, s, a);
             if (kq <= 64) {
                 hipLaunchKernelGGL((k_knn_pass<64, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
                 hipLaunchKernelGGL((k_knn_pass<64, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);

@@ -267,18 +267,20 @@ def test_device_neighbours_with_exact_ties():
         np.testing.assert_array_equal(dk, od[:, -1])
 
 
+@pytest.mark.parametrize("first", [None, 1.2])
 @pytest.mark.parametrize("k,cap", [(32, 40), (64, 70), (64, 300)])
-def test_neighbours_with_small_candidate_lists(k, cap):
+def test_neighbours_with_small_candidate_lists(k, cap, first):
     """Candidate lists cut to `cap` entries push groups into member mode,
     radius shrinking and the heap pass; the neighbour sets and d_k stay
-    those of brute force."""
+    those of brute force.  Both shapes of the fast pass: the cell-density
+    start (first None) and a caller's first radius (first x d_k)."""
     from galaxy_formation_amd import _lib
     L = _lib.load()
     pos, *_ = disk_gas(6000, seed=13)
     od, onb = O.knn(pos, k)
     _lib.check(L.gf_set_test_limits(0, cap))
     try:
-        nb, dk, passes = _device_solve(pos, k)
+        nb, dk, passes = _device_solve(pos, k, None if first is None else first * od[:, -1])
         st = np.zeros(6, np.int64)
         _lib.check(L.gf_hydro_stats(st, 6))
     finally:
