@@ -114,6 +114,17 @@ int nbody_num_threads(void) {
     return g_ctx.n_cu;
 }
 
+int gf_set_quad_tolerance(double tol) {
+    if (!(tol >= 0.0) || tol > 1.0) {
+        gf_set_error("quadrupole tolerance must be in [0, 1]");
+        return -1;
+    }
+    g_ctx.walk.quad_tol = float(tol);
+    return 0;
+}
+
+double gf_get_quad_tolerance(void) { return double(g_ctx.walk.quad_tol); }
+
 int gf_set_test_limits(int walk_stack, int knn_list) {
     if (walk_stack < 0 || walk_stack > 1024 || knn_list < 0 || knn_list > 1024) {
         gf_set_error("test limits: walk_stack and knn_list must be in [0, 1024] (0 = default)");

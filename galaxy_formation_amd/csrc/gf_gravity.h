@@ -16,6 +16,10 @@ struct GfWalkScratch {
     void* gnode = nullptr;                 // packed nodes (128 bytes each)
     void* p4 = nullptr;                    // packed particles (x, y, z, m), tree order
     int cap_gnode = 0, cap_p4 = 0;
+    // precision 2: an accepted node whose quadrupole term is estimated below
+    // this share of its monopole at the group's nearest point is evaluated
+    // as a monopole (0 = every accepted node with its quadrupole)
+    float quad_tol = 0.01f;
 };
 
 int gf_walk_reserve(GfWalkScratch& W, int max_groups);

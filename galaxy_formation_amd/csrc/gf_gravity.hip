@@ -48,7 +48,9 @@ struct GNode {
     int leaf, pad0, pad1, pad2;
     int4 c_lo, c_hi;      // children in octant order, -1 = empty
     float q[6];           // traceless quadrupole (xx, yy, zz, xy, xz, yz), FP32
-    float qpad0, qpad1;
+    float qrel;           // 3 max |Q_ij| / m: the quadrupole term's share of the
+                          // monopole at distance d is at most about qrel / d^2
+    float qpad1;
 };
 static_assert(sizeof(GNode) == 128, "GNode layout");
 
@@ -78,6 +80,8 @@ struct WalkArgs {
     int* gstack;
     int lds_scap;              // LDS stack entries used by the first launch (<= SCAP)
     unsigned long long* inter_nodes;   // the node (monopole or quadrupole) share of inter
+    float quad_tol;            // QUAD: a node whose quadrupole term is below this share of
+                               // its monopole at the group's nearest point is a monopole
 };
 
 __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v) {
@@ -280,7 +284,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 const int node = lane < take ? stack[sn + lane] : -1;
                 __syncthreads();
                 int push = 0, nent = 0, first = 0;
-                bool mono = false;
+                bool mono = false, small_q = false;
                 int4 c_lo = make_int4(-1, -1, -1, -1), c_hi = c_lo;
                 double4 cm = make_double4(0, 0, 0, 0);
                 if (node >= 0) {
@@ -292,9 +296,16 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                     const int4 k_lo = g->c_lo, k_hi = g->c_hi;
                     const double dx = cm.x - bx, dy = cm.y - by, dz = cm.z - bz;
                     const double t = orr + br;
-                    if (dx * dx + dy * dy + dz * dz > t * t) {
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    if (d2 > t * t) {
                         mono = true;
                         nent = 1;
+                        if constexpr (QUAD) {
+                            // the quadrupole term's share at the group's
+                            // nearest point (an estimate: FP32 is enough)
+                            const float dm = sqrtf(float(d2)) - float(br);
+                            small_q = g->qrel < a.quad_tol * (dm * dm);
+                        }
                     } else if (gl) {
                         first = gs;
                         nent = gc;
@@ -326,25 +337,50 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 const uint64_t mmask = __ballot(mono);
                 const int nmono = __popcll(mmask);
                 if (nmono > 0) {
-                    total_nodes += nmono;
                     if constexpr (QUAD) {
-                        if (nq + nmono > QC) {
-                            __syncthreads();
-                            if (evl) A.flush_quad(s_qe, s_qa, s_qb, nq, sl, S, eps2r);
-                            total_len += nq;
-                            nq = 0;
-                            __syncthreads();
+                        // nodes whose quadrupole term is negligible join the
+                        // particle list as monopoles, the others the
+                        // quadrupole list
+                        const bool quad = mono && !small_q;
+                        const uint64_t qmask = __ballot(quad);
+                        const uint64_t smask = mmask & ~qmask;
+                        const int nquad = __popcll(qmask), nsm = __popcll(smask);
+                        total_nodes += nquad;
+                        if (nquad > 0) {
+                            if (nq + nquad > QC) {
+                                __syncthreads();
+                                if (evl) A.flush_quad(s_qe, s_qa, s_qb, nq, sl, S, eps2r);
+                                total_len += nq;
+                                nq = 0;
+                                __syncthreads();
+                            }
+                            if (quad) {
+                                const int slot = nq + __popcll(qmask & gf_lanemask_lt());
+                                const GNode* g = a.gn + node;
+                                s_qe[slot] = make_float4(float(cm.x - bx), float(cm.y - by),
+                                                         float(cm.z - bz), float(cm.w));
+                                s_qa[slot] = make_float4(g->q[0], g->q[1], g->q[2], g->q[3]);
+                                s_qb[slot] = make_float4(g->q[4], g->q[5], 0.0f, 0.0f);
+                            }
+                            nq += nquad;
                         }
-                        if (mono) {
-                            const int slot = nq + __popcll(mmask & gf_lanemask_lt());
-                            const GNode* g = a.gn + node;
-                            s_qe[slot] = make_float4(float(cm.x - bx), float(cm.y - by),
-                                                     float(cm.z - bz), float(cm.w));
-                            s_qa[slot] = make_float4(g->q[0], g->q[1], g->q[2], g->q[3]);
-                            s_qb[slot] = make_float4(g->q[4], g->q[5], 0.0f, 0.0f);
+                        if (nsm > 0) {
+                            if (ln + nsm > LC) {
+                                __syncthreads();
+                                if (evl) A.flush(s_list, ln, sl, S, eps2r);
+                                total_len += ln;
+                                ln = 0;
+                                __syncthreads();
+                            }
+                            if (mono && small_q) {
+                                const int slot = ln + __popcll(smask & gf_lanemask_lt());
+                                s_list[slot] = make_float4(float(cm.x - bx), float(cm.y - by),
+                                                           float(cm.z - bz), float(cm.w));
+                            }
+                            ln += nsm;
                         }
-                        nq += nmono;
                     } else {
+                        total_nodes += nmono;
                         if (ln + nmono > LC) {
                             __syncthreads();
                             if (evl) A.flush(s_list, ln, sl, S, eps2r);
@@ -472,8 +508,14 @@ __global__ __launch_bounds__(256) void k_gnode_pack(const int* __restrict__ coun
         const int4* cs = reinterpret_cast<const int4*>(child8 + 8 * size_t(id));
         g.c_lo = cs[0];
         g.c_hi = cs[1];
-        for (int k = 0; k < 6; ++k) g.q[k] = quad ? float(quad[6 * size_t(id) + k]) : 0.0f;
-        g.qpad0 = g.qpad1 = 0.0f;
+        double qmax = 0.0;
+        for (int k = 0; k < 6; ++k) {
+            const double qk = quad ? quad[6 * size_t(id) + k] : 0.0;
+            g.q[k] = float(qk);
+            qmax = fmax(qmax, fabs(qk));
+        }
+        g.qrel = g.com.w > 0 ? float(3.0 * qmax / g.com.w) : 0.0f;
+        g.qpad1 = 0.0f;
         gn[id] = g;
     }
 }
@@ -680,6 +722,7 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
     a.gstack = W.gstack;
     a.inter_nodes = W.inter_nodes;
     a.lds_scap = gf_test_limits().walk_stack > 0 ? std::min(SCAP, gf_test_limits().walk_stack) : SCAP;
+    a.quad_tol = W.quad_tol;
     hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, W.work, W.ovf_count);
     if (precision == GF_FP64) {
         hipLaunchKernelGGL((k_walk<true, false, false>), dim3(WALK_BLOCKS), dim3(64), 0, s, a);

@@ -15,12 +15,25 @@ class Engine:
     """Thin, typed interface to libgf.so (engine.py:39-72 of the reference).
 
     precision: "fp32" (FP32 pair arithmetic on group-relative coordinates,
-    compensated accumulation; the default) or "fp64".  Tree, centres of mass
-    and opening tests are FP64 in both."""
+    compensated accumulation; the default), "fp64", or "fp32q" (as fp32 with
+    the accepted nodes' quadrupoles).  Tree, centres of mass and opening tests
+    are FP64 in all three.  quad_tolerance (fp32q): a node whose quadrupole
+    term is estimated below this share of its monopole is evaluated as a
+    monopole (0 = every node with its quadrupole)."""
 
-    def __init__(self, precision="fp32"):
+    def __init__(self, precision="fp32", quad_tolerance=None):
         self.lib = _lib.load()
         self.precision = precision
+        if quad_tolerance is not None:
+            self.quad_tolerance = quad_tolerance
+
+    @property
+    def quad_tolerance(self):
+        return float(self.lib.gf_get_quad_tolerance())
+
+    @quad_tolerance.setter
+    def quad_tolerance(self, tol):
+        _lib.check(self.lib.gf_set_quad_tolerance(float(tol)))
 
     @property
     def precision(self):

@@ -68,6 +68,13 @@ int gf_device_info(char* buf, int len);
  * acceptance tests are FP64 in all three; the interaction set is the same. */
 int gf_set_precision(int precision);
 int gf_get_precision(void);
+/* Precision 2 only, no counterpart in the reference: an accepted node whose
+ * quadrupole term is estimated (3 max |Q_ij| / (m d^2), d the distance from
+ * its centre of mass to the nearest point of the group's sphere) below tol
+ * of its monopole is evaluated as a monopole.  Default 0.01; 0 evaluates
+ * every accepted node with its quadrupole.  tol in [0, 1]. */
+int gf_set_quad_tolerance(double tol);
+double gf_get_quad_tolerance(void);
 /* Testing hook, no counterpart in the reference: lowers the capacity of
  * the gravity walk's LDS stack (walk_stack entries, 1..1024) and of the
  * neighbour solve's candidate lists (knn_list, 1..1024) so that tests can

@@ -255,9 +255,15 @@ def test_global_stack_walk_equals_lds_walk(eng, limit):
 @pytest.mark.parametrize("theta", [0.4, 0.5, 0.7])
 def test_fp32_quadrupole_close_to_oracle(eng, theta):
     """Mode 2 (FP32 evaluation with node quadrupoles) against the oracle's
-    FP64 mode 2: the same interaction set, FP32 rounding apart."""
+    FP64 mode 2: the same interaction set, FP32 rounding apart.  Tolerance 0:
+    every accepted node with its quadrupole, as the oracle."""
     c = ics.config("cloud10k")
-    acc = run(eng, c.pos, c.mass, theta, c.eps, "fp32q")
+    tol = eng.quad_tolerance
+    eng.quad_tolerance = 0.0
+    try:
+        acc = run(eng, c.pos, c.mass, theta, c.eps, "fp32q")
+    finally:
+        eng.quad_tolerance = tol
     inter = eng.last_interactions()
     ref, oi = O.accel(c.pos, c.mass, theta, c.eps, mode=2)
     assert inter == oi
@@ -274,3 +280,30 @@ def test_quadrupoles_reduce_the_error_vs_direct_sum(eng):
         e1 = relerr(run(eng, c.pos, c.mass, theta, c.eps, "fp32"), exact)
         e2 = relerr(run(eng, c.pos, c.mass, theta, c.eps, "fp32q"), exact)
         assert np.sqrt(np.mean(e2 ** 2)) < 0.4 * np.sqrt(np.mean(e1 ** 2))
+
+
+def test_quadrupole_tolerance_trades_little_accuracy():
+    """The default tolerance (0.01) evaluates the nodes whose quadrupole term
+    is negligible as monopoles: fewer quadrupole entries, the same entry
+    count, and an error vs the direct sum within the 1e-3 gate and close to
+    the all-quadrupole one."""
+    from galaxy_formation_amd import _lib
+    from galaxy_formation_amd.engine import Engine
+    e = Engine("fp32q")
+    L = _lib.load()
+    pos, mass = ics.gaussian_blob(20000, 3)
+    eps = 0.01
+    exact = e.gravity_direct(pos, mass, eps)
+    res = {}
+    for tol in (0.0, 0.01):
+        e.quad_tolerance = tol
+        acc = run(e, pos, mass, 0.5, eps, "fp32q")
+        nodes = np.zeros(1, np.int64)
+        _lib.check(L.gf_node_interactions_dev(_lib.ptr(nodes)))
+        res[tol] = (acc, e.last_interactions(), int(nodes[0]))
+    e.quad_tolerance = 0.01
+    (a0, i0, q0), (a1, i1, q1) = res[0.0], res[0.01]
+    assert i0 == i1 and q1 < q0
+    r0 = np.sqrt(np.mean(relerr(a0, exact) ** 2))
+    r1 = np.sqrt(np.mean(relerr(a1, exact) ** 2))
+    assert r1 < 1e-3 and r1 < 2.0 * r0
