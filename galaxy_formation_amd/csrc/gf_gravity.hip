@@ -28,15 +28,27 @@
 
 namespace {
 
-constexpr int SCAP = 1024;                          // LDS stack entries
-constexpr int LCAP = 256;                           // LDS list entries
-constexpr int LCAP_QUAD = 192;                      // particle list with quadrupoles
-constexpr int QCAP = 64;                            // node list with quadrupoles (48 B each)
 #ifndef WALK_WAVES
 #define WALK_WAVES 4
 #endif
+#ifndef WALK_LCAP_QUAD
+#define WALK_LCAP_QUAD 192
+#endif
+#ifndef WALK_QCAP
+#define WALK_QCAP 64
+#endif
+#ifndef WALK_SCAP
+#define WALK_SCAP 1024
+#endif
+constexpr int SCAP = WALK_SCAP;                     // LDS stack entries
+constexpr int LCAP = 256;                           // LDS list entries
+constexpr int LCAP_QUAD = WALK_LCAP_QUAD;           // particle list with quadrupoles
+constexpr int QCAP = WALK_QCAP;                     // node list with quadrupoles (48 B each)
 constexpr int GSCAP = 512 * (GF_KEY_LEVELS + 2);    // global stack entries per block
-constexpr int WALK_BLOCKS = 8192;
+#ifndef WALK_GRID
+#define WALK_GRID 8192
+#endif
+constexpr int WALK_BLOCKS = WALK_GRID;              // persistent one-wave blocks
 constexpr int GST_BLOCKS = 256;
 
 // A tree node packed for the walk (128 bytes): everything one visit reads,

@@ -386,7 +386,13 @@ struct KnnWide {
 };
 constexpr int CMAX_MAX = 1024;               // largest candidate list (test limits)
 constexpr int FSCAP = 1024;                  // stack of the gather walk
-constexpr int FAST_BLOCKS = 2048;
+// persistent blocks of the fast pass: the narrow shape's residency (4 per CU
+// on 256 CUs); more only queue behind them and hold back the gravity walk
+// that runs beside the solve on another stream
+#ifndef KNN_FAST_GRID
+#define KNN_FAST_GRID 1024
+#endif
+constexpr int FAST_BLOCKS = KNN_FAST_GRID;
 constexpr uint32_t KEY_NONE = 0xFFFFFFFFu;
 constexpr double KEY_SCALE = 4294967294.0;   // keys d^2 / r^2 * KEY_SCALE < KEY_NONE
 constexpr double GROUP_REACH = 3.0;          // gather once while R <= 3 max r_i
