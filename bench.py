@@ -44,7 +44,7 @@ PEAK_FP64 = 78.6e12
 # 0 monopole, 2 quadrupole): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from
 # rocprofv3 --pmc (profiles/); counters cannot be read inside a run, so the
 # measured figures are carried here
-WALK_TRAFFIC_BYTES = {0: 3440326848, 2: None}
+WALK_TRAFFIC_BYTES = {0: 3433121792, 2: 3625940992}
 # the accuracy gates of the metric: RMS relative acceleration error vs the
 # FP64 direct sum, and density vs the plain h solve (with h equal to it)
 GATE_ACC_RMS = 1e-3
