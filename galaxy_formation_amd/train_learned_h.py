@@ -65,10 +65,13 @@ def main(argv=None):
     t0 = time.time()
     model = LearnedH()
     model.k = 64
-    # (label, gas particles, IC seed, leapfrog steps, step factor)
+    # (label, gas particles, IC seed, leapfrog steps, step factor); factor 1
+    # is the long step of simulation.py:117 (10th percentile of the
+    # admissible steps), 0.2 the scale of the sub-steps of a 6-rung ladder
     runs = [("cloud10k", 5000, 3, 40, 1.0), ("cloud10k", 5000, 4, 40, 3.0),
             ("cloud200k", 100000, 3, 10, 1.0), ("cloud200k", 100000, 5, 10, 3.0),
-            ("disk200k", 100000, 17, 8, 1.0), ("disk200k", 100000, 19, 8, 2.5)]
+            ("disk200k", 100000, 17, 8, 1.0), ("disk200k", 100000, 19, 8, 2.5),
+            ("cloud200k", 100000, 7, 12, 0.2), ("disk200k", 100000, 23, 12, 0.2)]
     x, t = build_dataset(model, runs, samples=a.samples)
     n = x.shape[0]
     perm = torch.randperm(n, device=x.device, generator=torch.Generator(device=x.device).manual_seed(1))
