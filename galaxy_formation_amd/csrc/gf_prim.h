@@ -5,8 +5,11 @@
 // Ints of temporary storage gf_scan_excl needs for n inputs.
 size_t gf_scan_temp_ints(int n);
 // out[i] = in[0] + ... + in[i-1] for i in [0, n]; out has n + 1 entries
-// (out[n] is the total).  in == out is allowed.
-int gf_scan_excl(const int* in, int* out, int n, int* temp, hipStream_t s);
+// (out[n] is the total).  in == out is allowed.  With len (device) given,
+// only the first m = min(n, *len) inputs are scanned: out[0..m] is written,
+// out past m is not.
+int gf_scan_excl(const int* in, int* out, int n, int* temp, hipStream_t s,
+                 const int* len = nullptr);
 
 // Ints of temporary storage gf_radix_sort_pairs needs for n pairs.
 size_t gf_sort_temp_ints(int n);

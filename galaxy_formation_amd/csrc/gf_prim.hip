@@ -28,9 +28,15 @@ __device__ int block_excl_scan_256(int v, int* lds /*4 ints*/, int* total) {
     return off + inc - v;
 }
 
+// len (device, may be null): scan only the first min(n, *len) inputs; the
+// blocks past them do no loads.
+__device__ __forceinline__ int scan_len(int n, const int* len) { return len ? min(n, *len) : n; }
+
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int* __restrict__ in, int n,
+                                                              const int* __restrict__ len,
                                                               int* __restrict__ partial) {
     __shared__ int lds[4];
+    n = scan_len(n, len);
     const long long base = (long long)blockIdx.x * SCAN_TILE;
     int s = 0;
 #pragma unroll
@@ -75,9 +81,13 @@ __global__ __launch_bounds__(1024) void k_scan_partials(int* partial, int nb) {
 // in and out may be the same array: every thread reads its items before the
 // block barrier and writes only those items after it.
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int* in, int n,
+                                                            const int* __restrict__ len,
                                                             const int* __restrict__ partial,
                                                             int nb, int* out) {
     __shared__ int lds[4];
+    n = scan_len(n, len);
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partial[nb];
+    if ((long long)blockIdx.x * SCAN_TILE >= n) return;      // a whole tile past the length
     const long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS];
     int s = 0;
@@ -95,43 +105,50 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int* in, int n
         if (i < n) out[i] = run;
         run += v[k];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partial[nb];
 }
 
 // ------------------------------------------------------------------ radix sort
-constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 16;                       // per lane
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;     // 4096 keys per block
-constexpr int RS_WAVE_SPAN = 64 * RS_ITEMS;        // 1024 contiguous keys per wave
+// A block sorts a tile of 4096 keys per pass with 16 waves of 4 keys per lane
+// (wide blocks keep a 1M-key pass at about 16 waves per CU: a tile per block
+// of 4 waves left most CUs with one block).
+#ifndef RS_THREADS_CFG
+#define RS_THREADS_CFG 1024
+#endif
+constexpr int RS_THREADS = RS_THREADS_CFG;
+constexpr int RS_TILE = 4096;                      // keys per block
+constexpr int RS_ITEMS = RS_TILE / RS_THREADS;     // per lane
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_WAVE_SPAN = 64 * RS_ITEMS;        // contiguous keys per wave
+static_assert(RS_ITEMS * RS_THREADS == RS_TILE && RS_THREADS >= 256, "radix tile shape");
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ key, int n,
                                                         int shift, int nblocks,
                                                         int* __restrict__ hist) {
     __shared__ int h[256];
-    h[threadIdx.x] = 0;
+    if (threadIdx.x < 256) h[threadIdx.x] = 0;
     __syncthreads();
     const long long base = (long long)blockIdx.x * RS_TILE;
-#pragma unroll 4
+#pragma unroll
     for (int k = 0; k < RS_ITEMS; ++k) {
         long long i = base + (long long)k * RS_THREADS + threadIdx.x;
         if (i < n) atomicAdd(&h[(key[i] >> shift) & 0xFF], 1);
     }
     __syncthreads();
-    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    if (threadIdx.x < 256) hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter: wave w of block b owns keys [b*TILE + w*1024, +1024), read
-// as 16 chunks of 64.  Inside a chunk, lanes holding the same digit are
-// ranked by lane order (8 ballots); chunks are ranked in order through a
-// per-wave digit counter; waves are ranked in order through an LDS prefix.
+// Stable scatter: wave w of block b owns keys [b*TILE + w*SPAN, +SPAN), read
+// as chunks of 64.  Inside a chunk, lanes holding the same digit are ranked
+// by lane order (8 ballots); chunks are ranked in order through a per-wave
+// digit counter; waves are ranked in order through an LDS prefix.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const uint64_t* __restrict__ key, const uint32_t* __restrict__ val, int n, int shift,
     int nblocks, const int* __restrict__ offs, uint64_t* __restrict__ key_out,
     uint32_t* __restrict__ val_out) {
-    __shared__ int cnt[4][256];
-    __shared__ int wbase[4][256];
+    __shared__ int cnt[RS_WAVES][256];
+    __shared__ int wbase[RS_WAVES][256];
     const int lane = __lane_id(), w = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < 4 * 256; i += RS_THREADS) (&cnt[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_THREADS) (&cnt[0][0])[i] = 0;
     __syncthreads();
 
     const long long base = (long long)blockIdx.x * RS_TILE + (long long)w * RS_WAVE_SPAN;
@@ -165,11 +182,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
         r_[c] = ok ? pre + rank : -1;
     }
     __syncthreads();
-    {
-        const int d = threadIdx.x;   // one digit per thread
+    for (int d = threadIdx.x; d < 256; d += RS_THREADS) {   // one digit per thread
         int run = offs[d * nblocks + blockIdx.x];
 #pragma unroll
-        for (int ww = 0; ww < 4; ++ww) {
+        for (int ww = 0; ww < RS_WAVES; ++ww) {
             wbase[ww][d] = run;
             run += cnt[ww][d];
         }
@@ -190,15 +206,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
 
 size_t gf_scan_temp_ints(int n) { return size_t(gf_div_up(n, SCAN_TILE) + 1); }
 
-int gf_scan_excl(const int* in, int* out, int n, int* partial, hipStream_t s) {
+int gf_scan_excl(const int* in, int* out, int n, int* partial, hipStream_t s, const int* len) {
     if (n <= 0) {
         GF_HIP(hipMemsetAsync(out, 0, sizeof(int), s));
         return 0;
     }
     const int nb = gf_div_up(n, SCAN_TILE);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, partial);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, len, partial);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, partial, nb);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, partial, nb, out);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, len, partial, nb,
+                       out);
     GF_LAUNCH_CHECK();
     return 0;
 }
@@ -222,10 +239,11 @@ int gf_radix_sort_pairs(uint64_t* key, uint32_t* val, uint64_t* key_alt, uint32_
         hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, s, ka, n, shift, nb, hist);
         // exclusive scan of the bin-major histogram, in place
         const int nbs = gf_div_up(nh, SCAN_TILE);
-        hipLaunchKernelGGL(k_scan_reduce, dim3(nbs), dim3(SCAN_THREADS), 0, s, hist, nh, partial);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nbs), dim3(SCAN_THREADS), 0, s, hist, nh, nullptr,
+                           partial);
         hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, partial, nbs);
-        hipLaunchKernelGGL(k_scan_down, dim3(nbs), dim3(SCAN_THREADS), 0, s, hist, nh, partial,
-                           nbs, hist);
+        hipLaunchKernelGGL(k_scan_down, dim3(nbs), dim3(SCAN_THREADS), 0, s, hist, nh, nullptr,
+                           partial, nbs, hist);
         hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_THREADS), 0, s, ka, va, n, shift, nb,
                            hist, kb, vb);
         GF_LAUNCH_CHECK();

@@ -632,7 +632,8 @@ int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float
     hipLaunchKernelGGL(k_group_flag, dim3(ng), dim3(TB), 0, s, T.counters, T.count, T.parent,
                        T.leaf, len, T.gflag);
     GF_LAUNCH_CHECK();
-    if (gf_scan_excl(T.gflag, T.gflag, len, T.scan_tmp, s)) return -1;
+    // only the node count's flags matter: counters[0] bounds the scan
+    if (gf_scan_excl(T.gflag, T.gflag, len, T.scan_tmp, s, T.counters)) return -1;
     hipLaunchKernelGGL(k_group_compact, dim3(ng), dim3(TB), 0, s, T.counters, T.count, T.parent,
                        T.leaf, T.gflag, T.groups);
     GF_LAUNCH_CHECK();
