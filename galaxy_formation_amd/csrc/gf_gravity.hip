@@ -716,6 +716,7 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
         return -1;
     }
     if (gf_walk_reserve(W, T.cap_nodes) || walk_pack_reserve(W, T.cap_nodes, T.n)) return -1;
+    T.want_quad = precision == GF_FP32_QUAD;
     if (precision == GF_FP32_QUAD && gf_tree_quadrupoles(T, s)) return -1;
     hipLaunchKernelGGL(k_gnode_pack, dim3(std::min(2048, gf_div_up(T.cap_nodes, 256))), dim3(256), 0,
                        s, T.counters, T.com, T.open_r, T.start, T.count, T.leaf, T.child8,

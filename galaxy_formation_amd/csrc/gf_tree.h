@@ -34,6 +34,8 @@ struct GfTree {
     double *quad = nullptr;                        // traceless quadrupole about the centre of
                                                    // mass (xx, yy, zz, xy, xz, yz), FP64
     mutable int quad_ok = 0;                       // quad matches the held centres of mass
+    mutable int want_quad = 0;                     // the last walk used quadrupoles: the next
+                                                   // centre-of-mass pass computes them too
     int *gflag = nullptr;                          // nodes + 1, then group offsets
     int *groups = nullptr;                         // group nodes, DFS order
 

@@ -385,13 +385,15 @@ __global__ __launch_bounds__(TB) void k_depth_scatter(const int* __restrict__ co
 // one depth; the deeper levels are done.  Leaves sum their particles in tree
 // order (nbody.cpp:124-131), internal nodes their children in octant order
 // (nbody.cpp:136-147), both in FP64.
+template <bool QUAD>
 __global__ __launch_bounds__(TB) void k_level_com(
     int d, const int* __restrict__ doff, const int* __restrict__ dlist,
     const int* __restrict__ start, const int* __restrict__ count, const int8_t* __restrict__ leaf,
     const int* __restrict__ child8, const double4* __restrict__ box,
     const double* __restrict__ tx, const double* __restrict__ ty, const double* __restrict__ tz,
     const double* __restrict__ tm, const double* __restrict__ tdisp, double4* __restrict__ com,
-    double* __restrict__ open_r, double* __restrict__ ndisp, double theta_d) {
+    double* __restrict__ open_r, double* __restrict__ ndisp, double theta_d,
+    double* __restrict__ quad) {
 #pragma clang fp contract(off)
     const int b0 = doff[d], b1 = doff[d + 1];
     for (int k = b0 + blockIdx.x * TB + threadIdx.x; k < b1; k += gridDim.x * TB) {
@@ -420,7 +422,26 @@ __global__ __launch_bounds__(TB) void k_level_com(
                 dm = fmax(dm, ndisp[c]);
             }
         }
-        finalize_node(id, M, sx, sy, sz, dm, theta_d, box, com, open_r, ndisp);
+        const double4 c = finalize_node(id, M, sx, sy, sz, dm, theta_d, box, com, open_r, ndisp);
+        if constexpr (QUAD) {
+            // the traceless quadrupole about the new centre of mass, as
+            // k_level_quad forms it (same terms, same order)
+            double q[6] = {0, 0, 0, 0, 0, 0};
+            if (leaf[id]) {
+                const int b = start[id], e = b + count[id];
+                for (int i = b; i < e; ++i) quad_add(q, tm[i], tx[i] - c.x, ty[i] - c.y, tz[i] - c.z);
+            } else {
+                const int* slots = child8 + 8 * size_t(id);
+                for (int o = 0; o < 8; ++o) {
+                    const int ch = slots[o];
+                    if (ch < 0) continue;
+                    const double4 cm = com[ch];
+                    for (int j = 0; j < 6; ++j) q[j] += quad[6 * size_t(ch) + j];
+                    quad_add(q, cm.w, cm.x - c.x, cm.y - c.y, cm.z - c.z);
+                }
+            }
+            for (int j = 0; j < 6; ++j) quad[6 * size_t(id) + j] = q[j];
+        }
     }
 }
 
@@ -559,14 +580,21 @@ void gf_tree_free(GfTree& T) {
 // nodes, which sizes the grids of the shallow levels.
 static int com_pass(GfTree& T, hipStream_t s) {
     const int* doff = T.dmeta + 32;
+    // the quadrupoles ride along when the last walk of this tree wanted them
+    const bool fuse = T.want_quad != 0;
     for (int d = GF_KEY_LEVELS; d >= 0; --d) {
         const int g = d >= 6 ? 1024 : std::max(1, 1 << std::max(0, 3 * d - 8));
-        hipLaunchKernelGGL(k_level_com, dim3(g), dim3(TB), 0, s, d, doff, T.dlist, T.start,
-                           T.count, T.leaf, T.child8, T.box, T.tx, T.ty, T.tz, T.tm, T.tdisp,
-                           T.com, T.open_r, T.ndisp, double(T.theta));
+        if (fuse)
+            hipLaunchKernelGGL(k_level_com<true>, dim3(g), dim3(TB), 0, s, d, doff, T.dlist,
+                               T.start, T.count, T.leaf, T.child8, T.box, T.tx, T.ty, T.tz, T.tm,
+                               T.tdisp, T.com, T.open_r, T.ndisp, double(T.theta), T.quad);
+        else
+            hipLaunchKernelGGL(k_level_com<false>, dim3(g), dim3(TB), 0, s, d, doff, T.dlist,
+                               T.start, T.count, T.leaf, T.child8, T.box, T.tx, T.ty, T.tz, T.tm,
+                               T.tdisp, T.com, T.open_r, T.ndisp, double(T.theta), nullptr);
     }
     GF_LAUNCH_CHECK();
-    T.quad_ok = 0;
+    T.quad_ok = fuse ? 1 : 0;
     return 0;
 }
 
