@@ -13,7 +13,9 @@
 //     their particles; opened internal nodes push their children.
 //   * The list lives in LDS and is evaluated whenever it fills.  Small groups
 //     split the list over several lanes per particle ("slices"), so a group
-//     of P particles keeps 64 / P lanes busy per particle.
+//     of P particles keeps 64 / P lanes busy per particle.  A group whose P
+//     leaves many lanes idle is evaluated as two parts (32, 16 or 8
+//     particles, then the rest), each with its own slices.
 //   * FP32 evaluation works on coordinates relative to the group centre b,
 //     rounded to FP32 once per list entry and group; each flush's FP32 partial
 //     sum is added to a compensated (two-sum) FP32 accumulator.
@@ -197,6 +199,26 @@ struct Acc<true> {
     __device__ double3 total() const { return make_double3(ax, ay, az); }
 };
 
+// A chunk of P <= 64 particles is evaluated in one part or in two.  A part of
+// q particles runs 64/q slices over the list, so its pass costs 1/(64/q) of a
+// full-wave pass: one part of 40 particles leaves 24 lanes idle for a whole
+// pass, where parts of 32 and 8 take 1/2 + 1/8 of one.  Returns the size of
+// the first part (32, 16 or 8), or P when a second part would not save at
+// least 10 %.
+__device__ __forceinline__ int chunk_first_part(int P) {
+    int best = P, bnum = 1, bden = 64 / P;          // cost bnum / bden passes
+    for (int q = 32; q >= 8; q >>= 1) {
+        if (P <= q) continue;
+        const int sa = 64 / q, sb = 64 / (P - q);
+        if (10 * (sa + sb) * bden < 9 * sa * sb * bnum) {
+            best = q;
+            bnum = sa + sb;
+            bden = sa * sb;
+        }
+    }
+    return best;
+}
+
 // QUAD (FP32 only): accepted nodes carry their quadrupole and go to a list of
 // their own; particles keep the monopole list.
 template <bool F64, bool GST, bool QUAD>
@@ -267,21 +289,37 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
 
         for (int c0 = 0; c0 < np; c0 += 64) {
             const int P = min(64, np - c0);
-            const int S = 64 / P;
-            const int p = lane % P, sl = lane / P;
-            const bool evl = sl < S;
-            const int ip = b + c0 + p;
-            Acc<F64> A;
+            // parts A (particles [0, Pa)) and B ([Pa, P), possibly empty):
+            // lane l evaluates slice l / Pa of particle l % Pa, and slice
+            // l / Pb of particle Pa + l % Pb
+            const int Pa = chunk_first_part(P), Pb = P - Pa;
+            const int Sa = 64 / Pa, Sb = Pb > 0 ? 64 / Pb : 0;
+            const int sla = lane / Pa, slb = Pb > 0 ? lane / Pb : 64;
+            const bool evla = sla < Sa, evlb = slb < Sb;
+            Acc<F64> A, B;
             {
-                const double4 q = a.p4[ip];
+                const double4 q = a.p4[b + c0 + lane % Pa];
+                const double4 r = a.p4[b + c0 + (Pb > 0 ? Pa + lane % Pb : lane % Pa)];
                 if constexpr (F64) {
                     A.px = q.x; A.py = q.y; A.pz = q.z;
+                    B.px = r.x; B.py = r.y; B.pz = r.z;
                 } else {
                     A.px = float(q.x - bx); A.py = float(q.y - by); A.pz = float(q.z - bz);
+                    B.px = float(r.x - bx); B.py = float(r.y - by); B.pz = float(r.z - bz);
                 }
             }
+            auto eval_list = [&](int len) {
+                if (evla) A.flush(s_list, len, sla, Sa, eps2r);
+                if (evlb) B.flush(s_list, len, slb, Sb, eps2r);
+            };
+            auto eval_quad = [&](int len) {
+                if constexpr (QUAD) {
+                    if (evla) A.flush_quad(s_qe, s_qa, s_qb, len, sla, Sa, eps2r);
+                    if (evlb) B.flush_quad(s_qe, s_qa, s_qb, len, slb, Sb, eps2r);
+                }
+            };
             bool chunk_act = false;
-            if (lane < P) chunk_act = a.active ? a.active[a.idx[ip]] != 0 : true;
+            if (lane < P) chunk_act = a.active ? a.active[a.idx[b + c0 + lane]] != 0 : true;
             const int n_act = __popcll(__ballot(chunk_act));
             if (n_act == 0) continue;
 
@@ -361,7 +399,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                         if (nquad > 0) {
                             if (nq + nquad > QC) {
                                 __syncthreads();
-                                if (evl) A.flush_quad(s_qe, s_qa, s_qb, nq, sl, S, eps2r);
+                                eval_quad(nq);
                                 total_len += nq;
                                 nq = 0;
                                 __syncthreads();
@@ -379,7 +417,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                         if (nsm > 0) {
                             if (ln + nsm > LC) {
                                 __syncthreads();
-                                if (evl) A.flush(s_list, ln, sl, S, eps2r);
+                                eval_list(ln);
                                 total_len += ln;
                                 ln = 0;
                                 __syncthreads();
@@ -395,7 +433,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                         total_nodes += nmono;
                         if (ln + nmono > LC) {
                             __syncthreads();
-                            if (evl) A.flush(s_list, ln, sl, S, eps2r);
+                            eval_list(ln);
                             total_len += ln;
                             ln = 0;
                             __syncthreads();
@@ -418,7 +456,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 for (int e0 = 0; e0 < ptot2;) {
                     if (ln == LC) {
                         __syncthreads();
-                        if (evl) A.flush(s_list, ln, sl, S, eps2r);
+                        eval_list(ln);
                         total_len += ln;
                         ln = 0;
                         __syncthreads();
@@ -458,37 +496,41 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
             }
             if (ln > 0) {
                 __syncthreads();
-                if (evl) A.flush(s_list, ln, sl, S, eps2r);
+                eval_list(ln);
                 total_len += ln;
                 ln = 0;
             }
             if constexpr (QUAD) {
                 if (nq > 0) {
                     __syncthreads();
-                    if (evl) A.flush_quad(s_qe, s_qa, s_qb, nq, sl, S, eps2r);
+                    eval_quad(nq);
                     total_len += nq;
                     nq = 0;
                 }
             }
-            // reduce the slices of each particle in FP64
-            double3 v = make_double3(0, 0, 0);
-            if (evl) v = A.total();
-            __syncthreads();
-            s_red[0][lane] = v.x;
-            s_red[1][lane] = v.y;
-            s_red[2][lane] = v.z;
-            __syncthreads();
-            if (lane < P && chunk_act) {
-                double sx = 0, sy = 0, sz = 0;
-                for (int q = 0; q < S; ++q) {
-                    sx += s_red[0][q * P + lane];
-                    sy += s_red[1][q * P + lane];
-                    sz += s_red[2][q * P + lane];
+            // reduce the slices of each particle in FP64, part A then part B
+            const bool act_b = __shfl(chunk_act, Pa + lane, 64);
+            for (int part = 0; part < (Pb > 0 ? 2 : 1); ++part) {
+                const int Pp = part ? Pb : Pa, Sp = part ? Sb : Sa, p0 = part ? Pa : 0;
+                double3 v = make_double3(0, 0, 0);
+                if (part ? evlb : evla) v = part ? B.total() : A.total();
+                __syncthreads();
+                s_red[0][lane] = v.x;
+                s_red[1][lane] = v.y;
+                s_red[2][lane] = v.z;
+                __syncthreads();
+                if (lane < Pp && (part ? act_b : chunk_act)) {
+                    double sx = 0, sy = 0, sz = 0;
+                    for (int q = 0; q < Sp; ++q) {
+                        sx += s_red[0][q * Pp + lane];
+                        sy += s_red[1][q * Pp + lane];
+                        sz += s_red[2][q * Pp + lane];
+                    }
+                    const size_t o = 3 * size_t(a.idx[b + c0 + p0 + lane]);
+                    a.acc[o] = sx;
+                    a.acc[o + 1] = sy;
+                    a.acc[o + 2] = sz;
                 }
-                const size_t o = 3 * size_t(a.idx[ip]);
-                a.acc[o] = sx;
-                a.acc[o + 1] = sy;
-                a.acc[o + 2] = sz;
             }
             __syncthreads();
             if (lane == 0) {
