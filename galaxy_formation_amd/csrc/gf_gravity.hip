@@ -24,6 +24,7 @@
 // second launch whose stacks live in global memory (bounded by 512 entries
 // per tree level: the LIFO keeps entries sorted by depth, and all entries of
 // one depth were pushed by one batch of 64 nodes with at most 8 children each).
+#include <cstddef>
 #include <type_traits>
 
 #include "gf_gravity.h"
@@ -67,6 +68,12 @@ struct GNode {
     float qpad1;
 };
 static_assert(sizeof(GNode) == 128, "GNode layout");
+static_assert(offsetof(GNode, open_r) == 32 && offsetof(GNode, start) == 40 &&
+                  offsetof(GNode, leaf) == 48 && offsetof(GNode, c_lo) == 64 &&
+                  offsetof(GNode, c_hi) == 80 && offsetof(GNode, q) == 96 &&
+                  offsetof(GNode, qrel) == 120,
+              "the walk reads GNode as eight 16-byte words");
+typedef int v4i __attribute__((ext_vector_type(4)));
 
 struct WalkArgs {
     const GNode* gn;
@@ -337,13 +344,29 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 bool mono = false, small_q = false;
                 int4 c_lo = make_int4(-1, -1, -1, -1), c_hi = c_lo;
                 double4 cm = make_double4(0, 0, 0, 0);
+                v4i rq03 = {0, 0, 0, 0}, rq45 = {0, 0, 0, 0};   // QUAD: q[0..3]; q[4], q[5], qrel
                 if (node >= 0) {
-                    // the whole record in one round of loads
-                    const GNode* g = a.gn + node;
-                    cm = g->com;
-                    const double orr = g->open_r;
-                    const int gs = g->start, gc = g->count, gl = g->leaf;
-                    const int4 k_lo = g->c_lo, k_hi = g->c_hi;
+                    // the whole 128-byte record in one round of loads, held
+                    // in registers by the empty asm: the compiler would
+                    // otherwise move the loads of the leaf flag, children
+                    // and quadrupole into the branches that use them, one
+                    // dependent round trip each
+                    const v4i* r = reinterpret_cast<const v4i*>(a.gn + node);
+                    v4i w0 = r[0], w1 = r[1], w2 = r[2], w3 = r[3], w4 = r[4], w5 = r[5];
+                    if constexpr (QUAD) {
+                        rq03 = r[6];
+                        rq45 = r[7];
+                        asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5),
+                                     "+v"(rq03), "+v"(rq45));
+                    } else {
+                        asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5));
+                    }
+                    cm = make_double4(__hiloint2double(w0.y, w0.x), __hiloint2double(w0.w, w0.z),
+                                      __hiloint2double(w1.y, w1.x), __hiloint2double(w1.w, w1.z));
+                    const double orr = __hiloint2double(w2.y, w2.x);
+                    const int gs = w2.z, gc = w2.w, gl = w3.x;
+                    const int4 k_lo = make_int4(w4.x, w4.y, w4.z, w4.w);
+                    const int4 k_hi = make_int4(w5.x, w5.y, w5.z, w5.w);
                     const double dx = cm.x - bx, dy = cm.y - by, dz = cm.z - bz;
                     const double t = orr + br;
                     const double d2 = dx * dx + dy * dy + dz * dz;
@@ -354,7 +377,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                             // the quadrupole term's share at the group's
                             // nearest point (an estimate: FP32 is enough)
                             const float dm = sqrtf(float(d2)) - float(br);
-                            small_q = g->qrel < a.quad_tol * (dm * dm);
+                            small_q = __int_as_float(rq45.z) < a.quad_tol * (dm * dm);
                         }
                     } else if (gl) {
                         first = gs;
@@ -406,11 +429,12 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                             }
                             if (quad) {
                                 const int slot = nq + __popcll(qmask & gf_lanemask_lt());
-                                const GNode* g = a.gn + node;
                                 s_qe[slot] = make_float4(float(cm.x - bx), float(cm.y - by),
                                                          float(cm.z - bz), float(cm.w));
-                                s_qa[slot] = make_float4(g->q[0], g->q[1], g->q[2], g->q[3]);
-                                s_qb[slot] = make_float4(g->q[4], g->q[5], 0.0f, 0.0f);
+                                s_qa[slot] = make_float4(__int_as_float(rq03.x), __int_as_float(rq03.y),
+                                                         __int_as_float(rq03.z), __int_as_float(rq03.w));
+                                s_qb[slot] = make_float4(__int_as_float(rq45.x), __int_as_float(rq45.y),
+                                                         0.0f, 0.0f);
                             }
                             nq += nquad;
                         }
