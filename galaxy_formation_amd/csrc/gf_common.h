@@ -37,14 +37,18 @@ __device__ __forceinline__ uint64_t gf_lanemask_lt() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
-// Inclusive prefix sum over the 64 lanes of a wave.
+// Inclusive prefix sum over the 64 lanes of a wave, by DPP moves (no LDS):
+// row_shr 1, 2, 4, 8 scan each row of 16 lanes (a source before the row
+// start reads 0), then row_bcast:15 adds lane 15's total to row 1 and lane
+// 47's to row 3, and row_bcast:31 adds lane 31's (rows 0-1) to rows 2 and 3.
+// Every lane of the wave must be active.
 __device__ __forceinline__ int gf_wave_incl_scan(int v) {
-    const int l = __lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int t = __shfl_up(v, o, 64);
-        if (l >= o) v += t;
-    }
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
     return v;
 }
 

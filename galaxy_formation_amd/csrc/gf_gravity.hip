@@ -60,18 +60,18 @@ struct GNode {
     double4 com;          // centre of mass, mass (FP64)
     double open_r;        // opening radius (FP64)
     int start, count;     // particle range in tree order
-    int leaf, pad0, pad1, pad2;
-    int4 c_lo, c_hi;      // children in octant order, -1 = empty
+    int leaf, nchild, pad1, pad2;
+    int4 c_lo, c_hi;      // the nchild non-empty children in octant order, then -1
     float q[6];           // traceless quadrupole (xx, yy, zz, xy, xz, yz), FP32
-    float qrel;           // 3 max |Q_ij| / m: the quadrupole term's share of the
-                          // monopole at distance d is at most about qrel / d^2
+    float qlen;           // sqrt(3 max |Q_ij| / m): the quadrupole term's share of
+                          // the monopole at distance d is at most about (qlen / d)^2
     float qpad1;
 };
 static_assert(sizeof(GNode) == 128, "GNode layout");
 static_assert(offsetof(GNode, open_r) == 32 && offsetof(GNode, start) == 40 &&
-                  offsetof(GNode, leaf) == 48 && offsetof(GNode, c_lo) == 64 &&
-                  offsetof(GNode, c_hi) == 80 && offsetof(GNode, q) == 96 &&
-                  offsetof(GNode, qrel) == 120,
+                  offsetof(GNode, leaf) == 48 && offsetof(GNode, nchild) == 52 &&
+                  offsetof(GNode, c_lo) == 64 && offsetof(GNode, c_hi) == 80 &&
+                  offsetof(GNode, q) == 96 && offsetof(GNode, qlen) == 120,
               "the walk reads GNode as eight 16-byte words");
 typedef int v4i __attribute__((ext_vector_type(4)));
 
@@ -101,8 +101,9 @@ struct WalkArgs {
     int* gstack;
     int lds_scap;              // LDS stack entries used by the first launch (<= SCAP)
     unsigned long long* inter_nodes;   // the node (monopole or quadrupole) share of inter
-    float quad_tol;            // QUAD: a node whose quadrupole term is below this share of
-                               // its monopole at the group's nearest point is a monopole
+    float quad_rtol;           // QUAD: 1 / sqrt(tolerance), +inf for 0: a node whose
+                               // quadrupole term is below the tolerance's share of its
+                               // monopole at the group's nearest point is a monopole
 };
 
 __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v) {
@@ -340,68 +341,57 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 sn -= take;
                 const int node = lane < take ? stack[sn + lane] : -1;
                 __syncthreads();
-                int push = 0, nent = 0, first = 0;
-                bool mono = false, small_q = false;
-                int4 c_lo = make_int4(-1, -1, -1, -1), c_hi = c_lo;
-                double4 cm = make_double4(0, 0, 0, 0);
-                v4i rq03 = {0, 0, 0, 0}, rq45 = {0, 0, 0, 0};   // QUAD: q[0..3]; q[4], q[5], qrel
-                if (node >= 0) {
-                    // the whole 128-byte record in one round of loads, held
-                    // in registers by the empty asm: the compiler would
-                    // otherwise move the loads of the leaf flag, children
-                    // and quadrupole into the branches that use them, one
-                    // dependent round trip each
-                    const v4i* r = reinterpret_cast<const v4i*>(a.gn + node);
-                    v4i w0 = r[0], w1 = r[1], w2 = r[2], w3 = r[3], w4 = r[4], w5 = r[5];
-                    if constexpr (QUAD) {
-                        rq03 = r[6];
-                        rq45 = r[7];
-                        asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5),
-                                     "+v"(rq03), "+v"(rq45));
-                    } else {
-                        asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5));
-                    }
-                    cm = make_double4(__hiloint2double(w0.y, w0.x), __hiloint2double(w0.w, w0.z),
-                                      __hiloint2double(w1.y, w1.x), __hiloint2double(w1.w, w1.z));
-                    const double orr = __hiloint2double(w2.y, w2.x);
-                    const int gs = w2.z, gc = w2.w, gl = w3.x;
-                    const int4 k_lo = make_int4(w4.x, w4.y, w4.z, w4.w);
-                    const int4 k_hi = make_int4(w5.x, w5.y, w5.z, w5.w);
-                    const double dx = cm.x - bx, dy = cm.y - by, dz = cm.z - bz;
-                    const double t = orr + br;
-                    const double d2 = dx * dx + dy * dy + dz * dz;
-                    if (d2 > t * t) {
-                        mono = true;
-                        nent = 1;
-                        if constexpr (QUAD) {
-                            // the quadrupole term's share at the group's
-                            // nearest point (an estimate: FP32 is enough)
-                            const float dm = sqrtf(float(d2)) - float(br);
-                            small_q = __int_as_float(rq45.z) < a.quad_tol * (dm * dm);
-                        }
-                    } else if (gl) {
-                        first = gs;
-                        nent = gc;
-                    } else {
-                        c_lo = k_lo;
-                        c_hi = k_hi;
-                        push = (c_lo.x >= 0) + (c_lo.y >= 0) + (c_lo.z >= 0) + (c_lo.w >= 0) +
-                               (c_hi.x >= 0) + (c_hi.y >= 0) + (c_hi.z >= 0) + (c_hi.w >= 0);
-                    }
+                // every lane loads a record (the root where it has no node),
+                // so the visit is free of branches; the empty asm keeps the
+                // eight loads together, where the compiler would otherwise
+                // move the leaf flag, children and quadrupole into the paths
+                // that use them, one dependent round trip each
+                const bool valid = node >= 0;
+                const v4i* r = reinterpret_cast<const v4i*>(a.gn + (valid ? node : 0));
+                v4i w0 = r[0], w1 = r[1], w2 = r[2], w3 = r[3], w4 = r[4], w5 = r[5];
+                v4i rq03 = {0, 0, 0, 0}, rq45 = {0, 0, 0, 0};   // QUAD: q[0..3]; q[4], q[5], qlen
+                if constexpr (QUAD) {
+                    rq03 = r[6];
+                    rq45 = r[7];
+                    asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5),
+                                 "+v"(rq03), "+v"(rq45));
+                } else {
+                    asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5));
+                }
+                const double4 cm = make_double4(__hiloint2double(w0.y, w0.x), __hiloint2double(w0.w, w0.z),
+                                                __hiloint2double(w1.y, w1.x), __hiloint2double(w1.w, w1.z));
+                const double orr = __hiloint2double(w2.y, w2.x);
+                const double dx = cm.x - bx, dy = cm.y - by, dz = cm.z - bz;
+                const double t = orr + br;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                const bool far = d2 > t * t;
+                const bool mono = valid && far;                 // accepted: one list entry
+                const bool opened = valid && !far;
+                const int first = w2.z;
+                const int pc = opened && w3.x ? w2.w : 0;      // an opened leaf's particles
+                const int push = opened && !w3.x ? w3.y : 0;   // an opened node's children
+                bool small_q = false;
+                if constexpr (QUAD) {
+                    // the quadrupole term's share at the group's nearest
+                    // point, about (qlen / (d - br))^2, is below the
+                    // tolerance iff qlen / sqrt(tolerance) + br < d (an
+                    // estimate: FP32 is enough)
+                    const float x = fmaf(__int_as_float(rq45.z), a.quad_rtol, float(br));
+                    small_q = x * x < float(d2);
                 }
                 // push children (lane order keeps the stack sorted by depth)
                 const int pinc = gf_wave_incl_scan(push);
-                const int ptot = __shfl(pinc, 63, 64);
+                const int ptot = __builtin_amdgcn_readlane(pinc, 63);
                 if (sn + ptot > scap) {
                     overflow = true;
                     break;
                 }
                 {
-                    int o = sn + pinc - push;
-                    const int cc[8] = {c_lo.x, c_lo.y, c_lo.z, c_lo.w, c_hi.x, c_hi.y, c_hi.z, c_hi.w};
+                    const int o = sn + pinc - push;
+                    const int cc[8] = {w4.x, w4.y, w4.z, w4.w, w5.x, w5.y, w5.z, w5.w};
 #pragma unroll
                     for (int k = 0; k < 8; ++k)
-                        if (cc[k] >= 0) stack[o++] = cc[k];
+                        if (k < push) stack[o + k] = cc[k];
                 }
                 sn += ptot;
                 // append this batch's entries, evaluating the list whenever it
@@ -474,9 +464,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                         ln += nmono;
                     }
                 }
-                const int pc = mono ? 0 : nent;
                 const int qinc = gf_wave_incl_scan(pc);
-                const int ptot2 = __shfl(qinc, 63, 64);
+                const int ptot2 = __builtin_amdgcn_readlane(qinc, 63);
                 for (int e0 = 0; e0 < ptot2;) {
                     if (ln == LC) {
                         __syncthreads();
@@ -582,17 +571,24 @@ __global__ __launch_bounds__(256) void k_gnode_pack(const int* __restrict__ coun
         g.start = start[id];
         g.count = count[id];
         g.leaf = leaf[id];
-        g.pad0 = g.pad1 = g.pad2 = 0;
-        const int4* cs = reinterpret_cast<const int4*>(child8 + 8 * size_t(id));
-        g.c_lo = cs[0];
-        g.c_hi = cs[1];
+        g.pad1 = g.pad2 = 0;
+        // the non-empty children first, in octant order
+        int cc[8], nc = 0;
+        for (int k = 0; k < 8; ++k) cc[k] = -1;
+        for (int k = 0; k < 8; ++k) {
+            const int c = child8[8 * size_t(id) + k];
+            if (c >= 0) cc[nc++] = c;
+        }
+        g.nchild = nc;
+        g.c_lo = make_int4(cc[0], cc[1], cc[2], cc[3]);
+        g.c_hi = make_int4(cc[4], cc[5], cc[6], cc[7]);
         double qmax = 0.0;
         for (int k = 0; k < 6; ++k) {
             const double qk = quad ? quad[6 * size_t(id) + k] : 0.0;
             g.q[k] = float(qk);
             qmax = fmax(qmax, fabs(qk));
         }
-        g.qrel = g.com.w > 0 ? float(3.0 * qmax / g.com.w) : 0.0f;
+        g.qlen = g.com.w > 0 ? float(sqrt(3.0 * qmax / g.com.w)) : 0.0f;
         g.qpad1 = 0.0f;
         gn[id] = g;
     }
@@ -801,7 +797,7 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
     a.gstack = W.gstack;
     a.inter_nodes = W.inter_nodes;
     a.lds_scap = gf_test_limits().walk_stack > 0 ? std::min(SCAP, gf_test_limits().walk_stack) : SCAP;
-    a.quad_tol = W.quad_tol;
+    a.quad_rtol = W.quad_tol > 0 ? float(1.0 / std::sqrt(double(W.quad_tol))) : INFINITY;
     hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, W.work, W.ovf_count);
     if (precision == GF_FP64) {
         hipLaunchKernelGGL((k_walk<true, false, false>), dim3(WALK_BLOCKS), dim3(64), 0, s, a);
