@@ -58,16 +58,39 @@ __device__ __forceinline__ int gf_wave_sum_i(int v) {
     return v;
 }
 
+// A 64-bit value rotated within each row of 16 lanes by a DPP row_ror.
+template <int CTRL>
+__device__ __forceinline__ double gf_dpp_ror_d(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double gf_readlane_d(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// Minimum and maximum over the 64 lanes (every lane gets the result), by
+// DPP: row_ror 8, 4, 2, 1 leave each lane with its row's extreme, and the
+// four rows are combined from lanes 0, 16, 32, 48.  The order of the
+// comparisons does not change the result.  Every lane must be active.
 __device__ __forceinline__ double gf_wave_min_d(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
+    v = fmin(v, gf_dpp_ror_d<0x128>(v));
+    v = fmin(v, gf_dpp_ror_d<0x124>(v));
+    v = fmin(v, gf_dpp_ror_d<0x122>(v));
+    v = fmin(v, gf_dpp_ror_d<0x121>(v));
+    return fmin(fmin(gf_readlane_d(v, 0), gf_readlane_d(v, 16)),
+                fmin(gf_readlane_d(v, 32), gf_readlane_d(v, 48)));
 }
 
 __device__ __forceinline__ double gf_wave_max_d(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
+    v = fmax(v, gf_dpp_ror_d<0x128>(v));
+    v = fmax(v, gf_dpp_ror_d<0x124>(v));
+    v = fmax(v, gf_dpp_ror_d<0x122>(v));
+    v = fmax(v, gf_dpp_ror_d<0x121>(v));
+    return fmax(fmax(gf_readlane_d(v, 0), gf_readlane_d(v, 16)),
+                fmax(gf_readlane_d(v, 32), gf_readlane_d(v, 48)));
 }
 
 __device__ __forceinline__ double gf_wave_sum_d(double v) {

@@ -223,10 +223,10 @@ __device__ int q_gather(const QueryArgs& a, double qx, double qy, double qz, dou
             for (int k = 0; k < 8; ++k)
                 if (cc[k] >= 0) stack[o++] = cc[k];
         }
-        sn += __shfl(pinc, 63, 64);
+        sn += __builtin_amdgcn_readlane(pinc, 63);
         // the opened leaves' sources, flattened over the lanes
         const int qinc = gf_wave_incl_scan(nent);
-        const int tot = __shfl(qinc, 63, 64);
+        const int tot = __builtin_amdgcn_readlane(qinc, 63);
         for (int e0 = 0; e0 < tot; e0 += 64) {
             const int e = e0 + lane;
             int own = 0;

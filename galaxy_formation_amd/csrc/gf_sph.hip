@@ -241,7 +241,7 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
                     }
                 }
                 const int pinc = gf_wave_incl_scan(push);
-                const int ptot = __shfl(pinc, 63, 64);
+                const int ptot = __builtin_amdgcn_readlane(pinc, 63);
                 if (sn + ptot > scap) {
                     overflow = true;
                     break;
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
                 int rem = nent, src = 0;
                 while (true) {
                     const int einc = gf_wave_incl_scan(rem);
-                    const int etot = __shfl(einc, 63, 64);
+                    const int etot = __builtin_amdgcn_readlane(einc, 63);
                     if (etot == 0) break;
                     if (cn == CH) flush();
                     const int space = CH - cn;
@@ -482,7 +482,7 @@ __device__ __forceinline__ int push_children(const NodeHit& h, int* stack, int s
 #pragma unroll
     for (int k = 0; k < 8; ++k)
         if (cc[k] >= 0) stack[o++] = cc[k];
-    return __shfl(pinc, 63, 64);
+    return __builtin_amdgcn_readlane(pinc, 63);
 }
 
 // The whole block gathers the particles within (b, R) of the subtree of
@@ -519,7 +519,7 @@ __device__ int gather_group(const KnnArgs& a, int root, double bx, double by, do
             }
             const uint64_t lm = __ballot(h.cnt > 0);
             const int einc = gf_wave_incl_scan(h.cnt);
-            const int ebatch = __shfl(einc, 63, 64);
+            const int ebatch = __builtin_amdgcn_readlane(einc, 63);
             const int nlb = __popcll(lm);
             // ranges touched at most CMAX, particles in them at most 4 CMAX
             if (nl + nlb > CMAX || et + ebatch > 4 * CMAX) {
@@ -678,7 +678,7 @@ __device__ int select_member(const KnnArgs& a, int t, double xi, double yi, doub
         const int sum = h[0] + h[1] + h[2] + h[3];
         const int incl = gf_wave_incl_scan(sum);
         if (shift == 24) {
-            const int total = __shfl(incl, 63, 64);
+            const int total = __builtin_amdgcn_readlane(incl, 63);
             if (total < kq) {
                 if (lane == 0) {
                     a.passes[t] = npass + 1;
@@ -709,9 +709,9 @@ __device__ int select_member(const KnnArgs& a, int t, double xi, double yi, doub
             }
         }
         const int src = __ffsll((unsigned long long)__ballot(here)) - 1;
-        bsel = __shfl(bsel, src, 64);
-        need = __shfl(nneed, src, 64);
-        bcount = __shfl(bc, src, 64);
+        bsel = __builtin_amdgcn_readlane(bsel, src);
+        need = __builtin_amdgcn_readlane(nneed, src);
+        bcount = __builtin_amdgcn_readlane(bc, src);
         prefix |= uint64_t(bsel) << shift;
         __builtin_amdgcn_wave_barrier();
         if (bcount <= 64 || shift == 0) break;
@@ -901,9 +901,9 @@ __global__ __launch_bounds__(FT, Shape::BLOCKS_PER_CU) void k_knn_fast(KnnArgs a
                 for (int mm = 0; mm < np; ++mm) {
                     if (((rg >> mm) & 1) == 0) continue;
                     if ((rank_in_r++ & 3) != w) continue;
-                    const double xi = __shfl(xm, mm, 64), yi = __shfl(ym, mm, 64);
-                    const double zi = __shfl(zm, mm, 64), ri = __shfl(rm, mm, 64);
-                    const int npass = __shfl(pm_, mm, 64);
+                    const double xi = gf_readlane_d(xm, mm), yi = gf_readlane_d(ym, mm);
+                    const double zi = gf_readlane_d(zm, mm), ri = gf_readlane_d(rm, mm);
+                    const int npass = __builtin_amdgcn_readlane(pm_, mm);
                     const int res = select_member<CMAX>(a, b0 + mm, xi, yi, zi, ri, npass, cx, cy, cz,
                                                   ci, nc, hist[w], tie_d[w], tie_i[w], tie_p[w]);
                     if (res < 0) handoff = 1;
@@ -960,8 +960,8 @@ __global__ __launch_bounds__(FT, Shape::BLOCKS_PER_CU) void k_knn_fast(KnnArgs a
                 // the heap pass
                 const int mm = __ffsll((unsigned long long)rg) - 1;
                 const int t = b0 + mm;
-                const double ri = __shfl(rm, mm, 64);
-                const int npass = __shfl(pm_, mm, 64);
+                const double ri = gf_readlane_d(rm, mm);
+                const int npass = __builtin_amdgcn_readlane(pm_, mm);
                 const int seen = ctl[7];
                 if (threadIdx.x == 0) {
                     const double lo = a.r_lo[t];
