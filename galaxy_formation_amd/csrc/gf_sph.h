@@ -21,6 +21,7 @@ struct GfSph {
     int *passes = nullptr;
     int *inv = nullptr;          // input index -> tree position
     int *leaf_of = nullptr;      // leaf node of each tree position
+    int *anc = nullptr;          // per group: its node and ancestors, GF_ANC ints (-1 past the root)
     int *nb = nullptr;           // [k][n] neighbours: tree positions (or input indices)
     int *state = nullptr;        // pass counters (layout in gf_sph.hip)
     int *ovf = nullptr, *ovf2 = nullptr;

@@ -30,6 +30,7 @@ constexpr int KSCAP = 1024;                             // LDS stack entries
 constexpr int KGSCAP = 512 * (GF_KEY_LEVELS + 2);       // global stack entries per block
 constexpr int KNN_BLOCKS = 4096;
 constexpr int KNN_GST_BLOCKS = 128;
+constexpr int GF_ANC = 32;                             // ancestors held per group (depth <= 21)
 
 // A gas-tree node packed for the neighbour walks (64 bytes): the box, then
 // the 8 children in octant order (-1 = empty), or, for a node of at most
@@ -59,6 +60,7 @@ struct KnnArgs {
     const int* inv;
     const int* parent_;
     const int* leaf_of;
+    const int* anc;         // per group position: the group node and its ancestors
     double* r;
     double* r_lo;   // largest radius known to hold fewer than kq particles (0 = none)
     double* r_hi;   // smallest radius known to overflow the member list (inf = none)
@@ -411,6 +413,28 @@ __device__ int containing_node(const KnnArgs& a, int node, double bx, double by,
             return node;
         node = p;
     }
+}
+
+// containing_node from the group node, from the table of its ancestors: lane
+// k tests ancestor k (one round of loads instead of a chain), and the
+// deepest one that contains the sphere, or the root, is the answer (the
+// boxes grow towards the root, so the tests pass from some depth up).
+__device__ int group_containing_node(const KnnArgs& a, int gp, double bx, double by, double bz,
+                                     double R) {
+#pragma clang fp contract(off)
+    const int lane = __lane_id();
+    const int node = lane < GF_ANC ? a.anc[size_t(gp) * GF_ANC + lane] : -1;
+    bool ok = false;
+    if (node >= 0) {
+        const double4 bb = a.box[node];
+        const double lim = bb.w * (1.0 - 1e-12);
+        ok = fabs(bx - bb.x) + R < lim && fabs(by - bb.y) + R < lim && fabs(bz - bb.z) + R < lim;
+    }
+    const uint64_t okm = __ballot(ok), valid = __ballot(node >= 0);
+    const int last = 63 - __clzll(valid);
+    if (okm == 0 && last == GF_ANC - 1)      // a chain longer than the table: walk on
+        return containing_node(a, __builtin_amdgcn_readlane(node, last), bx, by, bz, R);
+    return __builtin_amdgcn_readlane(node, okm ? __ffsll((unsigned long long)okm) - 1 : last);
 }
 
 // One lane's visit of a packed node against the sphere (b, R2): a range
@@ -887,9 +911,10 @@ __global__ __launch_bounds__(FT, Shape::BLOCKS_PER_CU) void k_knn_fast(KnnArgs a
                     msph[__popcll(rg & gf_lanemask_lt())] = make_double4(xm, ym, zm, rr * rr);
                 }
                 __syncthreads();
-                const int root = containing_node(
-                    a, nr == 1 ? a.leaf_of[b0 + __ffsll((unsigned long long)rg) - 1] : G, S.bx,
-                    S.by, S.bz, S.R);
+                const int root =
+                    nr == 1 ? containing_node(a, a.leaf_of[b0 + __ffsll((unsigned long long)rg) - 1],
+                                              S.bx, S.by, S.bz, S.R)
+                            : group_containing_node(a, gp, S.bx, S.by, S.bz, S.R);
                 nc = gather_group<CMAX>(a, root, S.bx, S.by, S.bz, S.R * S.R, msph, nm, cx, cy, cz, ci,
                                   cap, stack, lf, lincl, ctl);
             }
@@ -1059,6 +1084,21 @@ __global__ __launch_bounds__(TB) void k_knn_init(int n, int kq, const double* __
     r_hi[t] = INFINITY;
     done[t] = 0;
     passes[t] = 0;
+}
+
+// The node of every group and its ancestors, GF_ANC per group (-1 past the root).
+__global__ __launch_bounds__(TB) void k_group_anc(const int* __restrict__ counters,
+                                                  const int* __restrict__ groups,
+                                                  const int* __restrict__ parent,
+                                                  int* __restrict__ anc) {
+    const int ng = counters[1];
+    for (int g = blockIdx.x * TB + threadIdx.x; g < ng; g += gridDim.x * TB) {
+        int node = groups[g];
+        for (int k = 0; k < GF_ANC; ++k) {
+            anc[size_t(g) * GF_ANC + k] = node;
+            node = node >= 0 ? parent[node] : -1;
+        }
+    }
 }
 
 __global__ __launch_bounds__(TB) void k_leaf_of(const int* __restrict__ counters,
@@ -1423,7 +1463,7 @@ int sph_reserve(GfSph& S, int n, int k) {
     const int ng = gf_tree_node_bound(S.cap_n);
     if (ng > S.cap_groups) {
         if (dalloc(S.ovf, ng) || dalloc(S.ovf2, ng) || dalloc(S.retry, 2 * size_t(ng)) ||
-            dalloc(S.knode, 8 * size_t(ng)))
+            dalloc(S.knode, 8 * size_t(ng)) || dalloc(S.anc, size_t(GF_ANC) * size_t(ng)))
             return -1;
         S.cap_groups = ng;
     }
@@ -1436,6 +1476,7 @@ void gf_sph_free(GfSph& S) {
     gf_tree_free(S.tree);
     dfree(S.vx); dfree(S.vy); dfree(S.vz); dfree(S.u); dfree(S.r); dfree(S.dk); dfree(S.h);
     dfree(S.rho); dfree(S.done); dfree(S.passes); dfree(S.inv); dfree(S.leaf_of); dfree(S.r_lo); dfree(S.r_hi); dfree(S.rec); dfree(S.p4); dfree(S.gmark); dfree(S.retry); dfree(S.knode);
+    dfree(S.anc);
     dfree(S.nb); dfree(S.state); dfree(S.ovf); dfree(S.ovf2); dfree(S.gstack);
     S = GfSph();
 }
@@ -1468,6 +1509,8 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
     // leaf of each particle: first radius and start of the member walks
     hipLaunchKernelGGL(k_leaf_of, dim3(std::min(2048, gf_div_up(T.cap_nodes, TB))), dim3(TB), 0, s,
                        T.counters, T.start, T.count, T.leaf, S.leaf_of);
+    hipLaunchKernelGGL(k_group_anc, dim3(std::min(2048, gf_div_up(T.cap_nodes, TB))), dim3(TB), 0,
+                       s, T.counters, T.groups, T.parent, S.anc);
     hipLaunchKernelGGL(k_knn_init, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, kq, r_init, T.idx,
                        S.leaf_of, T.parent, T.count, T.box, S.r, S.r_lo, S.r_hi, S.done,
                        S.passes, S.gmark);
@@ -1486,6 +1529,7 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
     a.start = T.start; a.count = T.count; a.leaf = T.leaf; a.child8 = T.child8; a.box = T.box;
     a.groups = T.groups; a.counters = T.counters; a.tx = T.tx; a.ty = T.ty; a.tz = T.tz;
     a.idx = T.idx; a.inv = S.inv; a.parent_ = T.parent; a.leaf_of = S.leaf_of; a.r = S.r;
+    a.anc = S.anc;
     a.r_lo = S.r_lo; a.r_hi = S.r_hi; a.dk = S.dk; a.done = S.done; a.passes = S.passes;
     a.nb = S.nb; a.n = n; a.kq = kq; a.tree_pos_out = tree_pos_out; a.sorted = sorted;
     a.state = S.state; a.ovf = S.ovf; a.ovf2 = S.ovf2; a.gstack = S.gstack;
