@@ -44,7 +44,7 @@ PEAK_FP64 = 78.6e12
 # 0 monopole, 2 quadrupole): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from
 # rocprofv3 --pmc (profiles/); counters cannot be read inside a run, so the
 # measured figures are carried here
-WALK_TRAFFIC_BYTES = {0: 3433121792, 2: 3625940992}
+WALK_TRAFFIC_BYTES = {0: 3527188960, 2: 3679273984}
 # the accuracy gates of the metric: RMS relative acceleration error vs the
 # FP64 direct sum, and density vs the plain h solve (with h equal to it)
 GATE_ACC_RMS = 1e-3
@@ -68,7 +68,43 @@ def parse():
     p.add_argument("--substeps", type=int, default=224)
     p.add_argument("--model", default=os.path.join(ROOT, "galaxy_formation_amd", "data",
                                                   "learned_h_k64.npz"))
+    p.add_argument("--walk-alone", type=int, default=None, metavar="PREC",
+                   help="profiling aid: build the workload's tree, then one warm-up and "
+                        "three walks of precision PREC (0 FP32, 1 FP64, 2 FP32 quadrupoles), "
+                        "nothing else; prints their mean duration")
+    p.add_argument("--knn-alone", action="store_true",
+                   help="profiling aid: the gas neighbour solve alone, three times from a "
+                        "first radius of 1.083 d_k (about the learned proposal's margin)")
     return p.parse_args()
+
+
+def knn_alone_only(a):
+    """The neighbour solve of the workload's gas and nothing else, for counter runs."""
+    B = Bench(a)
+    torch, L, chk = B.torch, B.L, B._lib.check
+    chk(L.gf_hydro_setup_dev(B.ng, B.p(B.pos), B.p(B.vel), B.p(B.mass), B.p(B.u), B.sp))
+    chk(L.gf_hydro_knn_dev(a.k, None, B.sp))
+    chk(L.gf_hydro_dk_dev(B.p(B.dk_prev), B.sp))
+    r0 = (B.dk_prev * 1.083).contiguous()
+    chk(L.gf_hydro_knn_dev(a.k, B.p(r0), B.sp))
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(B.stream)
+    passes = 0
+    for _ in range(3):
+        passes = chk(L.gf_hydro_knn_dev(a.k, B.p(r0), B.sp))
+    e1.record(B.stream)
+    torch.cuda.synchronize()
+    print(f"neighbour solve alone, {a.workload} gas, k = {a.k}: {e0.elapsed_time(e1) / 3:.3f} ms "
+          f"per solve ({passes} pass launches)", flush=True)
+
+
+def walk_alone_only(a):
+    """The gravity walk of the workload and nothing else, for counter runs."""
+    B = Bench(a)
+    ms = B.walk_alone(a.walk_alone, launches=3)
+    print(f"walk alone, precision {a.walk_alone}, {a.workload}: {ms:.3f} ms per launch "
+          f"(mean of 3 after one warm-up)", flush=True)
 
 
 class Bench:
@@ -283,6 +319,12 @@ def main():
         sys.exit(2)
     if a.steploop:
         steploop(a)
+        return
+    if a.walk_alone is not None:
+        walk_alone_only(a)
+        return
+    if a.knn_alone:
+        knn_alone_only(a)
         return
     B = Bench(a)
     torch, n, ng, c = B.torch, B.n, B.ng, B.c
