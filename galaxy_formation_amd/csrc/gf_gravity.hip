@@ -18,7 +18,7 @@
 //     particles, then the rest), each with its own slices.
 //   * FP32 evaluation works on coordinates relative to the group centre b,
 //     rounded to FP32 once per list entry and group; each flush's FP32 partial
-//     sum is added to a compensated (two-sum) FP32 accumulator.
+//     sum is added to an FP64 accumulator.
 //   * FP64 evaluation uses v_rsq_f64 refined by one Newton step.
 // A group whose LDS stack would overflow is queued and walked again by a
 // second launch whose stacks live in global memory (bounded by 512 entries
@@ -106,20 +106,12 @@ struct WalkArgs {
                                // monopole at the group's nearest point is a monopole
 };
 
-__device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v) {
-    const float s = hi + v;
-    const float bp = s - hi;
-    const float err = (hi - (s - bp)) + (v - bp);
-    hi = s;
-    lo += err;
-}
-
 template <bool F64>
 struct Acc;
 
 template <>
 struct Acc<false> {
-    float hx = 0, hy = 0, hz = 0, lx = 0, ly = 0, lz = 0;
+    double sx = 0, sy = 0, sz = 0;      // FP64 sum of the flushes' FP32 partial sums
     float px, py, pz;
     __device__ void flush(const float4* L, int ln, int sl, int S, float eps2) {
         float ax = 0, ay = 0, az = 0;
@@ -149,42 +141,48 @@ struct Acc<false> {
             ay = fmaf(f, dy, ay);
             az = fmaf(f, dz, az);
         }
-        two_sum_add(hx, lx, ax);
-        two_sum_add(hy, ly, ay);
-        two_sum_add(hz, lz, az);
+        sx += double(ax);
+        sy += double(ay);
+        sz += double(az);
     }
     // Node entries with quadrupoles: e = (x, y, z, m) relative to the group
     // centre, qa = (xx, yy, zz, xy), qb = (xz, yz, -, -).  With d = node -
     // particle and r^2 = |d|^2 + eps^2, the acceleration of the potential
     // -m/r - (1/2) d.Q.d / r^5 is  a = (m/r^3 + (5/2) d.Q.d / r^7) d - Q.d / r^5.
+    __device__ static void quad_term(const float4& e, const float4& qa, const float4& qb,
+                                     float px, float py, float pz, float eps2, float& ax,
+                                     float& ay, float& az) {
+        const float dx = e.x - px, dy = e.y - py, dz = e.z - pz;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float inv2 = inv * inv;
+        const float inv3 = inv * inv2;
+        const float inv5 = inv3 * inv2;
+        const float qx = fmaf(qa.x, dx, fmaf(qa.w, dy, qb.x * dz));
+        const float qy = fmaf(qa.w, dx, fmaf(qa.y, dy, qb.y * dz));
+        const float qz = fmaf(qb.x, dx, fmaf(qb.y, dy, qa.z * dz));
+        const float dqd = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
+        const float f = fmaf(2.5f * dqd, inv5 * inv2, e.w * inv3);
+        ax = fmaf(f, dx, fmaf(-inv5, qx, ax));
+        ay = fmaf(f, dy, fmaf(-inv5, qy, ay));
+        az = fmaf(f, dz, fmaf(-inv5, qz, az));
+    }
     __device__ void flush_quad(const float4* E, const float4* QA, const float4* QB, int ln, int sl,
                                int S, float eps2) {
         float ax = 0, ay = 0, az = 0;
-        for (int j = sl; j < ln; j += S) {
-            const float4 e = E[j], qa = QA[j], qb = QB[j];
-            const float dx = e.x - px, dy = e.y - py, dz = e.z - pz;
-            const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-            const float inv = __builtin_amdgcn_rsqf(r2);
-            const float inv2 = inv * inv;
-            const float inv3 = inv * inv2;
-            const float inv5 = inv3 * inv2;
-            const float qx = fmaf(qa.x, dx, fmaf(qa.w, dy, qb.x * dz));
-            const float qy = fmaf(qa.w, dx, fmaf(qa.y, dy, qb.y * dz));
-            const float qz = fmaf(qb.x, dx, fmaf(qb.y, dy, qa.z * dz));
-            const float dqd = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
-            const float f = fmaf(2.5f * dqd, inv5 * inv2, e.w * inv3);
-            ax = fmaf(f, dx, fmaf(-inv5, qx, ax));
-            ay = fmaf(f, dy, fmaf(-inv5, qy, ay));
-            az = fmaf(f, dz, fmaf(-inv5, qz, az));
+        int j = sl;
+        for (; j + S < ln; j += 2 * S) {
+            const float4 e0 = E[j], qa0 = QA[j], qb0 = QB[j];
+            const float4 e1 = E[j + S], qa1 = QA[j + S], qb1 = QB[j + S];
+            quad_term(e0, qa0, qb0, px, py, pz, eps2, ax, ay, az);
+            quad_term(e1, qa1, qb1, px, py, pz, eps2, ax, ay, az);
         }
-        two_sum_add(hx, lx, ax);
-        two_sum_add(hy, ly, ay);
-        two_sum_add(hz, lz, az);
+        if (j < ln) quad_term(E[j], QA[j], QB[j], px, py, pz, eps2, ax, ay, az);
+        sx += double(ax);
+        sy += double(ay);
+        sz += double(az);
     }
-    __device__ double3 total() const {
-        return make_double3(double(hx) + double(lx), double(hy) + double(ly),
-                            double(hz) + double(lz));
-    }
+    __device__ double3 total() const { return make_double3(sx, sy, sz); }
 };
 
 template <>
