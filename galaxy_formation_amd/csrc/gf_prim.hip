@@ -78,6 +78,75 @@ __global__ __launch_bounds__(1024) void k_scan_partials(int* partial, int nb) {
     if (t == 0) partial[nb] = tot;
 }
 
+// One block of 1024 threads scans up to SCAN_ONE_MAX inputs (exclusive) and
+// writes the total at out[n], in one launch.  Thread t holds items 4t..4t+3
+// of each chunk of 4096 (16-byte loads, coalesced across the block).  The
+// first round loads every chunk before the first wait and keeps only the
+// waves' totals; the second reads the chunks again (unchanged until the
+// stores) and writes the prefixes.  in and out must be 16-byte aligned and
+// may be the same array: every store follows the last barrier, and the
+// second round's loads of a thread's items precede its own stores to them.
+constexpr int SCAN_ONE_CHUNKS = 16;
+constexpr int SCAN_ONE_MAX = SCAN_ONE_CHUNKS * 4096;
+
+__device__ __forceinline__ int4 scan_one_load(const int* in, int n, int i) {
+    if (i + 3 < n) return *reinterpret_cast<const int4*>(in + i);
+    return make_int4(i < n ? in[i] : 0, i + 1 < n ? in[i + 1] : 0, i + 2 < n ? in[i + 2] : 0, 0);
+}
+
+__global__ __launch_bounds__(1024) void k_scan_one(const int* in, int n,
+                                                   const int* __restrict__ len, int* out) {
+    __shared__ int wtot[SCAN_ONE_CHUNKS][16];   // per chunk, per wave: the wave's total
+    __shared__ int woff[SCAN_ONE_CHUNKS][16];   // ... the prefix of the waves before it
+    __shared__ int cpre[SCAN_ONE_CHUNKS + 1];   // prefix of the chunk totals
+    n = scan_len(n, len);
+    const int t = threadIdx.x, lane = __lane_id(), w = t >> 6;
+    {
+        int4 v[SCAN_ONE_CHUNKS];
+#pragma unroll
+        for (int c = 0; c < SCAN_ONE_CHUNKS; ++c) v[c] = scan_one_load(in, n, c * 4096 + 4 * t);
+#pragma unroll
+        for (int c = 0; c < SCAN_ONE_CHUNKS; ++c) {
+            const int inc = gf_wave_incl_scan(v[c].x + v[c].y + v[c].z + v[c].w);
+            if (lane == 63) wtot[c][w] = inc;
+        }
+    }
+    __syncthreads();
+    if (t < SCAN_ONE_CHUNKS) {
+        int run = 0;
+        for (int q = 0; q < 16; ++q) {
+            woff[t][q] = run;
+            run += wtot[t][q];
+        }
+        cpre[t + 1] = run;                        // chunk t's total, summed below
+    }
+    __syncthreads();
+    if (t == 0) {
+        cpre[0] = 0;
+        for (int c = 1; c <= SCAN_ONE_CHUNKS; ++c) cpre[c] += cpre[c - 1];
+    }
+    __syncthreads();
+    int4 v[SCAN_ONE_CHUNKS];
+#pragma unroll
+    for (int c = 0; c < SCAN_ONE_CHUNKS; ++c) v[c] = scan_one_load(in, n, c * 4096 + 4 * t);
+#pragma unroll
+    for (int c = 0; c < SCAN_ONE_CHUNKS; ++c) {
+        const int i = c * 4096 + 4 * t;
+        const int sum = v[c].x + v[c].y + v[c].z + v[c].w;
+        const int base = cpre[c] + woff[c][w] + gf_wave_incl_scan(sum) - sum;
+        const int4 o = make_int4(base, base + v[c].x, base + v[c].x + v[c].y,
+                                 base + v[c].x + v[c].y + v[c].z);
+        if (i + 3 < n) {
+            *reinterpret_cast<int4*>(out + i) = o;
+        } else {
+            if (i < n) out[i] = o.x;
+            if (i + 1 < n) out[i + 1] = o.y;
+            if (i + 2 < n) out[i + 2] = o.z;
+        }
+    }
+    if (t == 0) out[n] = cpre[SCAN_ONE_CHUNKS];
+}
+
 // in and out may be the same array: every thread reads its items before the
 // block barrier and writes only those items after it.
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int* in, int n,
@@ -211,6 +280,11 @@ int gf_scan_excl(const int* in, int* out, int n, int* partial, hipStream_t s, co
         GF_HIP(hipMemsetAsync(out, 0, sizeof(int), s));
         return 0;
     }
+    if (n <= SCAN_ONE_MAX && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0) {
+        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, s, in, n, len, out);
+        GF_LAUNCH_CHECK();
+        return 0;
+    }
     const int nb = gf_div_up(n, SCAN_TILE);
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, len, partial);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, partial, nb);
@@ -238,12 +312,7 @@ int gf_radix_sort_pairs(uint64_t* key, uint32_t* val, uint64_t* key_alt, uint32_
     for (int shift = begin_bit; shift < end_bit; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, s, ka, n, shift, nb, hist);
         // exclusive scan of the bin-major histogram, in place
-        const int nbs = gf_div_up(nh, SCAN_TILE);
-        hipLaunchKernelGGL(k_scan_reduce, dim3(nbs), dim3(SCAN_THREADS), 0, s, hist, nh, nullptr,
-                           partial);
-        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, partial, nbs);
-        hipLaunchKernelGGL(k_scan_down, dim3(nbs), dim3(SCAN_THREADS), 0, s, hist, nh, nullptr,
-                           partial, nbs, hist);
+        if (gf_scan_excl(hist, hist, nh, partial, s)) return -1;
         hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_THREADS), 0, s, ka, va, n, shift, nb,
                            hist, kb, vb);
         GF_LAUNCH_CHECK();
