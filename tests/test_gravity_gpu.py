@@ -217,6 +217,20 @@ def test_small_and_ragged_sizes(eng, n):
         assert relerr(acc, ref).max() < (1e-12 if p == "fp64" else 5e-5)
 
 
+def test_every_group_size_equals_oracle(eng):
+    """A cloud of n <= 64 particles is one group of n particles, so n = 1..64
+    runs the evaluation with every chunk size: one part of 64 / n slices, or
+    two parts (32, 16 or 8 particles, then the rest) with their own slices."""
+    for n in range(1, 65):
+        pos, mass = ics.gaussian_blob(n, 100 + n)
+        ref, _ = O.accel(pos, mass, 0.5, 0.05, mode=1)
+        ref2, _ = O.accel(pos, mass, 0.5, 0.05, mode=2)
+        for p, r, tol in (("fp64", ref, 1e-12), ("fp32", ref, 5e-5), ("fp32q", ref2, 5e-5)):
+            acc = run(eng, pos, mass, 0.5, 0.05, p)
+            err = relerr(acc, r).max()
+            assert err < tol, (n, p, err)
+
+
 def test_empty_input(eng):
     acc = np.zeros((0, 3))
     eng.gravity(np.zeros((0, 3)), np.zeros(0), acc, 0.5, 0.1)
