@@ -72,6 +72,9 @@ def parse():
                    help="profiling aid: build the workload's tree, then one warm-up and "
                         "three walks of precision PREC (0 FP32, 1 FP64, 2 FP32 quadrupoles), "
                         "nothing else; prints their mean duration")
+    p.add_argument("--knn-blocks-beside-walk", type=int, default=512,
+                   help="persistent blocks of the neighbour pass while the gravity walk runs "
+                        "on the other stream (0 = the library default, every resident slot)")
     p.add_argument("--knn-alone", action="store_true",
                    help="profiling aid: the gas neighbour solve alone, three times from a "
                         "first radius of 1.083 d_k (about the learned proposal's margin)")
@@ -176,6 +179,9 @@ class Bench:
         the first stream, 2-5 on the gas stream)."""
         torch, L, chk, sp = self.torch, self.L, self._lib.check, self.sp
         gs = self.stream2 if overlap else self.stream
+        # beside the walk the neighbour pass takes fewer persistent blocks, so
+        # that the walk keeps more of each CU; alone it takes them all
+        chk(L.gf_set_knn_blocks(self.a.knn_blocks_beside_walk if overlap else 0))
 
         def rec(i, st):
             if ev is not None:

@@ -27,12 +27,17 @@ struct GfSph {
     int *ovf = nullptr, *ovf2 = nullptr;
     int *gstack = nullptr;
     int cap_groups = 0;
+    int fast_blocks = 0;         // persistent blocks of the fast neighbour pass (0 = default)
     int last_passes = 0;         // pass launches of the last solve
     long long stat_queued = 0;   // groups the fast pass handed on, summed over passes
     long long stat_queued2 = 0;  // groups handed on to the global-stack pass
 };
 
 void gf_sph_free(GfSph& S);
+
+// Persistent blocks the fast neighbour pass launches (S.fast_blocks, or the
+// default of one per resident slot of the narrow shape: 1024).
+int gf_sph_fast_blocks(const GfSph& S);
 
 // Builds the gas tree and gathers velocities and energies into tree order.
 // pos, vel (n, 3), mass, u (n) are device arrays in input order (vel and u

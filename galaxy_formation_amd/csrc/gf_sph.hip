@@ -1472,13 +1472,17 @@ int sph_reserve(GfSph& S, int n, int k) {
 
 }  // namespace
 
+int gf_sph_fast_blocks(const GfSph& S) { return S.fast_blocks > 0 ? S.fast_blocks : FAST_BLOCKS; }
+
 void gf_sph_free(GfSph& S) {
     gf_tree_free(S.tree);
     dfree(S.vx); dfree(S.vy); dfree(S.vz); dfree(S.u); dfree(S.r); dfree(S.dk); dfree(S.h);
     dfree(S.rho); dfree(S.done); dfree(S.passes); dfree(S.inv); dfree(S.leaf_of); dfree(S.r_lo); dfree(S.r_hi); dfree(S.rec); dfree(S.p4); dfree(S.gmark); dfree(S.retry); dfree(S.knode);
     dfree(S.anc);
     dfree(S.nb); dfree(S.state); dfree(S.ovf); dfree(S.ovf2); dfree(S.gstack);
+    const int fb = S.fast_blocks;
     S = GfSph();
+    S.fast_blocks = fb;
 }
 
 int gf_sph_setup(GfSph& S, int n, const double* pos, const double* vel, const double* mass,
@@ -1556,7 +1560,7 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
             hipLaunchKernelGGL(k_knn_reset, dim3(1), dim3(64), 0, s, S.state, pass == 0 ? 1 : 0);
             if (!sorted)
                 hipLaunchKernelGGL(r_init ? k_knn_fast<KnnNarrow> : k_knn_fast<KnnWide>,
-                                   dim3(FAST_BLOCKS), dim3(FT), 0, s, a);
+                                   dim3(gf_sph_fast_blocks(S)), dim3(FT), 0, s, a);
             if (kq <= 64) {
                 hipLaunchKernelGGL((k_knn_pass<64, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
                 hipLaunchKernelGGL((k_knn_pass<64, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);

@@ -75,6 +75,15 @@ int gf_get_precision(void);
  * every accepted node with its quadrupole.  tol in [0, 1]. */
 int gf_set_quad_tolerance(double tol);
 double gf_get_quad_tolerance(void);
+/* Scheduling, no counterpart in the reference (sph.py:165-166 calls a k-d
+ * tree): the number of persistent blocks of the fast neighbour pass.
+ * 0 = the default, 1024: every resident slot of the pass (4 blocks of 4
+ * waves per CU), fastest when the solve has the GPU to itself.  A solve
+ * run on a second stream beside the gravity walk leaves the walk more of
+ * each CU with fewer (512: the measured step of bench.py).  The neighbours
+ * do not depend on it.  blocks in [0, 65536]. */
+int gf_set_knn_blocks(int blocks);
+int gf_get_knn_blocks(void);
 /* Testing hook, no counterpart in the reference: lowers the capacity of
  * the gravity walk's LDS stack (walk_stack entries, 1..1024) and of the
  * neighbour solve's candidate lists (knn_list, 1..1024) so that tests can

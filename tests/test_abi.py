@@ -40,3 +40,13 @@ def test_loader_declares_argtypes():
     L = _lib.load()
     assert L.nbody_accel_active.argtypes is not None
     assert L.gf_version() >= 1
+
+
+def test_knn_blocks_setting_is_validated():
+    """The neighbour pass's block count: default 1024, 0 restores it,
+    negative values are refused (no GPU needed)."""
+    L = _lib.load()
+    assert L.gf_get_knn_blocks() == 1024
+    assert L.gf_set_knn_blocks(-1) < 0
+    assert L.gf_set_knn_blocks(512) == 0 and L.gf_get_knn_blocks() == 512
+    assert L.gf_set_knn_blocks(0) == 0 and L.gf_get_knn_blocks() == 1024

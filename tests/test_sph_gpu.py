@@ -243,6 +243,24 @@ def _device_solve(pos, k, r_init=None):
     return nb, dk, passes
 
 
+@pytest.mark.parametrize("blocks", [1, 37, 512])
+def test_neighbours_independent_of_pass_blocks(blocks):
+    """The fast pass is a persistent kernel whose blocks take groups from a
+    shared cursor: any number of blocks gives the same exact neighbours."""
+    from galaxy_formation_amd import _lib
+    L = _lib.load()
+    pos, *_ = disk_gas(20000, seed=17)
+    od, onb = O.knn(pos, 64)
+    _lib.check(L.gf_set_knn_blocks(blocks))
+    try:
+        assert L.gf_get_knn_blocks() == blocks
+        nb, dk, _ = _device_solve(pos, 64, 1.05 * od[:, -1])
+    finally:
+        _lib.check(L.gf_set_knn_blocks(0))
+    np.testing.assert_array_equal(np.sort(nb, axis=1), np.sort(onb, axis=1))
+    np.testing.assert_array_equal(dk, od[:, -1])
+
+
 @pytest.mark.parametrize("k,scale", [(32, None), (64, 1.0), (64, 0.4), (64, 3.0), (96, None)])
 def test_device_neighbour_sets_equal_bruteforce(k, scale):
     pos, *_ = disk_gas(20000, seed=11)
