@@ -48,6 +48,9 @@ WALK_TRAFFIC_BYTES = {0: 3527188960, 2: 3679273984}
 # the accuracy gates of the metric: RMS relative acceleration error vs the
 # FP64 direct sum, and density vs the plain h solve (with h equal to it)
 GATE_ACC_RMS = 1e-3
+# quadrupole walk: a node whose quadrupole term is bounded below this fraction
+# of the group's smallest previous-step |acc| is evaluated as a monopole
+QUAD_REL_TOL = 1e-4
 GATE_RHO = 1e-4
 PEAK_HBM = 8.0e12
 GAMMA, ALPHA, BETA = 5.0 / 3.0, 1.0, 2.0
@@ -102,9 +105,42 @@ def knn_alone_only(a):
           f"per solve ({passes} pass launches)", flush=True)
 
 
+def previous_state(B):
+    """The state one step earlier: the same particles drifted back by a time
+    step D = the 10th percentile of the particles' admissible steps (the long
+    step of simulation.py:117).  Its d_k, div v and Courant steps seed the
+    fast path's first radius; its FP64 gravity of every particle is the
+    quadrupole walk's entry acc.  Returns D."""
+    torch, c, n, ng = B.torch, B.c, B.n, B.ng
+    B.step(1, None)
+    torch.cuda.synchronize()
+    dt_c = B.out[1]["dt"].cpu().numpy()
+    ag = B.acc_g[1].cpu().numpy()
+    agas = ag[:ng] + B.out[1]["acc"].cpu().numpy()
+    amag = np.linalg.norm(np.concatenate([agas, ag[ng:]]), axis=1)
+    dt_ind = 0.3 * np.sqrt(c.eps / np.maximum(amag, 1e-30))
+    dt_ind[:ng] = np.minimum(dt_ind[:ng], dt_c)
+    D = float(np.percentile(dt_ind, 10))
+    pos_prev = B.pos[:ng] - D * B.vel[:ng]
+    B.hydro(1, pos_prev, None)
+    B._lib.check(B.L.gf_hydro_dk_dev(B.p(B.dk_prev), B.sp))
+    B.divv_prev.copy_(B.out[1]["divv"])
+    B.dtc_prev.copy_(B.out[1]["dt"])
+    B.D = D
+    acc_prev = torch.zeros_like(B.acc_g[1])
+    B._lib.check(B.L.gf_tree_build_dev(n, B.p((B.pos - D * B.vel).contiguous()), B.p(B.mass),
+                                       B.a.theta, B.sp))
+    B._lib.check(B.L.gf_walk_dev(n, None, B.p(acc_prev), c.eps, 1, B.sp))
+    torch.cuda.synchronize()
+    B.acc_prev = acc_prev
+    return D
+
+
 def walk_alone_only(a):
     """The gravity walk of the workload and nothing else, for counter runs."""
     B = Bench(a)
+    if a.walk_alone == 2:
+        previous_state(B)      # the entry accelerations of the relative rule
     ms = B.walk_alone(a.walk_alone, launches=3)
     print(f"walk alone, precision {a.walk_alone}, {a.workload}: {ms:.3f} ms per launch "
           f"(mean of 3 after one warm-up)", flush=True)
@@ -143,6 +179,9 @@ class Bench:
         self.sp2 = self.stream2.cuda_stream
         self.fork = torch.cuda.Event()
         self.join = torch.cuda.Event()
+        # the previous step's gravity (set by main): the quadrupole walk reads
+        # it from its acc array on entry, as a simulation's acc_grav holds it
+        self.acc_prev = None
 
     def p(self, t):
         return None if t is None else t.data_ptr()
@@ -173,6 +212,15 @@ class Bench:
         rec(5) if rec else None
         return passes
 
+    def entry_acc(self, prec):
+        """The quadrupole walk's entry accelerations: the previous step's
+        gravity is put back into its acc array (on the first stream), and the
+        relative rule is switched on for it alone."""
+        rel = prec == 2 and self.acc_prev is not None
+        self._lib.check(self.L.gf_set_quad_relative_tolerance(QUAD_REL_TOL if rel else 0.0))
+        if rel:
+            self.acc_g[2].copy_(self.acc_prev)
+
     def step(self, prec, first, ev=None, overlap=True):
         """Gravity on the first stream, the gas on the second (or, without
         overlap, after it on the first), joined at the end (events 0-1, 6 on
@@ -187,6 +235,7 @@ class Bench:
             if ev is not None:
                 ev[i].record(st)
         rec(0, self.stream)
+        self.entry_acc(prec)          # inside the timed step
         self.fork.record(self.stream)
         chk(L.gf_tree_build_dev(self.n, self.p(self.pos), self.p(self.mass), self.a.theta, sp))
         rec(1, self.stream)
@@ -201,19 +250,22 @@ class Bench:
 
     def walk_alone(self, prec, launches=5):
         """Average duration of the walk on a built tree, nothing else running
-        (HIP events on the stream it is launched on)."""
+        (HIP events on the stream it is launched on, around each launch; the
+        entry accelerations are put back between launches)."""
         torch, L, chk = self.torch, self.L, self._lib.check
         chk(L.gf_tree_build_dev(self.n, self.p(self.pos), self.p(self.mass), self.a.theta, self.sp))
+        self.entry_acc(prec)
         chk(L.gf_walk_dev(self.n, None, self.p(self.acc_g[prec]), self.c.eps, prec, self.sp))
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(launches)]
         torch.cuda.synchronize()
-        e0.record(self.stream)
-        for _ in range(launches):
+        for e0, e1 in evs:
+            self.entry_acc(prec)
+            e0.record(self.stream)
             chk(L.gf_walk_dev(self.n, None, self.p(self.acc_g[prec]), self.c.eps, prec, self.sp))
-        e1.record(self.stream)
+            e1.record(self.stream)
         torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / launches
+        return float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs]))
 
     def timed(self, prec, first, steps=None, overlap=True):
         torch = self.torch
@@ -276,7 +328,8 @@ def steploop(a):
         log = lambda m: print(f"[steploop {a.workload} {name}] {m}", file=sys.stderr,  # noqa: E731
                               flush=True)
         log("start: forces of all particles, energy")
-        S = RungStepper(c, theta=a.theta, k=a.k, precision=prec, first=first, model=model)
+        S = RungStepper(c, theta=a.theta, k=a.k, precision=prec, first=first, model=model,
+                        quad_relative_tolerance=QUAD_REL_TOL if prec == "fp32q" else None)
         S.start()
         torch.cuda.synchronize()
         log(f"timed: {a.substeps} sub-steps")
@@ -343,25 +396,7 @@ def main():
     torch.cuda.synchronize()
     del cal, cal2
 
-    # previous state of the gas: the same particles one step earlier (drifted
-    # back by a time step D = 10th percentile of the particles' admissible
-    # steps, the long step of simulation.py:117); its h seeds the fast path
-    B.step(1, None)
-    torch.cuda.synchronize()
-    dt_c = B.out[1]["dt"].cpu().numpy()
-    ag = B.acc_g[1].cpu().numpy()
-    agas = ag[:ng] + B.out[1]["acc"].cpu().numpy()
-    amag = np.linalg.norm(np.concatenate([agas, ag[ng:]]), axis=1)
-    dt_ind = 0.3 * np.sqrt(c.eps / np.maximum(amag, 1e-30))
-    dt_ind[:ng] = np.minimum(dt_ind[:ng], dt_c)
-    D = float(np.percentile(dt_ind, 10))
-    pos_prev = B.pos[:ng] - D * B.vel[:ng]
-    B.hydro(1, pos_prev, None)
-    B._lib.check(B.L.gf_hydro_dk_dev(B.p(B.dk_prev), B.sp))
-    B.divv_prev.copy_(B.out[1]["divv"])
-    B.dtc_prev.copy_(B.out[1]["dt"])
-    B.D = D
-    torch.cuda.synchronize()
+    D = previous_state(B)
 
     from galaxy_formation_amd.learned_h import LearnedH
     model = LearnedH().load(a.model)
@@ -453,6 +488,10 @@ def main():
                                "gravity + SPH (kNN, h, density, forces)",
                    "n": n, "n_gas": ng, "theta": a.theta, "eps": c.eps, "k": a.k,
                    "path": head, "first_radius_fast": "learned proposal",
+                   "quad_rule": {"relative_tolerance": QUAD_REL_TOL,
+                                 "entry_acc": "previous step's FP64 gravity (every particle "
+                                              "drifted back by previous_step_D), restored "
+                                              "inside each timed step"},
                    "gates": {"acc_rms_vs_direct": GATE_ACC_RMS, "rho_vs_plain": GATE_RHO},
                    "meets_gates": fast["meets_accuracy_gates"]},
         # compute-bound ("mfma" names the compute roof): the walk evaluates

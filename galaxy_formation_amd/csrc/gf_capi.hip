@@ -125,6 +125,17 @@ int gf_set_quad_tolerance(double tol) {
 
 double gf_get_quad_tolerance(void) { return double(g_ctx.walk.quad_tol); }
 
+int gf_set_quad_relative_tolerance(double tol) {
+    if (!(tol >= 0.0) || tol > 1.0) {
+        gf_set_error("relative quadrupole tolerance must be in [0, 1]");
+        return -1;
+    }
+    g_ctx.walk.quad_rel_tol = float(tol);
+    return 0;
+}
+
+double gf_get_quad_relative_tolerance(void) { return double(g_ctx.walk.quad_rel_tol); }
+
 int gf_set_knn_blocks(int blocks) {
     if (blocks < 0 || blocks > 65536) {
         gf_set_error("neighbour-pass blocks must be in [0, 65536] (0 = default)");

@@ -20,6 +20,11 @@ struct GfWalkScratch {
     // this share of its monopole at the group's nearest point is evaluated
     // as a monopole (0 = every accepted node with its quadrupole)
     float quad_tol = 0.01f;
+    // precision 2, 0 = off: an accepted node whose quadrupole term is bounded
+    // below this fraction of the smallest |acc| the group's active members
+    // hold on entry (the caller's previous accelerations) is evaluated as a
+    // monopole; groups without that history use quad_tol
+    float quad_rel_tol = 0.0f;
 };
 
 int gf_walk_reserve(GfWalkScratch& W, int max_groups);

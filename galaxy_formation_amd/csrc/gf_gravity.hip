@@ -47,6 +47,8 @@ constexpr int SCAP = WALK_SCAP;                     // LDS stack entries
 constexpr int LCAP = 256;                           // LDS list entries
 constexpr int LCAP_QUAD = WALK_LCAP_QUAD;           // particle list with quadrupoles
 constexpr int QCAP = WALK_QCAP;                     // node list with quadrupoles (48 B each)
+// one batch of 64 nodes appends up to 64 entries to a list after one flush
+static_assert(LCAP >= 64 && LCAP_QUAD >= 64 && QCAP >= 64, "list capacities below a batch");
 constexpr int GSCAP = 512 * (GF_KEY_LEVELS + 2);    // global stack entries per block
 #ifndef WALK_GRID
 #define WALK_GRID 8192
@@ -65,13 +67,16 @@ struct GNode {
     float q[6];           // traceless quadrupole (xx, yy, zz, xy, xz, yz), FP32
     float qlen;           // sqrt(3 max |Q_ij| / m): the quadrupole term's share of
                           // the monopole at distance d is at most about (qlen / d)^2
-    float qpad1;
+    float qabs4;          // (2.858 |Q|_F)^(1/4): the quadrupole term's acceleration at
+                          // distance d is at most (qabs4 / d)^4 (3.5 |Q|_2 / d^4 with
+                          // |Q|_2 <= sqrt(2/3) |Q|_F for a traceless symmetric Q)
 };
 static_assert(sizeof(GNode) == 128, "GNode layout");
 static_assert(offsetof(GNode, open_r) == 32 && offsetof(GNode, start) == 40 &&
                   offsetof(GNode, leaf) == 48 && offsetof(GNode, nchild) == 52 &&
                   offsetof(GNode, c_lo) == 64 && offsetof(GNode, c_hi) == 80 &&
-                  offsetof(GNode, q) == 96 && offsetof(GNode, qlen) == 120,
+                  offsetof(GNode, q) == 96 && offsetof(GNode, qlen) == 120 &&
+                  offsetof(GNode, qabs4) == 124,
               "the walk reads GNode as eight 16-byte words");
 typedef int v4i __attribute__((ext_vector_type(4)));
 
@@ -104,6 +109,12 @@ struct WalkArgs {
     float quad_rtol;           // QUAD: 1 / sqrt(tolerance), +inf for 0: a node whose
                                // quadrupole term is below the tolerance's share of its
                                // monopole at the group's nearest point is a monopole
+    float quad_atol;           // QUAD, 0 = off: a node whose quadrupole term at the
+                               // group's nearest point is below quad_atol times the
+                               // smallest |acc| the group's active members hold on entry
+                               // (the previous step's) is a monopole; groups without
+                               // that history (a zero or non-finite row, more than 64
+                               // members) keep the quad_rtol rule
 };
 
 template <bool F64>
@@ -260,6 +271,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
 
         // activity and the sphere around the massive members (FP64)
         bool any_act = false;
+        bool no_hist = false;           // QUAD, relative rule: an active member whose
+        double amin2 = INFINITY;        // entry |acc|^2 is zero or not finite; the least
         double x0 = 1e300, y0 = 1e300, z0 = 1e300, x1 = -1e300, y1 = -1e300, z1 = -1e300;
         for (int c0 = 0; c0 < np; c0 += 64) {
             const int i = b + c0 + lane;
@@ -267,6 +280,16 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
             bool act = false;
             if (in) {
                 act = a.active ? a.active[a.idx[i]] != 0 : true;
+                if constexpr (QUAD) {
+                    if (act && a.quad_atol > 0.0f) {
+                        const double* o = a.acc + 3 * size_t(a.idx[i]);
+                        const double a2 = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+                        if (a2 > 0.0 && a2 < INFINITY)
+                            amin2 = fmin(amin2, a2);
+                        else
+                            no_hist = true;
+                    }
+                }
                 const double4 q = a.p4[i];
                 if (q.w > 0) {
                     const double x = q.x, y = q.y, z = q.z;
@@ -292,6 +315,22 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
             }
         }
         const double br = sqrt(gf_wave_max_d(br2));
+        // the quadrupole rule of this group: relative to the members' entry
+        // accelerations where every active member has one (and the group is
+        // one chunk, so no member's row is rewritten before the test), else
+        // the share of the node's own monopole
+        bool qrel = false;
+        float qfac = a.quad_rtol;
+        if constexpr (QUAD) {
+            if (a.quad_atol > 0.0f && np <= 64) {
+                const bool bad = __ballot(no_hist) != 0;
+                const double m2 = gf_wave_min_d(amin2);
+                if (!bad && m2 < INFINITY) {
+                    qrel = true;
+                    qfac = float(1.0 / sqrt(sqrt(double(a.quad_atol) * sqrt(m2))));
+                }
+            }
+        }
 
         for (int c0 = 0; c0 < np; c0 += 64) {
             const int P = min(64, np - c0);
@@ -372,9 +411,12 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 if constexpr (QUAD) {
                     // the quadrupole term's share at the group's nearest
                     // point, about (qlen / (d - br))^2, is below the
-                    // tolerance iff qlen / sqrt(tolerance) + br < d (an
-                    // estimate: FP32 is enough)
-                    const float x = fmaf(__int_as_float(rq45.z), a.quad_rtol, float(br));
+                    // tolerance iff qlen / sqrt(tolerance) + br < d; with the
+                    // relative rule, its bound (qabs4 / (d - br))^4 is below
+                    // quad_atol |acc|_min iff qabs4 qfac + br < d (estimates:
+                    // FP32 is enough)
+                    const float fld = __int_as_float(qrel ? rq45.w : rq45.z);
+                    const float x = fmaf(fld, qfac, float(br));
                     small_q = x * x < float(d2);
                 }
                 // push children (lane order keeps the stack sorted by depth)
@@ -587,7 +629,13 @@ __global__ __launch_bounds__(256) void k_gnode_pack(const int* __restrict__ coun
             qmax = fmax(qmax, fabs(qk));
         }
         g.qlen = g.com.w > 0 ? float(sqrt(3.0 * qmax / g.com.w)) : 0.0f;
-        g.qpad1 = 0.0f;
+        double qf2 = 0.0;
+        if (quad) {
+            const double* Q = quad + 6 * size_t(id);
+            qf2 = Q[0] * Q[0] + Q[1] * Q[1] + Q[2] * Q[2] +
+                  2.0 * (Q[3] * Q[3] + Q[4] * Q[4] + Q[5] * Q[5]);
+        }
+        g.qabs4 = float(sqrt(sqrt(2.858 * sqrt(qf2))));
         gn[id] = g;
     }
 }
@@ -796,6 +844,7 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
     a.inter_nodes = W.inter_nodes;
     a.lds_scap = gf_test_limits().walk_stack > 0 ? std::min(SCAP, gf_test_limits().walk_stack) : SCAP;
     a.quad_rtol = W.quad_tol > 0 ? float(1.0 / std::sqrt(double(W.quad_tol))) : INFINITY;
+    a.quad_atol = W.quad_rel_tol;
     hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, W.work, W.ovf_count);
     if (precision == GF_FP64) {
         hipLaunchKernelGGL((k_walk<true, false, false>), dim3(WALK_BLOCKS), dim3(64), 0, s, a);

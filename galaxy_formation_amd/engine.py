@@ -19,13 +19,26 @@ class Engine:
     the accepted nodes' quadrupoles).  Tree, centres of mass and opening tests
     are FP64 in all three.  quad_tolerance (fp32q): a node whose quadrupole
     term is estimated below this share of its monopole is evaluated as a
-    monopole (0 = every node with its quadrupole)."""
+    monopole (0 = every node with its quadrupole).  quad_relative_tolerance
+    (fp32q, 0 = off): where the acc array passed to gravity holds the previous
+    step's accelerations, a node whose quadrupole term is bounded below this
+    fraction of the group's smallest |acc| is evaluated as a monopole."""
 
-    def __init__(self, precision="fp32", quad_tolerance=None):
+    def __init__(self, precision="fp32", quad_tolerance=None, quad_relative_tolerance=None):
         self.lib = _lib.load()
         self.precision = precision
         if quad_tolerance is not None:
             self.quad_tolerance = quad_tolerance
+        if quad_relative_tolerance is not None:
+            self.quad_relative_tolerance = quad_relative_tolerance
+
+    @property
+    def quad_relative_tolerance(self):
+        return float(self.lib.gf_get_quad_relative_tolerance())
+
+    @quad_relative_tolerance.setter
+    def quad_relative_tolerance(self, tol):
+        _lib.check(self.lib.gf_set_quad_relative_tolerance(float(tol)))
 
     @property
     def quad_tolerance(self):
@@ -48,8 +61,15 @@ class Engine:
     def gravity(self, pos, mass, acc, theta, eps, active=None, rebuild=True):
         """Barnes-Hut accelerations (G = 1) written into acc for the active
         particles.  rebuild=False refits the previous tree instead of building
-        a new one (engine.py:45-50)."""
+        a new one (engine.py:45-50).  The reference checks dtype and layout
+        only; the shapes are checked here too, because the library reads and
+        writes 3 n doubles of pos and acc and n of mass and active."""
         n = len(mass)
+        if np.shape(pos) != (n, 3) or np.shape(acc) != (n, 3):
+            raise ValueError(f"pos and acc must have shape ({n}, 3), got {np.shape(pos)} "
+                             f"and {np.shape(acc)}")
+        if active is not None and np.size(active) != n:
+            raise ValueError(f"active must hold {n} entries, got {np.size(active)}")
         act = np.ones(n, np.uint8) if active is None else np.ascontiguousarray(active, dtype=np.uint8)
         _lib.check(self.lib.gf_set_precision(PRECISIONS[self._precision]))
         _lib.check(self.lib.nbody_accel_active(n, pos, mass, acc, theta, eps, _lib.ptr(act),
@@ -57,6 +77,8 @@ class Engine:
 
     def gravity_direct(self, pos, mass, eps):
         """Exact O(N^2) accelerations (engine.py:52-56)."""
+        if np.shape(pos) != (len(mass), 3):
+            raise ValueError(f"pos must have shape ({len(mass)}, 3), got {np.shape(pos)}")
         acc = np.zeros_like(pos)
         _lib.check(self.lib.nbody_accel_direct(len(mass), pos, mass, acc, eps))
         return acc

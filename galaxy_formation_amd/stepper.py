@@ -30,7 +30,7 @@ GAMMA, ALPHA, BETA = 5.0 / 3.0, 1.0, 2.0
 
 class ForceEvaluator:
     def __init__(self, n, n_gas, eps, theta=0.5, k=64, precision="fp32", device="cuda:0",
-                 stream=None):
+                 stream=None, quad_relative_tolerance=None):
         import torch
         self.torch = torch
         self.L = _lib.load()
@@ -39,6 +39,9 @@ class ForceEvaluator:
         self.eps, self.theta, self.k = eps, theta, k
         self.prec = {"fp32": 0, "fp64": 1, "fp32q": 2}[precision]
         self.stream = stream
+        # fp32q: the relative quadrupole rule, fed by acc_grav, which keeps the
+        # previous evaluation's gravity (None leaves the library's setting)
+        self.quad_rel = quad_relative_tolerance
         z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.dev)  # noqa: E731
         self.acc_grav = z(n, 3)
         self.acc = z(n, 3)
@@ -57,6 +60,8 @@ class ForceEvaluator:
 
     def gravity(self, pos, mass, active=None, rebuild=True, ev=None):
         chk, L = _lib.check, self.L
+        if self.prec == 2 and self.quad_rel is not None:
+            chk(L.gf_set_quad_relative_tolerance(float(self.quad_rel)))
         if ev is not None:
             ev[0].record()
         chk(L.gf_tree_build_dev(self.n, pos.data_ptr(), mass.data_ptr(), self.theta, self.sp)
@@ -222,7 +227,7 @@ class RungStepper:
     potential, the potential by the FP64 direct sum (gf_potential_dev)."""
 
     def __init__(self, particles, theta=0.5, k=64, precision="fp32", rungs=6, first="plain",
-                 model=None, device="cuda:0", refit=False):
+                 model=None, device="cuda:0", refit=False, quad_relative_tolerance=None):
         import torch
         self.torch = torch
         dev = torch.device(device)
@@ -238,7 +243,8 @@ class RungStepper:
         self.R = rungs
         self.refit = refit
         self.first, self.model = first, model
-        self.F = ForceEvaluator(p.n, self.ng, p.eps, theta, k, precision, device)
+        self.F = ForceEvaluator(p.n, self.ng, p.eps, theta, k, precision, device,
+                                quad_relative_tolerance=quad_relative_tolerance)
         self.L = self.F.L
         self.phi = torch.zeros(p.n, dtype=torch.float64, device=dev)
         self.rung = torch.zeros(p.n, dtype=torch.int64, device=dev)

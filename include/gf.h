@@ -75,6 +75,16 @@ int gf_get_precision(void);
  * every accepted node with its quadrupole.  tol in [0, 1]. */
 int gf_set_quad_tolerance(double tol);
 double gf_get_quad_tolerance(void);
+/* Precision 2 only, no counterpart in the reference, 0 = off (the default):
+ * the acceleration rows a gravity call receives in acc (the caller's
+ * previous step, as simulation.py:64 passes self.acc_grav) set the rule
+ * instead.  An accepted node whose quadrupole term is bounded (2.858 |Q|_F /
+ * d^4, d as above) below tol times the smallest |acc| of the group's active
+ * members is evaluated as a monopole.  Groups with an active member whose
+ * row is zero or not finite, and groups of more than 64 particles, keep the
+ * gf_set_quad_tolerance rule.  tol in [0, 1]. */
+int gf_set_quad_relative_tolerance(double tol);
+double gf_get_quad_relative_tolerance(void);
 /* Scheduling, no counterpart in the reference (sph.py:165-166 calls a k-d
  * tree): the number of persistent blocks of the fast neighbour pass.
  * 0 = the default, 1024: every resident slot of the pass (4 blocks of 4

@@ -50,3 +50,37 @@ def test_knn_blocks_setting_is_validated():
     assert L.gf_set_knn_blocks(-1) < 0
     assert L.gf_set_knn_blocks(512) == 0 and L.gf_get_knn_blocks() == 512
     assert L.gf_set_knn_blocks(0) == 0 and L.gf_get_knn_blocks() == 1024
+
+
+def test_quad_relative_tolerance_setting_is_validated():
+    """The relative quadrupole rule: off (0) by default, values outside
+    [0, 1] and NaN are refused and leave the setting as it was (no GPU
+    needed)."""
+    L = _lib.load()
+    assert L.gf_get_quad_relative_tolerance() == 0.0
+    assert L.gf_set_quad_relative_tolerance(-1e-4) < 0
+    assert L.gf_set_quad_relative_tolerance(float("nan")) < 0
+    assert L.gf_set_quad_relative_tolerance(2.0) < 0
+    assert L.gf_get_quad_relative_tolerance() == 0.0
+    assert L.gf_set_quad_relative_tolerance(1e-4) == 0
+    assert abs(L.gf_get_quad_relative_tolerance() - 1e-4) < 1e-4 * 1e-6   # held as FP32
+    assert L.gf_set_quad_relative_tolerance(0.0) == 0 and L.gf_get_quad_relative_tolerance() == 0.0
+
+
+def test_engine_rejects_misshapen_arrays():
+    """Engine.gravity refuses pos/acc that are not (n, 3) and an active mask
+    of another length before any library call (the library would read and
+    write 3 n doubles)."""
+    import numpy as np
+    import pytest
+    from galaxy_formation_amd.engine import Engine
+    e = Engine("fp32")
+    pos, mass = np.zeros((10, 3)), np.ones(10)
+    with pytest.raises(ValueError):
+        e.gravity(pos, mass, np.zeros((10, 1)), 0.5, 0.01)
+    with pytest.raises(ValueError):
+        e.gravity(np.zeros((9, 3)), mass, np.zeros((10, 3)), 0.5, 0.01)
+    with pytest.raises(ValueError):
+        e.gravity(pos, mass, np.zeros((10, 3)), 0.5, 0.01, active=np.ones(9, np.uint8))
+    with pytest.raises(ValueError):
+        e.gravity_direct(np.zeros((10, 2)), mass, 0.01)

@@ -336,3 +336,96 @@ def test_fused_quadrupole_pass_equals_separate_pass():
     np.testing.assert_array_equal(a1, a2)
     a3 = run(e, c.pos, c.mass, 0.5, c.eps, "fp32q", rebuild=False)   # refit, fused
     np.testing.assert_array_equal(a1, a3)
+
+
+def _prev_gravity(eng, c, dt):
+    """FP64 gravity of the particles drifted back by dt: the previous step's
+    accelerations a simulation holds in its acc array."""
+    return run(eng, np.ascontiguousarray(c.pos - dt * c.vel), c.mass, 0.5, c.eps, "fp64")
+
+
+def test_quad_relative_rule_without_history_is_the_share_rule():
+    """With the relative rule on, a group with an active member whose entry
+    row is zero or not finite keeps the share rule: entry accelerations of
+    zeros, or a NaN in every group, give bit-identical results to the rule
+    switched off.  Inactive rows do not count: huge values there change
+    nothing either."""
+    from galaxy_formation_amd.engine import Engine
+    e = Engine("fp32q")
+    c = ics.config("cloud10k")
+    base = run(e, c.pos, c.mass, 0.5, c.eps, "fp32q")
+    try:
+        e.quad_relative_tolerance = 1e-4
+        np.testing.assert_array_equal(run(e, c.pos, c.mass, 0.5, c.eps, "fp32q"), base)
+        nan = np.full_like(c.pos, np.nan)
+        np.testing.assert_array_equal(run(e, c.pos, c.mass, 0.5, c.eps, "fp32q", acc=nan), base)
+        # half the particles active: the inactive rows hold 1e300, the active
+        # ones zeros
+        act = (np.arange(c.n) % 2).astype(np.uint8)
+        entry = np.where(act[:, None] == 1, np.zeros_like(c.pos), 1e300)
+        ref = np.where(act[:, None] == 1, base, 1e300)
+        got = run(e, c.pos, c.mass, 0.5, c.eps, "fp32q", active=act, acc=entry)
+        np.testing.assert_array_equal(got, ref)
+    finally:
+        e.quad_relative_tolerance = 0.0
+
+
+def test_quad_relative_rule_trades_little_accuracy():
+    """Fed the previous step's gravity, the relative rule keeps the
+    interaction set, evaluates fewer quadrupoles than the share rule, stays
+    within the 1e-3 gate and close to the all-quadrupole error, and depends
+    little on whether the entry rows are the previous step's or exact."""
+    from galaxy_formation_amd import _lib
+    from galaxy_formation_amd.engine import Engine
+    e = Engine("fp32q")
+    L = _lib.load()
+    c = ics.config("cloud10k")
+    exact = e.gravity_direct(c.pos, c.mass, c.eps)
+    prev = _prev_gravity(e, c, 0.5)
+    now = run(e, c.pos, c.mass, 0.5, c.eps, "fp64")
+    assert np.sqrt(np.mean(relerr(prev, now) ** 2)) > 1e-3      # a real previous step
+
+    def quad(tol, rel, entry):
+        e.quad_tolerance, e.quad_relative_tolerance = tol, rel
+        acc = run(e, c.pos, c.mass, 0.5, c.eps, "fp32q", acc=entry.copy())
+        nodes = np.zeros(1, np.int64)
+        _lib.check(L.gf_node_interactions_dev(_lib.ptr(nodes)))
+        return acc, e.last_interactions(), int(nodes[0])
+
+    try:
+        a0, i0, q0 = quad(0.0, 0.0, prev)         # every accepted node with its quadrupole
+        a1, i1, q1 = quad(0.01, 0.0, prev)        # the share rule
+        a2, i2, q2 = quad(0.01, 1e-4, prev)       # the relative rule
+        a3, i3, q3 = quad(0.01, 1e-4, exact)
+    finally:
+        e.quad_tolerance, e.quad_relative_tolerance = 0.01, 0.0
+    assert i0 == i1 == i2 == i3
+    assert q2 < q1 < q0
+    r0, r2, r3 = (np.sqrt(np.mean(relerr(a, exact) ** 2)) for a in (a0, a2, a3))
+    assert r2 < 1e-3 and r2 < 1.5 * r0
+    assert abs(r2 - r3) < 0.1 * r0
+
+
+def test_quad_relative_tolerance_orders_the_quadrupole_count():
+    """A larger relative tolerance demotes more nodes: the quadrupole entries
+    fall and the error against the direct sum does not fall."""
+    from galaxy_formation_amd import _lib
+    from galaxy_formation_amd.engine import Engine
+    e = Engine("fp32q")
+    L = _lib.load()
+    c = ics.config("cloud10k")
+    exact = e.gravity_direct(c.pos, c.mass, c.eps)
+    prev = _prev_gravity(e, c, 0.5)
+    q, r = [], []
+    try:
+        for rel in (1e-6, 1e-5, 1e-4, 1e-3, 1e-2):
+            e.quad_relative_tolerance = rel
+            acc = run(e, c.pos, c.mass, 0.5, c.eps, "fp32q", acc=prev.copy())
+            nodes = np.zeros(1, np.int64)
+            _lib.check(L.gf_node_interactions_dev(_lib.ptr(nodes)))
+            q.append(int(nodes[0]))
+            r.append(np.sqrt(np.mean(relerr(acc, exact) ** 2)))
+    finally:
+        e.quad_relative_tolerance = 0.0
+    assert all(x > y for x, y in zip(q, q[1:])), q
+    assert r[-1] > r[0]
