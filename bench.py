@@ -136,9 +136,22 @@ def previous_state(B):
     return D
 
 
+def calibration_copy(B):
+    """A plain copy of known size before the timed region: the reference point
+    for rocprofv3's FETCH_SIZE / WRITE_SIZE counters (256 MiB read + written)."""
+    torch = B.torch
+    cal = torch.empty(32 << 20, dtype=torch.float64, device=B.dev)
+    cal.fill_(1.0)
+    cal2 = torch.empty_like(cal)
+    cal2.copy_(cal)
+    torch.cuda.synchronize()
+    del cal, cal2
+
+
 def walk_alone_only(a):
     """The gravity walk of the workload and nothing else, for counter runs."""
     B = Bench(a)
+    calibration_copy(B)
     if a.walk_alone == 2:
         previous_state(B)      # the entry accelerations of the relative rule
     ms = B.walk_alone(a.walk_alone, launches=3)
@@ -387,15 +400,7 @@ def main():
         return
     B = Bench(a)
     torch, n, ng, c = B.torch, B.n, B.ng, B.c
-    # a plain copy of known size before the timed region: the reference point
-    # for rocprofv3's FETCH_SIZE / WRITE_SIZE counters (256 MiB read + written)
-    cal = torch.empty(32 << 20, dtype=torch.float64, device=B.dev)
-    cal.fill_(1.0)
-    cal2 = torch.empty_like(cal)
-    cal2.copy_(cal)
-    torch.cuda.synchronize()
-    del cal, cal2
-
+    calibration_copy(B)
     D = previous_state(B)
 
     from galaxy_formation_amd.learned_h import LearnedH
