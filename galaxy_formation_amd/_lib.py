@@ -36,6 +36,8 @@ SIGNATURES = {
     "gf_get_quad_tolerance": (_cd, []),
     "gf_set_quad_relative_tolerance": (_ci, [_cd]),
     "gf_get_quad_relative_tolerance": (_cd, []),
+    "gf_set_sph_precision": (_ci, [_ci]),
+    "gf_get_sph_precision": (_ci, []),
     "gf_set_knn_blocks": (_ci, [_ci]),
     "gf_get_knn_blocks": (_ci, []),
     "gf_get_precision": (_ci, []),

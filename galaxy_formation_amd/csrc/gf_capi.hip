@@ -136,6 +136,17 @@ int gf_set_quad_relative_tolerance(double tol) {
 
 double gf_get_quad_relative_tolerance(void) { return double(g_ctx.walk.quad_rel_tol); }
 
+int gf_set_sph_precision(int precision) {
+    if (precision != 0 && precision != 1) {
+        gf_set_error("SPH precision must be 0 (FP64) or 1 (FP32 pair arithmetic)");
+        return -1;
+    }
+    g_ctx.sph.pair_f32 = precision;
+    return 0;
+}
+
+int gf_get_sph_precision(void) { return g_ctx.sph.pair_f32; }
+
 int gf_set_knn_blocks(int blocks) {
     if (blocks < 0 || blocks > 65536) {
         gf_set_error("neighbour-pass blocks must be in [0, 65536] (0 = default)");

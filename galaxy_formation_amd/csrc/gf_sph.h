@@ -28,6 +28,7 @@ struct GfSph {
     int *gstack = nullptr;
     int cap_groups = 0;
     int fast_blocks = 0;         // persistent blocks of the fast neighbour pass (0 = default)
+    int pair_f32 = 0;            // density and forces: FP32 pair arithmetic, FP64 sums
     int last_passes = 0;         // pass launches of the last solve
     long long stat_queued = 0;   // groups the fast pass handed on, summed over passes
     long long stat_queued2 = 0;  // groups handed on to the global-stack pass

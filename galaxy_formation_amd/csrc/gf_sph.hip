@@ -1313,6 +1313,149 @@ __global__ __launch_bounds__(TB) void k_force(int n, const int* __restrict__ nb,
     if (h_out) h_out[o] = hi_;
 }
 
+// FP32 pair arithmetic (the fast path's density and forces): separations
+// are formed in FP64 from the FP64 positions and rounded to FP32 once, the
+// pair terms are FP32 (v_sqrt_f32, v_rcp_f32, no branches), and every sum is
+// accumulated in FP64.  The neighbour table, h and the order of the sums are
+// those of the FP64 kernels; the density differs from theirs by FP32
+// rounding of the terms (about 1e-7 relative).
+// Record per particle (48 bytes, in the FP64 record's memory): v = (vx, vy,
+// vz, c), k = (h, 1/h, 1/(pi h^4), rho), d = (P / rho^2, m, 1/(pi h^3), -).
+struct SphRec32 {
+    float4 v, k, d;
+};
+
+__device__ __forceinline__ float kernel_w32(float r, float ihv, float s3) {
+    const float q = r * ihv;
+    const float t = fmaxf(2.0f - q, 0.0f);
+    const float inner = 1.0f - 1.5f * q * q + 0.75f * (q * q * q);
+    return s3 * (q < 1.0f ? inner : 0.25f * (t * t * t));
+}
+
+// (1/r) dW/dr without the 1/r: s4 {-3q + 2.25q^2; -0.75(2 - q)^2; 0}
+__device__ __forceinline__ float kernel_g32(float r, float ihv, float s4) {
+    const float q = r * ihv;
+    const float t = fmaxf(2.0f - q, 0.0f);
+    return s4 * (q < 1.0f ? (-3.0f * q + 2.25f * q * q) : -0.75f * (t * t));
+}
+
+__global__ __launch_bounds__(TB) void k_pack32(SphView v, double gamma, SphRec32* __restrict__ rec,
+                                               double4* __restrict__ a4) {
+    const int t = blockIdx.x * TB + threadIdx.x;
+    if (t >= v.n) return;
+    const double hv = v.h[t];
+    a4[t] = make_double4(v.x[t], v.y[t], v.z[t], v.m[t]);
+    SphRec32 r;
+    r.v = make_float4(float(v.vx[t]), float(v.vy[t]), float(v.vz[t]),
+                      float(sqrt(gamma * (gamma - 1) * v.u[t])));
+    r.k = make_float4(float(hv), float(1.0 / hv), float(1.0 / (M_PI * (hv * hv * hv * hv))), 0.0f);
+    r.d = make_float4(0.0f, float(v.m[t]), float(1.0 / (M_PI * (hv * hv * hv))), 0.0f);
+    rec[t] = r;
+}
+
+__global__ __launch_bounds__(TB) void k_density32(int n, const int* __restrict__ nb, int k,
+                                                  const double* __restrict__ u, double gamma,
+                                                  const double4* __restrict__ a4,
+                                                  SphRec32* __restrict__ rec,
+                                                  double* __restrict__ rho) {
+    const int gt = gf_xcd_block() * TB + threadIdx.x;   // particles in order per XCD
+    const int t = gt / SL, q = gt % SL;
+    const bool ok = t < n;
+    const int tt = ok ? t : 0;
+    const double4 at = a4[tt];
+    const float ihv = rec[tt].k.y, s3 = rec[tt].d.z;
+    double s = 0.0;
+    if (ok) {
+        for (int jj = q; jj < k; jj += SL) {
+            const int j = nb[size_t(jj) * n + t];
+            const double4 aj = a4[j];
+            const float dx = float(at.x - aj.x), dy = float(at.y - aj.y), dz = float(at.z - aj.z);
+            const float r = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+            s += double(float(aj.w) * kernel_w32(r, ihv, s3));
+        }
+    }
+    s = lanes_sum(s);
+    if (!ok || q != 0) return;
+    rho[t] = s;
+    rec[t].k.w = float(s);
+    const double P = (gamma - 1) * s * u[t];
+    rec[t].d.x = float(P / (s * s));
+}
+
+__global__ __launch_bounds__(TB) void k_force32(int n, const int* __restrict__ nb, int k,
+                                                const double4* __restrict__ a4,
+                                                const SphRec32* __restrict__ rec,
+                                                const double* __restrict__ rho_d,
+                                                const double* __restrict__ h_d, double alpha_d,
+                                                double beta_d, const uint32_t* __restrict__ out_idx,
+                                                double* __restrict__ acc, double* __restrict__ dudt,
+                                                double* __restrict__ dt, double* __restrict__ divv,
+                                                double* __restrict__ rho_out,
+                                                double* __restrict__ h_out) {
+    const int gt = gf_xcd_block() * TB + threadIdx.x;   // particles in order per XCD
+    const int i = gt / SL, q = gt % SL;
+    const bool ok = i < n;
+    const int ii = ok ? i : 0;
+    const double4 ai = a4[ii];
+    const SphRec32 ri = rec[ii];
+    const float alpha = float(alpha_d), beta = float(beta_d);
+    const float vxi = ri.v.x, vyi = ri.v.y, vzi = ri.v.z, ci = ri.v.w;
+    const float hi_ = ri.k.x, ihi = ri.k.y, sgi = ri.k.z, rhoi = ri.k.w;
+    const float pi2 = ri.d.x;
+    double ax = 0, ay = 0, az = 0, du = 0, dv = 0;
+    float mu_max = 0.0f;
+    if (ok) {
+        for (int jj = q; jj < k; jj += SL) {
+            const int j = nb[size_t(jj) * n + i];
+            if (j == i) continue;
+            const double4 aj = a4[j];
+            const SphRec32 rj = rec[j];
+            const float dx = float(ai.x - aj.x), dy = float(ai.y - aj.y), dz = float(ai.z - aj.z);
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float r = __builtin_sqrtf(r2);
+            const float ir = __builtin_amdgcn_rcpf(fmaxf(r, 1e-30f));
+            const float gi = kernel_g32(r, ihi, sgi) * ir;
+            const float gj = kernel_g32(r, rj.k.y, rj.k.z) * ir;
+            const float vr = (vxi - rj.v.x) * dx + (vyi - rj.v.y) * dy + (vzi - rj.v.z) * dz;
+            const float hb = 0.5f * (hi_ + rj.k.x);
+            const float mu = hb * vr * __builtin_amdgcn_rcpf(r2 + 0.01f * hb * hb);
+            const float vis = (-alpha * 0.5f * (ci + rj.v.w) * mu + beta * mu * mu) *
+                              (2.0f * __builtin_amdgcn_rcpf(rhoi + rj.k.w));
+            const bool app = vr < 0.0f;             // viscosity only for approaching pairs
+            const float visc = app ? vis : 0.0f;
+            mu_max = app ? fmaxf(mu_max, -mu) : mu_max;
+            const float gb = 0.5f * (gi + gj);
+            const float f = pi2 * gi + rj.d.x * gj + visc * gb;
+            const float mj = rj.d.y;
+            const float mf = mj * f;
+            ax -= double(mf * dx);
+            ay -= double(mf * dy);
+            az -= double(mf * dz);
+            du += double(mj * (pi2 * gi + 0.5f * visc * gb) * vr);
+            dv += double(mj * vr * gi);
+        }
+    }
+    ax = lanes_sum(ax);
+    ay = lanes_sum(ay);
+    az = lanes_sum(az);
+    du = lanes_sum(du);
+    dv = lanes_sum(dv);
+    const double mm = lanes_max(double(mu_max));
+    if (!ok || q != 0) return;
+    const size_t o = out_idx ? size_t(out_idx[i]) : size_t(i);
+    const double rho_i = rho_d[i], hd = h_d[i], cd = double(ci);
+    if (acc) {
+        acc[3 * o] = ax;
+        acc[3 * o + 1] = ay;
+        acc[3 * o + 2] = az;
+    }
+    if (dudt) dudt[o] = du;
+    if (dt) dt[o] = 0.3 * hd / (cd + 1.2 * (alpha_d * cd + beta_d * mm) + 1e-30);
+    if (divv) divv[o] = -dv / rho_i;
+    if (rho_out) rho_out[o] = rho_i;
+    if (h_out) h_out[o] = hd;
+}
+
 __global__ __launch_bounds__(TB) void k_smoothing(int n, const double* __restrict__ dk, double eps,
                                                   double* __restrict__ h) {
 #pragma clang fp contract(off)
@@ -1606,6 +1749,19 @@ int gf_sph_forces_tree(GfSph& S, int k, double gamma, double alpha, double beta,
     if (n == 0) return 0;
     SphView v{S.tree.tx, S.tree.ty, S.tree.tz, S.vx, S.vy, S.vz, S.tree.tm, S.u, S.h, n};
     const int g = gf_div_up(n, TB);
+    if (S.pair_f32) {
+        static_assert(sizeof(SphRec32) <= 16 * sizeof(double), "record space");
+        SphRec32* rec = reinterpret_cast<SphRec32*>(S.rec);
+        double4* a4 = reinterpret_cast<double4*>(S.p4);  // free after the solve
+        hipLaunchKernelGGL(k_pack32, dim3(g), dim3(TB), 0, s, v, gamma, rec, a4);
+        const int gl = gf_div_up((long long)n * SL, TB);
+        hipLaunchKernelGGL(k_density32, dim3(gl), dim3(TB), 0, s, n, S.nb, k, S.u, gamma, a4, rec,
+                           S.rho);
+        hipLaunchKernelGGL(k_force32, dim3(gl), dim3(TB), 0, s, n, S.nb, k, a4, rec, S.rho, S.h,
+                           alpha, beta, S.tree.idx, acc, dudt, dt, divv, rho, h_out);
+        GF_LAUNCH_CHECK();
+        return 0;
+    }
     SphRec* rec = reinterpret_cast<SphRec*>(S.rec);
     double4* a4 = reinterpret_cast<double4*>(S.p4);      // free after the solve
     hipLaunchKernelGGL(k_pack, dim3(g), dim3(TB), 0, s, v, gamma, rec, a4);

@@ -30,7 +30,7 @@ GAMMA, ALPHA, BETA = 5.0 / 3.0, 1.0, 2.0
 
 class ForceEvaluator:
     def __init__(self, n, n_gas, eps, theta=0.5, k=64, precision="fp32", device="cuda:0",
-                 stream=None, quad_relative_tolerance=None):
+                 stream=None, quad_relative_tolerance=None, sph_precision=None):
         import torch
         self.torch = torch
         self.L = _lib.load()
@@ -42,6 +42,9 @@ class ForceEvaluator:
         # fp32q: the relative quadrupole rule, fed by acc_grav, which keeps the
         # previous evaluation's gravity (None leaves the library's setting)
         self.quad_rel = quad_relative_tolerance
+        # density and forces: 0 FP64 pair terms, 1 FP32 pair terms with FP64
+        # sums (None leaves the library's setting)
+        self.sph_prec = sph_precision
         z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.dev)  # noqa: E731
         self.acc_grav = z(n, 3)
         self.acc = z(n, 3)
@@ -101,6 +104,8 @@ class ForceEvaluator:
                                              self.sp))
         if ev is not None:
             ev[4].record()
+        if self.sph_prec is not None:
+            chk(L.gf_set_sph_precision(int(self.sph_prec)))
         chk(L.gf_hydro_forces_dev(self.eps, GAMMA, ALPHA, BETA, self.h.data_ptr(),
                                   self.rho.data_ptr(), self.acc_sph.data_ptr(),
                                   self.dudt.data_ptr(), self.dt_c.data_ptr(),
@@ -227,7 +232,8 @@ class RungStepper:
     potential, the potential by the FP64 direct sum (gf_potential_dev)."""
 
     def __init__(self, particles, theta=0.5, k=64, precision="fp32", rungs=6, first="plain",
-                 model=None, device="cuda:0", refit=False, quad_relative_tolerance=None):
+                 model=None, device="cuda:0", refit=False, quad_relative_tolerance=None,
+                 sph_precision=None):
         import torch
         self.torch = torch
         dev = torch.device(device)
@@ -244,7 +250,8 @@ class RungStepper:
         self.refit = refit
         self.first, self.model = first, model
         self.F = ForceEvaluator(p.n, self.ng, p.eps, theta, k, precision, device,
-                                quad_relative_tolerance=quad_relative_tolerance)
+                                quad_relative_tolerance=quad_relative_tolerance,
+                                sph_precision=sph_precision)
         self.L = self.F.L
         self.phi = torch.zeros(p.n, dtype=torch.float64, device=dev)
         self.rung = torch.zeros(p.n, dtype=torch.int64, device=dev)

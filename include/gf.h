@@ -85,6 +85,13 @@ double gf_get_quad_tolerance(void);
  * gf_set_quad_tolerance rule.  tol in [0, 1]. */
 int gf_set_quad_relative_tolerance(double tol);
 double gf_get_quad_relative_tolerance(void);
+/* Pair arithmetic of the device-resident density and forces
+ * (gf_hydro_forces_dev; sph.py:57-106 is FP64 under Numba fastmath): 0 = FP64
+ * (the default), 1 = FP32 pair terms from FP64 separations, FP64 sums.  The
+ * neighbours and h do not depend on it; the density differs by about 1e-7
+ * relative.  The host-array sph_forces path (gf_sph_forces) stays FP64. */
+int gf_set_sph_precision(int precision);
+int gf_get_sph_precision(void);
 /* Scheduling, no counterpart in the reference (sph.py:165-166 calls a k-d
  * tree): the number of persistent blocks of the fast neighbour pass.
  * 0 = the default, 1024: every resident slot of the pass (4 blocks of 4

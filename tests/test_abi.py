@@ -84,3 +84,14 @@ def test_engine_rejects_misshapen_arrays():
         e.gravity(pos, mass, np.zeros((10, 3)), 0.5, 0.01, active=np.ones(9, np.uint8))
     with pytest.raises(ValueError):
         e.gravity_direct(np.zeros((10, 2)), mass, 0.01)
+
+
+def test_sph_precision_setting_is_validated():
+    """SPH pair precision: 0 (FP64) by default, 1 = FP32 pair terms; other
+    values are refused and leave the setting (no GPU needed)."""
+    L = _lib.load()
+    assert L.gf_get_sph_precision() == 0
+    assert L.gf_set_sph_precision(2) < 0 and L.gf_set_sph_precision(-1) < 0
+    assert L.gf_get_sph_precision() == 0
+    assert L.gf_set_sph_precision(1) == 0 and L.gf_get_sph_precision() == 1
+    assert L.gf_set_sph_precision(0) == 0 and L.gf_get_sph_precision() == 0

@@ -191,6 +191,48 @@ def test_hydro_pipeline_equals_oracle():
                                atol=1e-12 * np.abs(odivv).max())
 
 
+def test_hydro_pipeline_fp32_pairs_close_to_oracle():
+    """The device pipeline with FP32 pair arithmetic (gf_set_sph_precision(1),
+    the fast path's density and forces) against the oracle's FP64 kNN + h +
+    sph_forces: h bit-equal (the neighbours do not depend on it), density
+    within 1e-6 relative (the gate against the plain solve is 1e-4), forces,
+    du/dt and div v within 2e-5 of their largest magnitude, Courant steps
+    within 1e-5 relative."""
+    import torch
+
+    from galaxy_formation_amd import _lib
+    pos, vel, mass, u, eps = disk_gas(20000, seed=4)
+    n, k = len(pos), 64
+    dev = torch.device("cuda:0")
+    t = {name: torch.from_numpy(a).to(dev) for name, a in
+         (("pos", pos), ("vel", vel), ("mass", mass), ("u", u))}
+    out = {name: torch.zeros(shape, dtype=torch.float64, device=dev) for name, shape in
+           (("h", n), ("rho", n), ("acc", (n, 3)), ("dudt", n), ("dt", n), ("divv", n))}
+    torch.cuda.synchronize()
+    L = _lib.load()
+    _lib.check(L.gf_set_sph_precision(1))
+    try:
+        _lib.check(L.gf_hydro_dev(n, t["pos"].data_ptr(), t["vel"].data_ptr(),
+                                  t["mass"].data_ptr(), t["u"].data_ptr(), eps, k, None,
+                                  GAMMA, sph.ALPHA, sph.BETA,
+                                  *[out[x].data_ptr() for x in ("h", "rho", "acc", "dudt",
+                                                                "dt", "divv")], None))
+        _lib.check(L.gf_sync(None))
+    finally:
+        _lib.check(L.gf_set_sph_precision(0))
+    assert L.gf_get_sph_precision() == 0
+    od, onb = O.knn(pos, k)
+    h = O.smoothing_length(od[:, -1], eps)
+    orho, oacc, odudt, odt, odivv = O.sph_forces(pos, vel, mass, onb, h, u, GAMMA, sph.ALPHA,
+                                                 sph.BETA)
+    np.testing.assert_array_equal(out["h"].cpu().numpy(), h)
+    np.testing.assert_allclose(out["rho"].cpu().numpy(), orho, rtol=1e-6)
+    for name, ref in (("acc", oacc), ("dudt", odudt), ("divv", odivv)):
+        got = out[name].cpu().numpy()
+        assert np.abs(got - ref).max() <= 2e-5 * np.abs(ref).max(), name
+    np.testing.assert_allclose(out["dt"].cpu().numpy(), odt, rtol=1e-5)
+
+
 def test_special_cases():
     """Closed-form cases: u = v = 0 gives zero force; a receding pair has no
     viscosity; the interior of a uniform lattice has rho ~ rho_0."""
