@@ -40,6 +40,8 @@ SIGNATURES = {
     "gf_get_sph_precision": (_ci, []),
     "gf_set_knn_blocks": (_ci, [_ci]),
     "gf_get_knn_blocks": (_ci, []),
+    "gf_set_walk_blocks": (_ci, [_ci]),
+    "gf_get_walk_blocks": (_ci, []),
     "gf_get_precision": (_ci, []),
     "gf_last_interactions": (_cll, []),
     "gf_accel_direct_subset": (_ci, [_ci, _dbl, _dbl, _ci, _vp, _dbl, _cd]),

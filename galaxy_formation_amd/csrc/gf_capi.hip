@@ -158,6 +158,19 @@ int gf_set_knn_blocks(int blocks) {
 
 int gf_get_knn_blocks(void) { return gf_sph_fast_blocks(g_ctx.sph); }
 
+int gf_set_walk_blocks(int blocks) {
+    if (blocks < 0 || blocks > 65536) {
+        gf_set_error("walk blocks must be in [0, 65536] (0 = default)");
+        return -1;
+    }
+    g_ctx.walk.walk_blocks = blocks;
+    return 0;
+}
+
+int gf_get_walk_blocks(void) {
+    return g_ctx.walk.walk_blocks > 0 ? g_ctx.walk.walk_blocks : gf_walk_default_blocks();
+}
+
 int gf_set_test_limits(int walk_stack, int knn_list) {
     if (walk_stack < 0 || walk_stack > 1024 || knn_list < 0 || knn_list > 1024) {
         gf_set_error("test limits: walk_stack and knn_list must be in [0, 1024] (0 = default)");

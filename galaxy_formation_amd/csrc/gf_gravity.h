@@ -25,9 +25,13 @@ struct GfWalkScratch {
     // hold on entry (the caller's previous accelerations) is evaluated as a
     // monopole; groups without that history use quad_tol
     float quad_rel_tol = 0.0f;
+    // persistent one-wave blocks of the walk (0 = the default, 8192)
+    int walk_blocks = 0;
 };
 
 int gf_walk_reserve(GfWalkScratch& W, int max_groups);
+// The walk's persistent grid when walk_blocks is 0.
+int gf_walk_default_blocks();
 void gf_walk_free(GfWalkScratch& W);
 
 // Accelerations of the active particles (active: input order, may be null =

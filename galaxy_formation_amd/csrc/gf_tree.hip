@@ -399,9 +399,32 @@ __global__ __launch_bounds__(TB) void k_level_com(
     for (int k = b0 + blockIdx.x * TB + threadIdx.x; k < b1; k += gridDim.x * TB) {
         const int id = dlist[k];
         double M = 0, sx = 0, sy = 0, sz = 0, dm = 0;
-        if (leaf[id]) {
-            const int b = start[id], e = b + count[id];
-            for (int i = b; i < e; ++i) {
+        const bool lf = leaf[id] != 0;
+        const int b = start[id], cnt = count[id];
+        // a leaf of at most GF_LEAF_SIZE particles, and a node's children,
+        // are loaded in one round (slots past the count read an in-range
+        // entry that is not used), then summed in their order
+        double pm[GF_LEAF_SIZE], px[GF_LEAF_SIZE], py[GF_LEAF_SIZE], pz[GF_LEAF_SIZE];
+        const bool small = lf && cnt <= GF_LEAF_SIZE;
+        if (small) {
+#pragma unroll
+            for (int u = 0; u < GF_LEAF_SIZE; ++u) {
+                const int i = b + (u < cnt ? u : 0);
+                pm[u] = tm[i];
+                px[u] = tx[i];
+                py[u] = ty[i];
+                pz[u] = tz[i];
+                const double dd = tdisp[i];
+                if (u < cnt) {
+                    M += pm[u];
+                    sx += pm[u] * px[u];
+                    sy += pm[u] * py[u];
+                    sz += pm[u] * pz[u];
+                    dm = fmax(dm, dd);
+                }
+            }
+        } else if (lf) {
+            for (int i = b; i < b + cnt; ++i) {
                 const double m = tm[i];
                 M += m;
                 sx += m * tx[i];
@@ -410,16 +433,25 @@ __global__ __launch_bounds__(TB) void k_level_com(
                 dm = fmax(dm, tdisp[i]);
             }
         } else {
-            const int* slots = child8 + 8 * size_t(id);
+            const int4* sl = reinterpret_cast<const int4*>(child8 + 8 * size_t(id));
+            const int4 s0 = sl[0], s1 = sl[1];
+            const int cs[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+            double4 cm[8];
+            double nd[8];
+#pragma unroll
             for (int o = 0; o < 8; ++o) {
-                const int c = slots[o];
-                if (c < 0) continue;
-                const double4 cm = com[c];
-                M += cm.w;
-                sx += cm.w * cm.x;
-                sy += cm.w * cm.y;
-                sz += cm.w * cm.z;
-                dm = fmax(dm, ndisp[c]);
+                const int c = cs[o] >= 0 ? cs[o] : id;
+                cm[o] = com[c];
+                nd[o] = ndisp[c];
+            }
+#pragma unroll
+            for (int o = 0; o < 8; ++o) {
+                if (cs[o] < 0) continue;
+                M += cm[o].w;
+                sx += cm[o].w * cm[o].x;
+                sy += cm[o].w * cm[o].y;
+                sz += cm[o].w * cm[o].z;
+                dm = fmax(dm, nd[o]);
             }
         }
         const double4 c = finalize_node(id, M, sx, sy, sz, dm, theta_d, box, com, open_r, ndisp);
@@ -427,9 +459,12 @@ __global__ __launch_bounds__(TB) void k_level_com(
             // the traceless quadrupole about the new centre of mass, as
             // k_level_quad forms it (same terms, same order)
             double q[6] = {0, 0, 0, 0, 0, 0};
-            if (leaf[id]) {
-                const int b = start[id], e = b + count[id];
-                for (int i = b; i < e; ++i) quad_add(q, tm[i], tx[i] - c.x, ty[i] - c.y, tz[i] - c.z);
+            if (small) {
+#pragma unroll
+                for (int u = 0; u < GF_LEAF_SIZE; ++u)
+                    if (u < cnt) quad_add(q, pm[u], px[u] - c.x, py[u] - c.y, pz[u] - c.z);
+            } else if (lf) {
+                for (int i = b; i < b + cnt; ++i) quad_add(q, tm[i], tx[i] - c.x, ty[i] - c.y, tz[i] - c.z);
             } else {
                 const int* slots = child8 + 8 * size_t(id);
                 for (int o = 0; o < 8; ++o) {

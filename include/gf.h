@@ -101,6 +101,13 @@ int gf_get_sph_precision(void);
  * do not depend on it.  blocks in [0, 65536]. */
 int gf_set_knn_blocks(int blocks);
 int gf_get_knn_blocks(void);
+/* Scheduling, no counterpart in the reference (nbody.cpp:323-411 walks with
+ * OpenMP threads): the number of persistent one-wave blocks of the gravity
+ * walk, which draws groups from a queue.  0 = the default, 4096 (one per
+ * resident slot of the walk on MI355X).  A smaller grid leaves room on each
+ * CU for work on another stream.  blocks in [0, 65536]. */
+int gf_set_walk_blocks(int blocks);
+int gf_get_walk_blocks(void);
 /* Testing hook, no counterpart in the reference: lowers the capacity of
  * the gravity walk's LDS stack (walk_stack entries, 1..1024) and of the
  * neighbour solve's candidate lists (knn_list, 1..1024) so that tests can

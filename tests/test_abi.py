@@ -95,3 +95,13 @@ def test_sph_precision_setting_is_validated():
     assert L.gf_get_sph_precision() == 0
     assert L.gf_set_sph_precision(1) == 0 and L.gf_get_sph_precision() == 1
     assert L.gf_set_sph_precision(0) == 0 and L.gf_get_sph_precision() == 0
+
+
+def test_walk_blocks_setting_is_validated():
+    """The walk's persistent grid: default 4096, 0 restores it, values
+    outside [0, 65536] are refused (no GPU needed)."""
+    L = _lib.load()
+    assert L.gf_get_walk_blocks() == 4096
+    assert L.gf_set_walk_blocks(-1) < 0 and L.gf_set_walk_blocks(65537) < 0
+    assert L.gf_set_walk_blocks(3072) == 0 and L.gf_get_walk_blocks() == 3072
+    assert L.gf_set_walk_blocks(0) == 0 and L.gf_get_walk_blocks() == 4096
