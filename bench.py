@@ -45,6 +45,9 @@ PEAK_FP64 = 78.6e12
 # rocprofv3 --pmc (profiles/); counters cannot be read inside a run, so the
 # measured figures are carried here
 WALK_TRAFFIC_BYTES = {0: 3527188960, 2: 3668275230}
+# the same for one evaluation's FP32-pair SPH sums on the 1M disk's gas
+# (k_pack32 + k_density32 + k_force32)
+SPH_TRAFFIC_BYTES = 641058784
 # the accuracy gates of the metric: RMS relative acceleration error vs the
 # FP64 direct sum, and density vs the plain h solve (with h equal to it)
 GATE_ACC_RMS = 1e-3
@@ -515,7 +518,7 @@ def main():
                      "flop_per_launch": flops,
                      "unpacked_fp32_peak": PEAK_FP32_UNPACKED / 1e12,
                      "frac_of_unpacked": achieved / PEAK_FP32_UNPACKED},
-        "roofline_sph": sph_roofline(ng, a.k, fast["serial"]["sph_ms"]),
+        "roofline_sph": sph_roofline(ng, a.k, fast["serial"]["sph_ms"], a.workload),
         # the quadrupole walk: node entries cost 52 flop (20 + Q.d, d.Q.d and
         # the two extra terms), particle entries 20
         "roofline_monopole_walk": {"kernel": "k_walk<false, false, false>", "launch_ms": walk_ms,
@@ -670,16 +673,17 @@ def accuracy_table(B):
     return rows
 
 
-def sph_roofline(ng, k, ms):
-    """The density and force passes against HBM: algorithmic bytes per gas
-    particle 4k + 72 (density) and 4k + 120 (force): the int32 neighbour
-    column entries, the particle's own state and outputs; neighbour records
-    counted once per particle, as if cached."""
+def sph_roofline(ng, k, ms, workload):
+    """The density and force passes (FP32 pair terms, the fast paths) against
+    HBM: algorithmic bytes per gas particle 4k + 72 (density) and 4k + 120
+    (force): the int32 neighbour column entries, the particle's own state and
+    outputs; neighbour records counted once per particle, as if cached."""
     b = ng * ((4 * k + 72) + (4 * k + 120))
     achieved = b / (ms * 1e-3)
     return {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-            "frac": achieved / PEAK_HBM, "traffic": None, "bytes_algorithmic": b,
-            "kernels": "k_pack + k_density + k_force", "ms": ms}
+            "frac": achieved / PEAK_HBM,
+            "traffic": SPH_TRAFFIC_BYTES if workload == "disk1m" and k == 64 else None,
+            "bytes_algorithmic": b, "kernels": "k_pack32 + k_density32 + k_force32", "ms": ms}
 
 
 def cpu_baseline(B):
