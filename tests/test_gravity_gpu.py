@@ -429,3 +429,26 @@ def test_quad_relative_tolerance_orders_the_quadrupole_count():
         e.quad_relative_tolerance = 0.0
     assert all(x > y for x, y in zip(q, q[1:])), q
     assert r[-1] > r[0]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32q"])
+def test_walk_grid_does_not_change_results(precision):
+    """The walk draws groups from a queue and each group's list and sums are
+    its own, so the persistent grid (gf_set_walk_blocks) changes nothing:
+    bit-identical accelerations and the same interaction count for the
+    default grid, a quarter of it and a grid that is not a multiple of the
+    CU count."""
+    from galaxy_formation_amd import _lib
+    from galaxy_formation_amd.engine import Engine
+    L = _lib.load()
+    e = Engine(precision)
+    c = ics.config("cloud10k")
+    base = run(e, c.pos, c.mass, 0.5, c.eps, precision)
+    i0 = e.last_interactions()
+    try:
+        for blocks in (1024, 257):
+            _lib.check(L.gf_set_walk_blocks(blocks))
+            np.testing.assert_array_equal(run(e, c.pos, c.mass, 0.5, c.eps, precision), base)
+            assert e.last_interactions() == i0
+    finally:
+        _lib.check(L.gf_set_walk_blocks(0))
