@@ -36,8 +36,9 @@ struct GfSph {
 
 void gf_sph_free(GfSph& S);
 
-// Persistent blocks the fast neighbour pass launches (S.fast_blocks, or the
-// default of one per resident slot of the narrow shape: 1024).
+// Persistent blocks the fast neighbour pass launches from a caller's first
+// radius (S.fast_blocks, or the default of one per resident slot of the
+// narrow shape: 1280).  From the cell densities the default is 1024.
 int gf_sph_fast_blocks(const GfSph& S);
 
 // Builds the gas tree and gathers velocities and energies into tree order.

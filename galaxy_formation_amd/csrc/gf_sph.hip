@@ -380,21 +380,32 @@ constexpr int FT = 256;
 // member: a 512-entry list, 4 blocks per CU, and the union of the member
 // spheres gathered first.  The cell-density start overshoots: a 1024-entry
 // list and the bounding sphere first (fewer shrink passes).
+// five narrow blocks per CU (31 KB of LDS each, at most 102 VGPRs): the pass
+// waits on memory most of its time, and the fifth block hides more of it
+// (1.55 -> 1.39 ms alone on the 500k gas of the 1M disk)
+#ifndef KNN_NARROW_BLOCKS_PER_CU
+#define KNN_NARROW_BLOCKS_PER_CU 5
+#endif
 struct KnnNarrow {
-    static constexpr int CMAX = 512, BLOCKS_PER_CU = 4, UNION_FIRST = 1;
+    static constexpr int CMAX = 512, BLOCKS_PER_CU = KNN_NARROW_BLOCKS_PER_CU, UNION_FIRST = 1;
 };
 struct KnnWide {
     static constexpr int CMAX = 1024, BLOCKS_PER_CU = 3, UNION_FIRST = 0;
 };
 constexpr int CMAX_MAX = 1024;               // largest candidate list (test limits)
 constexpr int FSCAP = 1024;                  // stack of the gather walk
-// persistent blocks of the fast pass: the narrow shape's residency (4 per CU
-// on 256 CUs); more only queue behind them and hold back the gravity walk
+// persistent blocks of the fast pass by default: the narrow shape's
+// residency (5 per CU on 256 CUs), and 1024 for the wide shape (3 per CU);
+// more only queue behind the resident ones and hold back the gravity walk
 // that runs beside the solve on another stream
 #ifndef KNN_FAST_GRID
-#define KNN_FAST_GRID 1024
+#define KNN_FAST_GRID (256 * KNN_NARROW_BLOCKS_PER_CU)
+#endif
+#ifndef KNN_WIDE_GRID
+#define KNN_WIDE_GRID 1024
 #endif
 constexpr int FAST_BLOCKS = KNN_FAST_GRID;
+constexpr int WIDE_BLOCKS = KNN_WIDE_GRID;
 constexpr uint32_t KEY_NONE = 0xFFFFFFFFu;
 constexpr double KEY_SCALE = 4294967294.0;   // keys d^2 / r^2 * KEY_SCALE < KEY_NONE
 constexpr double GROUP_REACH = 3.0;          // gather once while R <= 3 max r_i
@@ -1617,6 +1628,8 @@ int sph_reserve(GfSph& S, int n, int k) {
 
 int gf_sph_fast_blocks(const GfSph& S) { return S.fast_blocks > 0 ? S.fast_blocks : FAST_BLOCKS; }
 
+static int wide_blocks(const GfSph& S) { return S.fast_blocks > 0 ? S.fast_blocks : WIDE_BLOCKS; }
+
 void gf_sph_free(GfSph& S) {
     gf_tree_free(S.tree);
     dfree(S.vx); dfree(S.vy); dfree(S.vz); dfree(S.u); dfree(S.r); dfree(S.dk); dfree(S.h);
@@ -1703,7 +1716,8 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
             hipLaunchKernelGGL(k_knn_reset, dim3(1), dim3(64), 0, s, S.state, pass == 0 ? 1 : 0);
             if (!sorted)
                 hipLaunchKernelGGL(r_init ? k_knn_fast<KnnNarrow> : k_knn_fast<KnnWide>,
-                                   dim3(gf_sph_fast_blocks(S)), dim3(FT), 0, s, a);
+                                   dim3(r_init ? gf_sph_fast_blocks(S) : wide_blocks(S)),
+                                   dim3(FT), 0, s, a);
             if (kq <= 64) {
                 hipLaunchKernelGGL((k_knn_pass<64, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
                 hipLaunchKernelGGL((k_knn_pass<64, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);

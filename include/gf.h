@@ -94,8 +94,9 @@ int gf_set_sph_precision(int precision);
 int gf_get_sph_precision(void);
 /* Scheduling, no counterpart in the reference (sph.py:165-166 calls a k-d
  * tree): the number of persistent blocks of the fast neighbour pass.
- * 0 = the default, 1024: every resident slot of the pass (4 blocks of 4
- * waves per CU), fastest when the solve has the GPU to itself.  A solve
+ * 0 = the default: 1280 from a caller's first radius, every resident slot
+ * of that pass (5 blocks of 4 waves per CU), fastest when the solve has the
+ * GPU to itself; 1024 from the cell densities (3 resident per CU).  A solve
  * run on a second stream beside the gravity walk leaves the walk more of
  * each CU with fewer (512: the measured step of bench.py).  The neighbours
  * do not depend on it.  blocks in [0, 65536]. */

@@ -43,13 +43,13 @@ def test_loader_declares_argtypes():
 
 
 def test_knn_blocks_setting_is_validated():
-    """The neighbour pass's block count: default 1024, 0 restores it,
+    """The neighbour pass's block count: default 1280, 0 restores it,
     negative values are refused (no GPU needed)."""
     L = _lib.load()
-    assert L.gf_get_knn_blocks() == 1024
+    assert L.gf_get_knn_blocks() == 1280
     assert L.gf_set_knn_blocks(-1) < 0
     assert L.gf_set_knn_blocks(512) == 0 and L.gf_get_knn_blocks() == 512
-    assert L.gf_set_knn_blocks(0) == 0 and L.gf_get_knn_blocks() == 1024
+    assert L.gf_set_knn_blocks(0) == 0 and L.gf_get_knn_blocks() == 1280
 
 
 def test_quad_relative_tolerance_setting_is_validated():
