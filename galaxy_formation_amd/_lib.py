@@ -78,6 +78,8 @@ SIGNATURES = {
     "gf_mlp_grad_dev": (_ci, [_ci, _vp, _vp, _vp, _cf, _vp, _vp, _vp]),
     "gf_mlp_train_step_dev": (_ci, [_ci, _vp, _vp, _vp, _vp, _vp, _ci, _cf, _cf, _cf, _cf, _cf,
                                     _vp, _vp]),
+    "gf_mlp_train_step_noalloc_dev": (_ci, [_ci, _vp, _vp, _vp, _vp, _vp, _ci, _cf, _cf, _cf, _cf,
+                                            _cf, _vp, _vp]),
     "gf_mlp_grad_host": (_ci, [_ci, np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS"), _f32,
                                _f32, _cf, _f32, _vp]),
     "gf_hydro_neighbours": (_ci, [_i64, _vp]),

@@ -690,6 +690,19 @@ int gf_mlp_train_step_dev(int n, const uint16_t* x, const float* target, float* 
                              g_ctx.d_mlp_part, gf_stream(stream));
 }
 
+// No gf_ctx_init here: the library context creates a stream, and this entry
+// point allocates nothing.  It runs on the caller's stream and device.
+int gf_mlp_train_step_noalloc_dev(int n, const uint16_t* x, const float* target, float* params,
+                                  float* m, float* v, int step, float lr, float beta1,
+                                  float beta2, float eps, float tau, float* loss, void* stream) {
+    if (step < 1) {
+        gf_set_error("gf_mlp_train_step_noalloc_dev: step counts from 1");
+        return -1;
+    }
+    return gf_mlp_train_step_noalloc(n, x, target, params, m, v, step, lr, beta1, beta2, eps, tau,
+                                     loss, gf_stream(stream));
+}
+
 int gf_mlp_features_dev(int n, int k, const double* dk_prev, const double* cell_rho,
                         const double* cell_count, const double* divv, const double* dtc_prev,
                         float dt, const float* norm, uint16_t* x, float* raw, void* stream) {

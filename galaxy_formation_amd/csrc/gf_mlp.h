@@ -18,6 +18,13 @@ int gf_mlp_grad(int n, int n_total, const uint16_t* x, const float* target, cons
 int gf_mlp_train_step(int n, const uint16_t* x, const float* target, float* params, float* m,
                       float* v, int step, float lr, float b1, float b2, float eps, float tau,
                       float* loss, float* partial, hipStream_t s);
+// Samples per call of gf_mlp_train_step_noalloc (its scratch is static).
+#define GF_MLP_NOALLOC_MAX 65536
+// gf_mlp_train_step with its partials in the library's static device array
+// instead of a caller's buffer; calls no allocator.  n <= GF_MLP_NOALLOC_MAX.
+int gf_mlp_train_step_noalloc(int n, const uint16_t* x, const float* target, float* params,
+                              float* m, float* v, int step, float lr, float b1, float b2,
+                              float eps, float tau, float* loss, hipStream_t s);
 int gf_mlp_features(int n, int k, const double* dk_prev, const double* cell_rho,
                     const double* cell_count, const double* divv, const double* dtc_prev,
                     float dt, const float* norm, uint16_t* x, float* raw, hipStream_t s);

@@ -289,6 +289,55 @@ int gf_mlp_train_step(int n, const uint16_t* x, const float* target, float* para
     return 0;
 }
 
+// ------------------------------------------- training step without allocation
+// The partial gradients and losses of gf_mlp_train_step_noalloc live in this
+// file-scope array, sized for GF_MLP_NOALLOC_MAX samples (512 partials of
+// GF_MLP_PARAMS + 1 floats, 2.56 MB per device, held until the process
+// ends).  In host code `scratch` names a host-side stand-in, not device
+// memory: kernels receive the device address that hipGetSymbolAddress
+// returns, looked up once per device on that device's first call and kept.
+// Each call overwrites the partials of its own blocks before it reads them,
+// so nothing has to start at zero; calls that overlap on one device race.
+static __device__ __align__(256) float scratch[(GF_MLP_NOALLOC_MAX / GB) * (GF_MLP_PARAMS + 1)];
+static_assert(GF_MLP_NOALLOC_MAX % GB == 0, "whole gradient blocks");
+
+namespace {
+constexpr int kMaxDevices = 64;
+float* g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+int gf_mlp_train_step_noalloc(int n, const uint16_t* x, const float* target, float* params,
+                              float* m, float* v, int step, float lr, float b1, float b2,
+                              float eps, float tau, float* loss, hipStream_t s) {
+    if (n > GF_MLP_NOALLOC_MAX) {
+        gf_set_error("gf_mlp_train_step_noalloc_dev: %d samples exceed its scratch of %d", n,
+                     GF_MLP_NOALLOC_MAX);
+        return -1;
+    }
+    if (n <= 0) return 0;
+    int dev = 0;
+    GF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        gf_set_error("gf_mlp_train_step_noalloc_dev: device %d beyond the %d supported", dev,
+                     kMaxDevices);
+        return -1;
+    }
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        GF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            gf_set_error("gf_mlp_train_step_noalloc_dev: no device address for its scratch");
+            return -1;
+        }
+        g_scratch_dev[dev] = static_cast<float*>(p);
+    }
+    static_assert(sizeof(scratch) / sizeof(float) >=
+                      size_t(GF_MLP_NOALLOC_MAX / GB) * (GF_MLP_PARAMS + 1),
+                  "scratch holds the partials of GF_MLP_NOALLOC_MAX samples");
+    return gf_mlp_train_step(n, x, target, params, m, v, step, lr, b1, b2, eps, tau, loss,
+                             g_scratch_dev[dev], s);
+}
+
 int gf_mlp_features(int n, int k, const double* dk_prev, const double* cell_rho,
                     const double* cell_count, const double* divv, const double* dtc_prev,
                     float dt, const float* norm, uint16_t* x, float* raw, hipStream_t s) {

@@ -256,6 +256,17 @@ int gf_mlp_grad_dev(int n, const uint16_t* x, const float* target, const float* 
 int gf_mlp_train_step_dev(int n, const uint16_t* x, const float* target, float* params, float* m,
                           float* v, int step, float lr, float beta1, float beta2, float eps,
                           float tau, float* loss, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight training loops, harnesses that
+ * forbid hipMalloc): gf_mlp_train_step_dev with its partial gradients in a
+ * static device array of the library instead of a buffer it allocates.  It
+ * calls no allocator and does not create the library's stream; it runs on
+ * the caller's stream and current device.  n <= 65536 samples per call (more
+ * is refused with an error); the array holds 2.56 MB on each device that runs
+ * it until the process ends.  Calls that overlap on one device race. */
+int gf_mlp_train_step_noalloc_dev(int n, const uint16_t* x, const float* target, float* params,
+                                  float* m, float* v, int step, float lr, float beta1, float beta2,
+                                  float eps, float tau, float* loss, void* stream);
 /* gf_mlp_grad_dev with host arrays (x as uint16 BF16 bit patterns). */
 int gf_mlp_grad_host(int n, const uint16_t* x, const float* target, const float* params,
                      float tau, float* grad, float* loss);
