@@ -81,10 +81,16 @@ class ForceEvaluator:
             ev[2].record()
 
     def hydro(self, pos, vel, mass, u, first="plain", dk_prev=None, divv_prev=None,
-              dtc_prev=None, dt=None, model=None, ev=None):
-        """Gas of the first n_gas particles.  Returns the number of passes."""
+              dtc_prev=None, dt=None, model=None, ev=None, margin=0.0):
+        """Gas of the first n_gas particles.  Returns the number of passes.
+        first="prev" starts from d_k(prev) exp(margin)."""
         chk, L, ng = _lib.check, self.L, self.ng
-        if ng == 0:
+        if ng <= 40:
+            # simulation.py:78-80: no hydro for 40 gas particles or fewer
+            for t in (self.h, self.rho, self.acc_sph, self.dudt, self.divv, self.dk):
+                t.zero_()
+            self.dt_c.fill_(float("inf"))
+            self.passes = 0
             return 0
         chk(L.gf_hydro_setup_dev(ng, pos.data_ptr(), vel.data_ptr(), mass.data_ptr(),
                                  u.data_ptr(), self.sp))
@@ -92,7 +98,8 @@ class ForceEvaluator:
             ev[3].record()
         r0 = None
         if first == "prev":
-            r0 = dk_prev
+            r0 = dk_prev if margin == 0.0 else self.torch.mul(dk_prev, float(np.exp(margin)),
+                                                              out=self.r0)
         elif first == "learned":
             self.cell_density()
             x = model.features(self.k, dk_prev, self.cell_rho, self.cell_count, divv_prev,
@@ -124,17 +131,25 @@ class ForceEvaluator:
 
     def evaluate(self, pos, vel, mass, u, **hydro_kw):
         """acc = gravity + SPH; dt_ind (simulation.py:61-72)."""
-        torch = self.torch
         self.gravity(pos, mass)
         passes = self.hydro(pos[:self.ng], vel[:self.ng], mass[:self.ng], u[:self.ng], **hydro_kw)
+        self.combine()
+        return passes
+
+    def combine(self):
+        """The caller's sum and time steps (simulation.py:67-70): acc =
+        acc_grav + acc_sph (gas rows), dt_ind = min(0.3 sqrt(eps / max(|a|,
+        1e-30)), dt_courant) with dt_courant only on the gas, and dt_grav =
+        0.3 sqrt(eps / max(max |a|, 1e-30)) (a device scalar)."""
+        torch = self.torch
         self.acc.copy_(self.acc_grav)
         if self.ng:
             self.acc[:self.ng] += self.acc_sph
-        amag = torch.linalg.vector_norm(self.acc, dim=1)
+        amag = torch.sqrt(torch.sum(self.acc * self.acc, dim=1))
         self.dt_ind = 0.3 * torch.sqrt(self.eps / torch.clamp(amag, min=1e-30))
+        self.dt_grav = 0.3 * torch.sqrt(self.eps / torch.clamp(amag.max(), min=1e-30))
         if self.ng:
             self.dt_ind[:self.ng] = torch.minimum(self.dt_ind[:self.ng], self.dt_c)
-        return passes
 
 
 class Stepper:
@@ -278,15 +293,17 @@ class RungStepper:
         if self.ng:
             self.passes.append(F.hydro(self.pos[:self.ng], self.vel[:self.ng],
                                        self.mass[:self.ng], self.u[:self.ng], **kw))
-            self.prev = (F.dk.clone(), F.divv.clone(), F.dt_c.clone())
-            self.t_hydro = t
-        F.acc.copy_(F.acc_grav)
-        if self.ng:
-            F.acc[:self.ng] += F.acc_sph
-        amag = torch.linalg.vector_norm(F.acc, dim=1)
-        F.dt_ind = 0.3 * torch.sqrt(self.eps / torch.clamp(amag, min=1e-30))
-        if self.ng:
-            F.dt_ind[:self.ng] = torch.minimum(F.dt_ind[:self.ng], F.dt_c)
+            if self.ng > 40:
+                self.prev = (F.dk.clone(), F.divv.clone(), F.dt_c.clone())
+                self.t_hydro = t
+        F.combine()
+
+    def _ladder(self):
+        """dt_min = the 1st percentile of dt_ind (linear interpolation, FP64),
+        dt_max = dt_min 2^(rungs-1), and every particle's rung."""
+        self.dt_min = float(self.torch.quantile(self.F.dt_ind, 0.01).item())
+        self.dt_max = self.dt_min * 2 ** (self.R - 1)
+        self.rung = self._rung_of(self.F.dt_ind)
 
     def _rung_of(self, dt):
         torch = self.torch
@@ -335,9 +352,7 @@ class RungStepper:
         torch = self.torch
         self.t = 0.0
         self._forces(None, True, 0.0)
-        self.dt_min = float(torch.quantile(self.F.dt_ind.float(), 0.01).item())
-        self.dt_max = self.dt_min * 2 ** (self.R - 1)
-        self.rung = self._rung_of(self.F.dt_ind)
+        self._ladder()
         self.E0 = self.energy()
         all_ = torch.ones(self.n, dtype=torch.bool, device=self.dev)
         self._kick(all_, self.dt_max / 2.0 ** self.rung.double())
@@ -359,9 +374,7 @@ class RungStepper:
         if long_step:
             # every particle is at a boundary: a new ladder of steps
             self.L_sync.append(self.angular_momentum())
-            self.dt_min = float(torch.quantile(self.F.dt_ind.float(), 0.01).item())
-            self.dt_max = self.dt_min * 2 ** (R - 1)
-            self.rung = self._rung_of(self.F.dt_ind)
+            self._ladder()
         else:
             new = self._rung_of(self.F.dt_ind)
             # a longer step only where its own boundary is now

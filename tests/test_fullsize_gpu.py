@@ -105,3 +105,54 @@ def test_fp32_pair_sph_matches_fp64_on_the_1m_disk_gas(disk):
     np.testing.assert_array_equal(h32, h64)
     assert np.max(np.abs(rho32 - rho64) / rho64) < 1e-6
     assert np.abs(a32 - a64).max() < 1e-4 * np.abs(a64).max()
+
+
+def test_learned_radius_on_the_1m_disk_gas(disk):
+    """The learned first radius on the 500k gas of the 1M disk (the workload
+    of the learned-h configuration), with the committed network
+    (data/learned_h_k64.npz).  The previous state is the gas one step
+    earlier: drifted back by D, the 10th percentile of the admissible time
+    steps (the long step of simulation.py:117).  From the cell-density start
+    (the plain iteration), from d_k(prev) exp(fixed margin) and from the
+    learned proposal: h and the neighbour sets are bit-equal to the plain
+    solve and rho within 1e-4 of it (equal to rounding); the learned start
+    takes fewer passes than the plain iteration, and overshoots the
+    converged d_k by less than the fixed margin does."""
+    import os
+
+    from galaxy_formation_amd.learned_h import LearnedH
+    from galaxy_formation_amd.stepper import ForceEvaluator
+    torch, lib, c = disk["torch"], disk["lib"], disk["c"]
+    L = disk["L"]
+    ng = len(c.gas)
+    model = LearnedH().load(os.path.join(os.path.dirname(ics.__file__), "data",
+                                         "learned_h_k64.npz"))
+    F = ForceEvaluator(c.n, ng, c.eps, theta=0.5, k=64, precision="fp64", sph_precision=0)
+    pos, vel, mass, u = disk["pos"], disk["vel"], disk["mass"], disk["u"]
+    F.evaluate(pos, vel, mass, u)
+    D = float(torch.quantile(F.dt_ind, 0.10).item())
+    back = (pos[:ng] - D * vel[:ng]).contiguous()
+    F.hydro(back, vel[:ng], mass[:ng], u[:ng])
+    dk_prev, divv_prev, dtc_prev = F.dk.clone(), F.divv.clone(), F.dt_c.clone()
+    out = {}
+    for name, first, margin in (("plain", "plain", 0.0),
+                                ("fixed_margin", "prev", model.margin_fixed),
+                                ("learned", "learned", 0.0)):
+        print(f"disk1m gas neighbours: {name}", flush=True)
+        F.hydro(pos[:ng], vel[:ng], mass[:ng], u[:ng], first=first, dk_prev=dk_prev,
+                divv_prev=divv_prev, dtc_prev=dtc_prev, dt=D, model=model, margin=margin)
+        torch.cuda.synchronize()
+        nb = np.empty((ng, 64), np.int64)
+        lib.check(L.gf_hydro_neighbours(nb, None))
+        tot = np.zeros(1, np.int64)
+        lib.check(L.gf_hydro_passes(None, lib.ptr(tot)))
+        over = None if first == "plain" else float(torch.log(F.r0 / F.dk).mean().item())
+        out[name] = dict(h=F.h.cpu().numpy(), rho=F.rho.cpu().numpy(), nb=np.sort(nb, axis=1),
+                         mean_passes=tot[0] / ng, launches=F.passes, overshoot=over)
+        print({k: v for k, v in out[name].items() if k not in ("h", "rho", "nb")}, flush=True)
+    for name in ("fixed_margin", "learned"):
+        np.testing.assert_array_equal(out[name]["h"], out["plain"]["h"])
+        np.testing.assert_array_equal(out[name]["nb"], out["plain"]["nb"])
+        assert np.max(np.abs(out[name]["rho"] - out["plain"]["rho"]) / out["plain"]["rho"]) < 1e-4
+    assert out["learned"]["mean_passes"] < out["plain"]["mean_passes"]
+    assert out["learned"]["overshoot"] < out["fixed_margin"]["overshoot"]
