@@ -546,7 +546,7 @@ def gas_physics(B, launches=20, n_new=1000, k_fb=16):
     timed alone with HIP events on its stream:
       * thermodynamics of every gas row (simulation.py:97-104): one pass,
         48 B per row (g, u, du/dt, rho, no_cool_until read; u written);
-      * supernova feedback of n_new new stars (star_formation.py:52-68): the
+      * supernova feedback of n_new new stars (star_formation.py:50-65): the
         k_fb-nearest query against all gas, its tree build included (the
         reference builds a cKDTree per call), and the heating; scipy's
         cKDTree build + query on the host beside it, and whether the two

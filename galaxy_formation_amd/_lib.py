@@ -61,6 +61,10 @@ SIGNATURES = {
                             _vp, _vp, _vp, _vp, _vp]),
     "gf_knn": (_ci, [_ci, _dbl, _ci, _vp, _i64, _dbl, _vp]),
     "gf_knn_from_candidates_host": (_ci, [_ci, _ci, _dbl, _i64, _ci, _i64, _dbl]),
+    "gf_nbcache_search": (_ci, [_ci, _ci, _i64, _dbl, _ci, _ci, _i64, _dbl]),
+    "gf_nbcache_reselect": (_ci, [_ci, _ci, _i64, _dbl, _ci, _i64, _dbl]),
+    "gf_nbcache_info": (_ci, [_vp, _vp, _vp]),
+    "gf_nbcache_export": (_ci, [_vp, _vp]),
     "gf_hydro_dev": (_ci, [_ci, _vp, _vp, _vp, _vp, _cd, _ci, _vp, _cd, _cd, _cd,
                            _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_hydro_setup_dev": (_ci, [_ci, _vp, _vp, _vp, _vp, _vp]),

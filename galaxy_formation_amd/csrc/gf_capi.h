@@ -8,6 +8,7 @@
 #define GF_ABI_VERSION 1
 
 #include "gf_gas.h"
+#include "gf_verlet.h"
 
 struct GfCtx {
     int inited = 0;
@@ -42,6 +43,8 @@ struct GfCtx {
     float* d_mlp = nullptr; size_t d_mlp_cap = 0;
     float* d_mlp_part = nullptr; size_t d_mlp_part_cap = 0;
     GfGas gas;                      // gas-physics scratch (cooling, star formation, feedback)
+    GfVerlet verlet;                // sph.NeighbourFinder's candidate cache
+    int64_t* d_g = nullptr; size_t d_g_cap = 0;
 };
 
 extern GfCtx g_ctx;

@@ -378,6 +378,18 @@ int gf_sync(void* stream) {
 
 int nbody_render(int n, const double* pos, const uint8_t* group, int ngroups, const double* rot,
                  const double* center, double scale, int W, int H, float* img) {
+    if (n < 0 || ngroups < 1 || W < 1 || H < 1 || !rot || !center || !img ||
+        (n > 0 && (!pos || !group))) {
+        gf_set_error("nbody_render: bad sizes or null pointers (n %d, ngroups %d, %d x %d)", n,
+                     ngroups, W, H);
+        return -1;
+    }
+    for (int i = 0; i < n; ++i)          // a group outside the image planes would write past them
+        if (group[i] >= ngroups) {
+            gf_set_error("nbody_render: particle %d has group %d of %d", i, int(group[i]),
+                         ngroups);
+            return -1;
+        }
     if (gf_ctx_init()) return -1;
     hipStream_t s = g_ctx.stream;
     const size_t bimg = sizeof(float) * size_t(ngroups) * W * H;
@@ -403,6 +415,12 @@ int nbody_render(int n, const double* pos, const uint8_t* group, int ngroups, co
 
 int nbody_tonemap(int W, int H, int ngroups, const float* img, const float* glow, int gw, int gh,
                   const float* colors, float inv_ref, float K, float exposure, uint8_t* out) {
+    // the glow's bilinear stencil reads two cells along each axis
+    if (W < 1 || H < 1 || ngroups < 1 || gw < 2 || gh < 2 || !img || !glow || !colors || !out) {
+        gf_set_error("nbody_tonemap: bad sizes or null pointers (%d x %d, %d groups, glow %d x %d)",
+                     W, H, ngroups, gw, gh);
+        return -1;
+    }
     if (gf_ctx_init()) return -1;
     hipStream_t s = g_ctx.stream;
     const size_t bimg = sizeof(float) * size_t(ngroups) * W * H;
@@ -540,6 +558,94 @@ int gf_knn_from_candidates_host(int n, int m, const double* pos, const int64_t* 
     GF_HIP(hipMemcpyAsync(&ok, g_ctx.d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
     GF_HIP(hipStreamSynchronize(s));
     return ok;
+}
+
+// sph.NeighbourFinder (sph.py:138-170): full search into the device cache,
+// re-selection from it.
+
+static int check_gas_list(const char* who, int n_total, int n, const int64_t* g) {
+    if (n_total < 0 || n < 0 || n > n_total || (n > 0 && !g)) {
+        gf_set_error("%s: %d gas particles of %d", who, n, n_total);
+        return -1;
+    }
+    for (int i = 0; i < n; ++i)
+        if (g[i] < 0 || g[i] >= n_total) {
+            gf_set_error("%s: gas index %lld outside [0, %d)", who, (long long)g[i], n_total);
+            return -1;
+        }
+    return 0;
+}
+
+int gf_nbcache_search(int n_total, int n, const int64_t* g, const double* pos, int m, int k,
+                      int64_t* nb, double* dist) {
+    if (check_gas_list("gf_nbcache_search", n_total, n, g)) return -1;
+    if (n == 0) return 0;
+    if (m < 1 || m > n || m > GF_VERLET_MAX_CAND || k < 1 || k > m) {
+        gf_set_error("gf_nbcache_search: need 1 <= k = %d <= m = %d <= min(n = %d, %d)", k, m, n,
+                     GF_VERLET_MAX_CAND);
+        return -1;
+    }
+    if (gf_ctx_init()) return -1;
+    hipStream_t s = g_ctx.stream;
+    const size_t b1 = sizeof(double) * size_t(n), b3 = 3 * b1;
+    if (STAGE(g_ctx.d_pos, b3) || STAGE(g_ctx.d_mass, b1) || STAGE(g_ctx.d_g, sizeof(int64_t) * n) ||
+        STAGE(g_ctx.d_nb, sizeof(int64_t) * size_t(n) * m) ||
+        STAGE(g_ctx.d_dist, sizeof(double) * size_t(n) * m))
+        return -1;
+    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(g_ctx.d_g, g, sizeof(int64_t) * size_t(n), hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemsetAsync(g_ctx.d_mass, 0, b1, s));
+    GfSph& S = g_ctx.sph;
+    if (gf_sph_setup(S, n, g_ctx.d_pos, nullptr, g_ctx.d_mass, nullptr, s)) return -1;
+    if (gf_sph_knn(S, m, nullptr, false, true, 64, s)) return -1;
+    g_ctx.sph_k = 0;   // the held table holds input indices: not usable for forces
+    if (gf_sph_export_knn(S, m, g_ctx.d_nb, g_ctx.d_dist, s)) return -1;
+    if (gf_verlet_store(g_ctx.verlet, n_total, n, m, g_ctx.d_g, g_ctx.d_nb, s)) return -1;
+    // the first k columns of each row
+    GF_HIP(hipMemcpy2DAsync(nb, sizeof(int64_t) * k, g_ctx.d_nb, sizeof(int64_t) * m,
+                            sizeof(int64_t) * k, size_t(n), hipMemcpyDeviceToHost, s));
+    GF_HIP(hipMemcpy2DAsync(dist, sizeof(double) * k, g_ctx.d_dist, sizeof(double) * m,
+                            sizeof(double) * k, size_t(n), hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int gf_nbcache_reselect(int n_total, int n, const int64_t* g, const double* pos, int k,
+                        int64_t* nb, double* dist) {
+    if (check_gas_list("gf_nbcache_reselect", n_total, n, g)) return -1;
+    if (g_ctx.verlet.rows <= 0) return 0;
+    if (n == 0) return 1;
+    if (gf_ctx_init()) return -1;
+    hipStream_t s = g_ctx.stream;
+    const size_t b3 = sizeof(double) * 3 * size_t(n);
+    if (STAGE(g_ctx.d_pos, b3) || STAGE(g_ctx.d_g, sizeof(int64_t) * n) ||
+        STAGE(g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k) ||
+        STAGE(g_ctx.d_dist, sizeof(double) * size_t(n) * k) ||
+        STAGE(g_ctx.d_scratch, sizeof(int) * size_t(n)))
+        return -1;
+    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(g_ctx.d_g, g, sizeof(int64_t) * size_t(n), hipMemcpyHostToDevice, s));
+    const int ok = gf_verlet_reselect(g_ctx.verlet, n_total, n, k, g_ctx.d_g, g_ctx.d_pos,
+                                      g_ctx.d_scratch, g_ctx.d_nb, g_ctx.d_dist, s);
+    if (ok <= 0) return ok;
+    GF_HIP(hipMemcpyAsync(nb, g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipMemcpyAsync(dist, g_ctx.d_dist, sizeof(double) * size_t(n) * k, hipMemcpyDeviceToHost,
+                          s));
+    GF_HIP(hipStreamSynchronize(s));
+    return 1;
+}
+
+int gf_nbcache_info(int* n_total, int* rows, int* m) {
+    if (n_total) *n_total = g_ctx.verlet.n_total;
+    if (rows) *rows = g_ctx.verlet.rows;
+    if (m) *m = g_ctx.verlet.m;
+    return g_ctx.verlet.rows;
+}
+
+int gf_nbcache_export(int64_t* owner, int64_t* cand) {
+    if (g_ctx.verlet.rows <= 0) return 0;
+    if (gf_ctx_init()) return -1;
+    return gf_verlet_export(g_ctx.verlet, owner, cand, g_ctx.stream);
 }
 
 // device-resident hydro: gas tree, neighbour solve, h, density, forces

@@ -40,22 +40,22 @@ int gf_thermo_launch(GfGas& G, int ng, const int64_t* g, int n_total, double* u,
                      double dt, double u_per_K, double u_floor, hipStream_t s);
 // Rows skipped since the last call (waits; clears the count).
 int gf_gas_errors(GfGas& G, hipStream_t s);
-// star_formation.py:37-47: gas rows (0 .. ng-1, ascending) of dense, cold gas
+// star_formation.py:37-43: gas rows (0 .. ng-1, ascending) of dense, cold gas
 // and their probabilities p = 1 - exp(-sf_eff dt / t_ff); u is indexed
 // through g like gf_thermo_launch, rho by row.  Returns the count (waits).
 int gf_sf_candidates(GfGas& G, int ng, const int64_t* g, int n_total, const double* u,
                      const double* rho, double u_per_K, double rho_sf, double sf_eff, double dt,
                      int64_t* rows, double* p, hipStream_t s);
-// star_formation.py:48: the rows whose uniform draw is below p, in order.
+// star_formation.py:44: the rows whose uniform draw is below p, in order.
 // Returns the count (waits).
 int gf_sf_select(GfGas& G, int m, const int64_t* rows, const double* p, const double* uniform,
                  int64_t* out, hipStream_t s);
 // The k nearest of each query point among the source points (input indices,
 // ascending in (distance, index)), as the cKDTree query of
-// star_formation.py:62.  dist may be null.  Waits for the result.
+// star_formation.py:59.  dist may be null.  Waits for the result.
 int gf_knn_query(GfGas& G, int n_src, const double* src, int n_q, const double* q, int k,
                  int64_t* nb, double* dist, hipStream_t s);
-// star_formation.py:63-68: u[targets] += du in the order np.add.at applies
+// star_formation.py:60-65: u[targets] += du in the order np.add.at applies
 // the flattened (star, neighbour) list; no_cool_until[targets] = until.
 int gf_feedback_heat(GfGas& G, int n_new, int k, const int64_t* targets, const double* new_mass,
                      double e_per_g, double until, int n_total, const double* mass, double* u,

@@ -1,9 +1,9 @@
 // Gas physics beside the force path, on the gas rows of one GPU:
 //   * radiative cooling of a primordial gas (galaxy_formation/cooling.py:24-43)
 //     and the explicit thermodynamics of a sub-step (simulation.py:97-104);
-//   * the star-formation candidates (star_formation.py:37-48): dense, cold
+//   * the star-formation candidates (star_formation.py:37-44): dense, cold
 //     gas and its conversion probability;
-//   * the supernova feedback (star_formation.py:52-68): the fb_ngb nearest
+//   * the supernova feedback (star_formation.py:50-65): the fb_ngb nearest
 //     gas particles of each new star (an exact k-nearest query against a
 //     tree of the gas), then the heating, applied in the order np.add.at
 //     applies the flattened (star, neighbour) list, so a particle heated by
@@ -19,7 +19,7 @@ namespace {
 
 constexpr int TB = 256;
 constexpr double T_FLOOR = 1e4;     // cooling.py:21
-constexpr double T_SF_MAX = 3e4;    // star_formation.py:32
+constexpr double T_SF_MAX = 3e4;    // star_formation.py:31
 
 struct Units {
     double rho_cgs, x_h, m_h, myr, sec_per_myr, u_cgs, msun, msun_g;
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(TB) void k_thermo(int ng, const int64_t* __restrict
     u[j] = fmax(u1 - du, u_floor);
 }
 
-// Dense, cold rows (star_formation.py:40): flag[i] = 1.
+// Dense, cold rows (star_formation.py:39): flag[i] = 1.
 __global__ __launch_bounds__(TB) void k_sf_flag(int ng, const int64_t* __restrict__ g, int n_total,
                                                 const double* __restrict__ u,
                                                 const double* __restrict__ rho, double u_per_K,
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(TB) void k_sf_flag(int ng, const int64_t* __restric
 }
 
 // Rows in order and p = 1 - exp(-sf_eff dt / t_ff), t_ff = sqrt(3 pi / (32 rho))
-// (star_formation.py:43-44).
+// (star_formation.py:42-43).
 __global__ __launch_bounds__(TB) void k_sf_compact(int ng, const int* __restrict__ off,
                                                    const double* __restrict__ rho,
                                                    double neg_eff_dt, int64_t* __restrict__ rows,
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(64) void k_knn_query(QueryArgs a) {
 }
 
 // ------------------------------------------------------ feedback heating
-// du of each (star, neighbour) entry, star_formation.py:64-66:
+// du of each (star, neighbour) entry, star_formation.py:61-63:
 //   E = e_per_g m_star MSUN MSUN_G,  du = E / k / (m_target MSUN MSUN_G) / U_CGS,
 // and the sort pairs (target, entry).  Targets outside [0, n_total) are
 // counted in err and get a key past every particle.

@@ -334,6 +334,11 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
             }
         }
 
+        // interactions of the whole group, counted once it has finished: a
+        // group of several 64-particle chunks whose later chunk overflows the
+        // LDS stack is redone from its first chunk by the global-stack walk
+        unsigned long long g_len = 0, g_nodes = 0;
+        bool g_ovf = false;
         for (int c0 = 0; c0 < np; c0 += 64) {
             const int P = min(64, np - c0);
             // parts A (particles [0, Pa)) and B ([Pa, P), possibly empty):
@@ -547,6 +552,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                     atomicAdd(&a.ovf_count[1], 1);
                 }
                 __syncthreads();
+                g_ovf = true;
                 break;                    // the whole group is redone by the queued walk
             }
             if (ln > 0) {
@@ -588,10 +594,12 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 }
             }
             __syncthreads();
-            if (lane == 0) {
-                atomicAdd(a.inter, (unsigned long long)total_len * (unsigned long long)n_act);
-                atomicAdd(a.inter_nodes, (unsigned long long)total_nodes * (unsigned long long)n_act);
-            }
+            g_len += (unsigned long long)total_len * (unsigned long long)n_act;
+            g_nodes += (unsigned long long)total_nodes * (unsigned long long)n_act;
+        }
+        if (!g_ovf && lane == 0) {
+            atomicAdd(a.inter, g_len);
+            atomicAdd(a.inter_nodes, g_nodes);
         }
     }
 }

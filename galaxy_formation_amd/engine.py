@@ -97,8 +97,10 @@ class Engine:
         return int(self.lib.gf_last_interactions())
 
     def project(self, pos, group, n_groups, rot, scale, size):
-        """Particle counts per pixel and per colour group, shape
-        (n_groups, size, size) (engine.py:58-63)."""
+        """Bilinear deposit of the particles, seen through the rotation rot
+        (rows: screen x, screen y, depth) at `scale` pixels per unit, into
+        one size x size float32 plane per colour group; the screen is
+        centred on the origin (engine.py:58-63)."""
         img = np.empty((n_groups, size, size), np.float32)
         _lib.check(self.lib.nbody_render(len(group), np.ascontiguousarray(pos), group, n_groups,
                                          np.ascontiguousarray(rot), np.zeros(3), scale, size, size,
@@ -106,8 +108,10 @@ class Engine:
         return img
 
     def tonemap(self, img, glow, colors, inv_ref, K, exposure):
-        """8-bit RGB image from the per-group count maps and their (4x
-        smaller) glow maps (engine.py:65-72)."""
+        """The (H, W, 3) uint8 frame: each group's deposit plane on a log
+        scale plus its glow plane (any coarser gh x gw grid, sampled
+        bilinearly), weighted by the group's RGB colour and saturated by
+        1 - exp(-exposure x) (engine.py:65-72)."""
         ng, H, W = img.shape
         out = np.empty((H, W, 3), np.uint8)
         _lib.check(self.lib.nbody_tonemap(W, H, ng, img, np.ascontiguousarray(glow), glow.shape[2],

@@ -2,7 +2,7 @@
 length 3.5 kpc, mass 4e10 Msun, G = 1, as galaxy_formation/units.py:17-35).
 
 Particles are ordered gas first, then dark matter, as the reference's
-make_cloud does (galaxy_formation/initial_conditions.py:104-116).  Every
+make_cloud does (galaxy_formation/initial_conditions.py:41-45).  Every
 generator draws from numpy's PCG64 with a fixed seed, so the same call gives
 the same bytes on every machine with this numpy.
 """
@@ -48,7 +48,7 @@ def make_cloud(n_gas, n_dm, seed=3, r0_kpc=100.0, mass_msun=2.8e11, f_baryon=0.1
     """Rotating cloud: radii r = R sqrt(u) (rho ~ 1/r), isotropic directions,
     solid-body rotation about z with spin lambda (Peebles convention,
     J |E|^(1/2) / (G M^(5/2)) with E the potential energy, as
-    initial_conditions.py:121-123 uses it).  For rho ~ 1/r inside R,
+    initial_conditions.py:50-53 uses it for its uniform cloud).  For rho ~ 1/r inside R,
     I = M R^2 / 3 and E = -2 G M^2 / (3 R), so
     Omega = 3 lambda sqrt(G M) / (sqrt(2/3) R^(3/2))."""
     rng = np.random.default_rng(seed)

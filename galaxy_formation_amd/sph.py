@@ -1,9 +1,9 @@
 """Smoothed Particle Hydrodynamics of the gas, with the reference's module
 interface (galaxy_formation/sph.py): kernel, kernel_grad, sph_forces,
 knn_from_candidates, NeighbourFinder and the constants N_NEIGHBOURS,
-N_CANDIDATES, ALPHA, BETA.  The arithmetic runs in libgf.so on the GPU; the
-host code here only moves arrays and keeps the index bookkeeping of
-NeighbourFinder (sph.py:151-170).
+N_CANDIDATES, ALPHA, BETA.  The arithmetic runs in libgf.so on the GPU,
+NeighbourFinder's candidate cache included; the host code here only moves
+arrays.
 
 Density, forces and time step follow sph.py:57-106:
 
@@ -105,40 +105,63 @@ def knn(p, k, r_init=None):
     return d, nb, {"passes": int(passes), "mean_passes": float(mean[0])}
 
 
-class NeighbourFinder:
-    """k nearest gas neighbours of every gas particle (sph.py:138-170).
+class VerletCache:
+    """Sizes of the candidate cache NeighbourFinder holds on the GPU (the
+    lists stay in device memory between calls; `export` copies them out)."""
 
-    A full search (N_CANDIDATES candidates) is done when `fresh` is True.
-    Otherwise the k nearest are re-selected among the candidates remembered
-    from the last full search (a Verlet list).  Indices are global particle
-    indices in the cache, so the list survives gas particles turning into
-    stars between two searches."""
+    def __init__(self, n_total, rows, candidates):
+        self.n_total, self.rows, self.candidates = n_total, rows, candidates
+
+    def export(self):
+        """(owner, cand): owner (n_total,) the cache row of each particle, -1
+        where it was not gas at the search; cand (rows, candidates) global
+        particle indices, nearest first."""
+        owner = np.empty(self.n_total, np.int64)
+        cand = np.empty((self.rows, self.candidates), np.int64)
+        _lib.check(_lib.load().gf_nbcache_export(_lib.ptr(owner), _lib.ptr(cand)))
+        return owner, cand
+
+
+class NeighbourFinder:
+    """k nearest gas neighbours of every gas particle, the particle itself
+    first (the contract of sph.py:138-170).
+
+    find(g, p, fresh) takes the global indices g of the current gas
+    particles and their positions p, and returns (dist, nb), nb in local
+    indices 0 .. len(g)-1.  With fresh=True, or when the cache cannot serve
+    the call, it searches all gas for each particle's n_candidates nearest
+    and keeps them on the GPU as global particle indices
+    (gf_nbcache_search).  Otherwise the k nearest are re-selected on the GPU
+    among those candidates that are still gas (gf_nbcache_reselect); a
+    particle without a cached row, or left with fewer than k candidates,
+    sends the call back to a full search.  The cache is the library's (one
+    per process, as the library's other state); `cache` describes it."""
 
     def __init__(self, n_total, k=None, n_candidates=None):
-        self.n_total = n_total
-        self.k = N_NEIGHBOURS if k is None else k
-        self.n_candidates = N_CANDIDATES if n_candidates is None else n_candidates
+        self.n_total = int(n_total)
+        self.k = N_NEIGHBOURS if k is None else int(k)
+        self.n_candidates = N_CANDIDATES if n_candidates is None else int(n_candidates)
+        if not 1 <= self.k <= self.n_candidates <= 64:
+            raise ValueError("need 1 <= k <= n_candidates <= 64")
         self.cache = None
-        self.last_passes = None
+        self.full_searches = 0
 
     def find(self, g, p, fresh):
-        """g: global indices of gas particles, p: their positions. Returns
-        (dist, nb) with nb in local (0..len(g)-1) indices."""
-        k = self.k
-        if not fresh and self.cache is not None:
-            row = self.cache["row"][g]
-            if (row >= 0).all():
-                loc = np.full(self.n_total, -1, np.int64)
-                loc[g] = np.arange(g.size)
-                cand = loc[self.cache["cand"][row]]      # candidates that are still gas
-                nb = np.empty((g.size, k), np.int64)
-                d = np.empty((g.size, k))
-                if knn_from_candidates(p, cand, k, nb, d):
-                    return d, nb
-        kq = min(self.n_candidates, g.size)
-        d, nb, info = knn(p, kq)
-        self.last_passes = info
-        row = np.full(self.n_total, -1, np.int64)
-        row[g] = np.arange(g.size)
-        self.cache = {"row": row, "cand": g[nb]}
-        return d[:, :k], np.ascontiguousarray(nb[:, :k])
+        g = np.ascontiguousarray(g, np.int64)
+        p = np.ascontiguousarray(p, np.float64)
+        n, k = g.size, self.k
+        nb = np.empty((n, k), np.int64)
+        d = np.empty((n, k))
+        L = _lib.load()
+        if not fresh and self.cache is not None and n:
+            if _lib.check(L.gf_nbcache_reselect(self.n_total, n, g, p, k, nb, d)) == 1:
+                return d, nb
+        if n == 0:
+            return d, nb
+        m = min(self.n_candidates, n)
+        if m < k:                 # fewer gas particles than k: every one is a neighbour
+            nb, d = nb[:, :m].copy(), d[:, :m].copy()
+        _lib.check(L.gf_nbcache_search(self.n_total, n, g, p, m, min(k, m), nb, d))
+        self.cache = VerletCache(self.n_total, n, m)
+        self.full_searches += 1
+        return d, nb

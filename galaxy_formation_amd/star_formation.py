@@ -2,14 +2,14 @@
 interface (galaxy_formation/star_formation.py): form_stars(sim, dt),
 supernova_feedback(sim, new), T_SF_MAX, E_SN_PER_MSUN; plus knn_query, the
 k-nearest query of points against a point set that the feedback needs
-(the cKDTree query of star_formation.py:62).
+(the cKDTree query of star_formation.py:59).
 
-Star formation (star_formation.py:37-49): gas with rho > rho_sf and
+Star formation (star_formation.py:35-47): gas with rho > rho_sf and
 T < 3e4 K becomes a star particle during dt with probability
 p = 1 - exp(-sf_eff dt / t_ff), t_ff = sqrt(3 pi / (32 G rho)); the draws are
 the simulation's own rng.uniform(size=m), taken in the same order.
 
-Feedback (star_formation.py:52-68): each new star heats its fb_ngb nearest
+Feedback (star_formation.py:50-65): each new star heats its fb_ngb nearest
 gas particles with fb_eff 1e49 erg per Msun formed, shared equally, and
 switches their cooling off for fb_myr Myr.
 
@@ -26,8 +26,8 @@ from .cooling import _dev, _torch
 from .ics import GAS, STAR
 from .units import MSUN_G, MYR
 
-T_SF_MAX = 3e4          # star_formation.py:32
-E_SN_PER_MSUN = 1e49    # star_formation.py:33
+T_SF_MAX = 3e4          # star_formation.py:31
+E_SN_PER_MSUN = 1e49    # star_formation.py:32
 
 
 def knn_query(src, q, k):
@@ -47,7 +47,7 @@ def knn_query(src, q, k):
 
 def form_stars(sim, dt):
     """Turns dense, cold gas into stars; returns the global indices of the
-    new stars (star_formation.py:37-49).  Uses sim.gas_idx, rho_gas, u,
+    new stars (star_formation.py:35-47).  Uses sim.gas_idx, rho_gas, u,
     u_per_K, rho_sf, cfg.sf_eff, rng, t; sets sim.kind and sim.birth."""
     g = sim.gas_idx
     if len(g) == 0:
@@ -75,7 +75,7 @@ def form_stars(sim, dt):
 
 def supernova_feedback(sim, new):
     """Heats the gas neighbours of the new star particles and blocks their
-    cooling (star_formation.py:52-68).  Uses sim.cfg (fb_eff, fb_ngb, fb_myr),
+    cooling (star_formation.py:50-65).  Uses sim.cfg (fb_eff, fb_ngb, fb_myr),
     kind, pos, mass, u, no_cool_until, t; updates u and no_cool_until."""
     cfg = sim.cfg
     if cfg.fb_eff <= 0 or new.size == 0:

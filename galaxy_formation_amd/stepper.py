@@ -137,7 +137,7 @@ class ForceEvaluator:
         return passes
 
     def combine(self):
-        """The caller's sum and time steps (simulation.py:67-70): acc =
+        """The caller's sum and time steps (simulation.py:68-71): acc =
         acc_grav + acc_sph (gas rows), dt_ind = min(0.3 sqrt(eps / max(|a|,
         1e-30)), dt_courant) with dt_courant only on the gas, and dt_grav =
         0.3 sqrt(eps / max(max |a|, 1e-30)) (a device scalar)."""

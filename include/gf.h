@@ -188,6 +188,30 @@ int gf_knn(int n, const double* pos, int k, const double* r_init, int64_t* nb, d
 int gf_knn_from_candidates_host(int n, int m, const double* pos, const int64_t* cand, int k,
                                 int64_t* nb, double* dist);
 
+/* sph.NeighbourFinder.find (sph.py:151-170), full search: the exact m
+ * nearest gas neighbours (the particle itself first) of the n gas particles
+ * g (global indices into the simulation's n_total particles) at pos (n, 3),
+ * kept on the GPU as candidate lists of global indices; nb (n, k) int64 local
+ * indices and dist (n, k) receive the first k, rows ascending in (distance,
+ * index).  1 <= k <= m <= min(n, 64).  Replaces the cKDTree query and the
+ * cache of sph.py:165-169. */
+int gf_nbcache_search(int n_total, int n, const int64_t* g, const double* pos, int m, int k,
+                      int64_t* nb, double* dist);
+/* sph.NeighbourFinder.find with fresh = False (sph.py:155-163): the k nearest
+ * of each gas particle among the candidates of the last search that are
+ * still gas (ascending in distance, then local index).  Returns 1 when every
+ * particle was served, 0 when the cache cannot serve this gas list (a
+ * particle without a cached row, fewer than k candidates left, another
+ * n_total): the caller then searches again; outputs are incomplete then. */
+int gf_nbcache_reselect(int n_total, int n, const int64_t* g, const double* pos, int k,
+                        int64_t* nb, double* dist);
+/* Sizes of the held cache (any pointer may be NULL); returns its row count. */
+int gf_nbcache_info(int* n_total, int* rows, int* m);
+/* The held cache as host arrays: owner (n_total) cache row of each particle
+ * (-1 = not gas at the search), cand (rows, m) global indices.  Returns the
+ * row count. */
+int gf_nbcache_export(int64_t* owner, int64_t* cand);
+
 /* Device-resident hydro of the gas (simulation.py:74-95): gas octree, exact
  * kNN by radius iteration (r_init: first radius per particle, NULL = from
  * the tree's cell densities), h = max(d_k / 2, eps / 2) (simulation.py:84),
@@ -298,25 +322,25 @@ int gf_thermo_dev(int ng, const int64_t* g, int n_total, double* u, const double
                   double u_per_K, double u_floor, void* stream);
 /* Rows skipped by gf_thermo_dev since the last call (waits; clears). */
 int gf_gas_errors_dev(void* stream);
-/* star_formation.form_stars (star_formation.py:37-47), first half: the gas
+/* star_formation.form_stars (star_formation.py:35-47), first half: the gas
  * rows of dense (rho > rho_sf), cold (u[g] / u_per_K < 3e4 K) gas, ascending,
  * into rows, and p = 1 - exp(-sf_eff dt / t_ff), t_ff = sqrt(3 pi / (32 rho)).
  * Returns the count m (waits). */
 int gf_sf_candidates_dev(int ng, const int64_t* g, int n_total, const double* u,
                          const double* rho, double u_per_K, double rho_sf, double sf_eff,
                          double dt, int64_t* rows, double* p, void* stream);
-/* Second half (star_formation.py:46): rows[i] for which uniform[i] < p[i],
+/* Second half (star_formation.py:44): rows[i] for which uniform[i] < p[i],
  * in order, into out.  The m uniform draws come from the caller's generator
  * (numpy's rng.uniform(size=m) in the reference).  Returns the count (waits). */
 int gf_sf_select_dev(int m, const int64_t* rows, const double* p, const double* uniform,
                      int64_t* out, void* stream);
 /* The k nearest source points of each query point: cKDTree(src).query(q, k)
- * of star_formation.py:62.  nb (n_q, k) gets source input indices ascending in
+ * of star_formation.py:59.  nb (n_q, k) gets source input indices ascending in
  * (distance, index), dist (may be NULL) the distances.  1 <= k <= 256.
  * Waits. */
 int gf_knn_query_dev(int n_src, const double* src, int n_q, const double* q, int k, int64_t* nb,
                      double* dist, void* stream);
-/* Supernova heating (star_formation.py:63-68): for the (n_new, k) targets
+/* Supernova heating (star_formation.py:60-65): for the (n_new, k) targets
  * (particle indices), du = E / k / (mass[t] MSUN MSUN_G) / U_CGS with
  * E = e_per_g new_mass MSUN MSUN_G, added to u in the order np.add.at adds the
  * flattened list; no_cool_until[targets] = until.  Waits for the index check. */

@@ -1,7 +1,7 @@
 """CPU restatement (numpy, FP64) of the gas physics around the force path:
 radiative cooling (galaxy_formation/cooling.py:24-43), the sub-step
 thermodynamics (simulation.py:97-104), star formation
-(star_formation.py:37-49) and supernova feedback (star_formation.py:52-68),
+(star_formation.py:35-47) and supernova feedback (star_formation.py:50-65),
 with a brute-force k-nearest query in place of scipy's cKDTree.
 
 TEST INFRASTRUCTURE ONLY.  Only tests/ may import this module; the product
@@ -35,8 +35,8 @@ X_H = 0.76
 MU_IONISED = 0.6
 
 T_FLOOR = 1e4            # cooling.py:21
-T_SF_MAX = 3e4           # star_formation.py:32
-E_SN_PER_MSUN = 1e49     # star_formation.py:33
+T_SF_MAX = 3e4           # star_formation.py:31
+E_SN_PER_MSUN = 1e49     # star_formation.py:32
 GAS, STAR = 0, 1         # initial_conditions.py:22
 
 
@@ -69,7 +69,7 @@ def thermodynamics(u, g, dudt, rho, no_cool_until, t, dt, u_per_K, u_floor):
 
 
 def sf_candidates(g, u, rho, u_per_K, rho_sf, sf_eff, dt):
-    """star_formation.py:39-45: gas rows of dense, cold gas and p."""
+    """star_formation.py:37-43: gas rows of dense, cold gas and p."""
     dense = (rho > rho_sf) & (u[g] / u_per_K < T_SF_MAX)
     rows = np.flatnonzero(dense)
     t_ff = np.sqrt(3 * np.pi / (32 * rho[dense]))
@@ -91,7 +91,7 @@ def knn_query(src, q, k):
 
 
 def feedback_heat(u, no_cool_until, mass, targets, new_mass, fb_eff, k, until):
-    """star_formation.py:63-68 on copies; returns (u, no_cool_until)."""
+    """star_formation.py:60-65 on copies; returns (u, no_cool_until)."""
     u, ncu = u.copy(), no_cool_until.copy()
     e_per_g = fb_eff * E_SN_PER_MSUN / MSUN_G
     E = e_per_g * new_mass * MSUN * MSUN_G

@@ -17,6 +17,7 @@
  *   - galaxy_formation/sph.py:165-166 exact kNN (cKDTree query), restated as
  *                                     a brute-force scan ordered by (d^2, index)
  *   - galaxy_formation/simulation.py:84 h = max(d_k / 2, eps / 2)
+ *   - cpp/nbody.cpp:445-503  particle deposit and tone map (rendering), FP32
  *
  * Two precision modes share one tree:
  *   mode 0 ("ref"):  the reference arithmetic: FP32 node payload, FP32 MAC and
@@ -707,4 +708,65 @@ int orc_knn_from_candidates(int n, int m, const double *pos, const int64_t *cand
         free(bd); free(bi);
     }
     return ok;
+}
+
+/* ------------------------------------------------------------- rendering */
+
+/* cpp/nbody.cpp:445-469: image (ngroups, H, W) cleared, then each particle in
+ * turn projected orthographically (rot rows = screen x, y and depth axes,
+ * centre subtracted first, screen y downwards, pixel centres at integers)
+ * and, when its (u, v) lies in [0, W-1) x [0, H-1), deposited with bilinear
+ * weights into the four pixels around it in its group's plane.  Sequential:
+ * every pixel sums its contributions in particle order. */
+void orc_render(int n, const double *pos, const uint8_t *group, int ngroups, const double *rot,
+                const double *center, double scale, int W, int H, float *img) {
+    const size_t plane = (size_t)W * H;
+    memset(img, 0, sizeof(float) * plane * (size_t)ngroups);
+    for (int i = 0; i < n; ++i) {
+        const double x = pos[3 * i] - center[0], y = pos[3 * i + 1] - center[1],
+                     z = pos[3 * i + 2] - center[2];
+        const double u = 0.5 * W + scale * (rot[0] * x + rot[1] * y + rot[2] * z) - 0.5;
+        const double v = 0.5 * H - scale * (rot[3] * x + rot[4] * y + rot[5] * z) - 0.5;
+        if (!(u >= 0 && v >= 0 && u < W - 1 && v < H - 1)) continue;
+        const int iu = (int)u, iv = (int)v;
+        const float fu = (float)(u - iu), fv = (float)(v - iv);
+        float *p = img + (size_t)group[i] * plane + (size_t)iv * W + iu;
+        p[0] += (1 - fu) * (1 - fv);
+        p[1] += fu * (1 - fv);
+        p[W] += (1 - fu) * fv;
+        p[W + 1] += fu * fv;
+    }
+}
+
+/* cpp/nbody.cpp:474-503, FP32: per pixel and group, l = log1p(K inv_ref img)
+ * / log1p(K) plus the group's glow map (gw x gh) sampled bilinearly at the
+ * pixel centre (sample coordinates clamped to [0, g - 1.001]); the colour
+ * sum c = sum_g l colors[g] becomes 255 (1 - exp(-exposure c)) rounded half
+ * up to a byte. */
+void orc_tonemap(int W, int H, int ngroups, const float *img, const float *glow, int gw, int gh,
+                 const float *colors, float inv_ref, float K, float exposure, uint8_t *out) {
+    const size_t plane = (size_t)W * H, gplane = (size_t)gw * gh;
+    const float inv_logK = 1.0f / log1pf(K);
+    const float sx = (float)gw / W, sy = (float)gh / H;
+    for (int y = 0; y < H; ++y) {
+        const float gyf = fminf(fmaxf((y + 0.5f) * sy - 0.5f, 0.0f), gh - 1.001f);
+        const int gy = (int)gyf;
+        const float fy = gyf - gy;
+        for (int x = 0; x < W; ++x) {
+            const float gxf = fminf(fmaxf((x + 0.5f) * sx - 0.5f, 0.0f), gw - 1.001f);
+            const int gx = (int)gxf;
+            const float fx = gxf - gx;
+            float c[3] = {0.0f, 0.0f, 0.0f};
+            for (int g = 0; g < ngroups; ++g) {
+                float l = log1pf(K * inv_ref * img[g * plane + (size_t)y * W + x]) * inv_logK;
+                const float *q = glow + g * gplane + (size_t)gy * gw + gx;
+                l += (1 - fy) * ((1 - fx) * q[0] + fx * q[1]) +
+                     fy * ((1 - fx) * q[gw] + fx * q[gw + 1]);
+                for (int ch = 0; ch < 3; ++ch) c[ch] += l * colors[3 * g + ch];
+            }
+            for (int ch = 0; ch < 3; ++ch)
+                out[3 * ((size_t)y * W + x) + ch] =
+                    (uint8_t)(255.0f * (1 - expf(-exposure * c[ch])) + 0.5f);
+        }
+    }
 }

@@ -54,6 +54,8 @@ def lib():
         L.orc_knn_from_candidates.restype = _ci
         L.orc_knn_from_candidates.argtypes = [_ci, _ci, _dbl, _i64, _ci, _i64, _dbl]
         L.orc_num_threads.restype = _ci
+        L.orc_render.argtypes = [_ci, _dbl, _u8, _ci, _dbl, _dbl, _cd, _ci, _ci, _f32]
+        L.orc_tonemap.argtypes = [_ci, _ci, _ci, _f32, _f32, _ci, _ci, _f32, _cf, _cf, _cf, _u8]
         _lib = L
     return _lib
 
@@ -70,6 +72,9 @@ def ref_lib():
         L.nbody_num_threads.restype = _ci
         L.ref_tree_export.restype = _ci
         L.ref_tree_export.argtypes = [_vp] * 9
+        L.nbody_render.argtypes = [_ci, _dbl, _u8, _ci, _dbl, _dbl, _cd, _ci, _ci, _f32]
+        L.nbody_tonemap.argtypes = [_ci, _ci, _ci, _f32, _f32, _ci, _ci, _f32, _cf, _cf, _cf,
+                                    _u8]
         _ref = L
     return _ref
 
@@ -214,3 +219,51 @@ def knn_from_candidates(pos, cand, k):
 def smoothing_length(dk, eps):
     """simulation.py:84: h = max(d_k / 2, eps / 2)."""
     return np.maximum(0.5 * dk, 0.5 * eps)
+
+
+def render(pos, group, ngroups, rot, center, scale, W, H):
+    """cpp/nbody.cpp:445-469 restated: (ngroups, H, W) float32 deposit."""
+    img = np.empty((ngroups, H, W), np.float32)
+    lib().orc_render(len(group), np.ascontiguousarray(pos, np.float64),
+                     np.ascontiguousarray(group, np.uint8), int(ngroups),
+                     np.ascontiguousarray(rot, np.float64), np.ascontiguousarray(center, np.float64),
+                     float(scale), int(W), int(H), img)
+    return img
+
+
+def tonemap(img, glow, colors, inv_ref, K, exposure):
+    """cpp/nbody.cpp:474-503 restated: (H, W, 3) uint8."""
+    ng, H, W = img.shape
+    out = np.empty((H, W, 3), np.uint8)
+    lib().orc_tonemap(W, H, ng, np.ascontiguousarray(img, np.float32),
+                      np.ascontiguousarray(glow, np.float32), glow.shape[2], glow.shape[1],
+                      np.ascontiguousarray(colors, np.float32).ravel(), float(inv_ref), float(K),
+                      float(exposure), out)
+    return out
+
+
+def ref_render(pos, group, ngroups, rot, center, scale, W, H):
+    """The reference nbody_render (built from source), or None if absent."""
+    R = ref_lib()
+    if R is None:
+        return None
+    img = np.empty((ngroups, H, W), np.float32)
+    R.nbody_render(len(group), np.ascontiguousarray(pos, np.float64),
+                   np.ascontiguousarray(group, np.uint8), int(ngroups),
+                   np.ascontiguousarray(rot, np.float64), np.ascontiguousarray(center, np.float64),
+                   float(scale), int(W), int(H), img)
+    return img
+
+
+def ref_tonemap(img, glow, colors, inv_ref, K, exposure):
+    """The reference nbody_tonemap (built from source), or None if absent."""
+    R = ref_lib()
+    if R is None:
+        return None
+    ng, H, W = img.shape
+    out = np.empty((H, W, 3), np.uint8)
+    R.nbody_tonemap(W, H, ng, np.ascontiguousarray(img, np.float32),
+                    np.ascontiguousarray(glow, np.float32), glow.shape[2], glow.shape[1],
+                    np.ascontiguousarray(colors, np.float32).ravel(), float(inv_ref), float(K),
+                    float(exposure), out)
+    return out

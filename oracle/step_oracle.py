@@ -3,14 +3,14 @@
 TEST INFRASTRUCTURE ONLY: used by tests/ to check the device-resident step
 loop (galaxy_formation_amd/stepper.py); the product never imports it.
 
-accelerations() restates the caller of the force path,
+Forces restates the caller of the force path,
 GalaxyFormation.accelerations / _hydro (galaxy_formation/simulation.py:61-95):
 gravity of the active particles (inactive rows keep their last value, as
 nbody_accel_active leaves them, cpp/nbody.cpp:419-421), then, when there are
 more than 40 gas particles (simulation.py:80), neighbours, h = max(d_k / 2,
 eps / 2) (:84) and sph_forces (:88-90), the sum acc = acc_grav + acc_sph
-(:67), dt_ind = min(0.3 sqrt(eps / max(|a|, 1e-30)), dt_courant) (:68-69)
-and dt_grav = 0.3 sqrt(eps / max(max |a|, 1e-30)) (:70).  Gravity here is
+(:68), dt_ind = min(0.3 sqrt(eps / max(|a|, 1e-30)), dt_courant) (:69-70)
+and dt_grav = 0.3 sqrt(eps / max(max |a|, 1e-30)) (:71).  Gravity here is
 the FP64 direct sum (gf_oracle.c orc_accel_direct) and the SPH sums the
 C restatement of sph.py.
 

@@ -246,6 +246,54 @@ def test_clustered_duplicates(eng):
     assert relerr(acc, ref).max() < 1e-12
 
 
+def test_deep_cells_at_the_key_depth(eng):
+    """Sites of 12 particles each, coincident or a few 1e-10 apart: the
+    21-level keys cannot separate them, so their cells become leaves of
+    more than 8 particles (counted as deep cells), where the reference
+    divides them down to depth 48 (cpp/nbody.cpp:122).  A node of
+    coincident particles has its monopole at their common position, so the
+    accelerations equal the oracle's (which follows the reference's depth)
+    to FP64 rounding."""
+    rng = np.random.default_rng(21)
+    sites = rng.normal(size=(400, 3))
+    pos = np.repeat(sites, 12, axis=0)
+    pos[len(pos) // 2:] += rng.normal(scale=1e-10, size=(len(pos) - len(pos) // 2, 3))
+    pos = np.ascontiguousarray(pos)
+    mass = rng.uniform(0.5, 1.5, len(pos)) / len(pos)
+    for prec, tol in (("fp64", 1e-12), ("fp32", 2e-6)):
+        acc = run(eng, pos, mass, 0.5, 0.05, prec)
+        assert eng.tree_info()[2] > 0                       # deep cells present
+        ref, _ = O.accel(pos, mass, 0.5, 0.05, mode=1)
+        assert relerr(acc, ref).max() < tol, prec
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_groups_over_64_particles_through_the_global_stack(eng, prec):
+    """Sites of 100 coincident particles: each site is a depth-capped leaf
+    and a group of two 64-particle chunks.  With the LDS stack cut to 8
+    entries every group is redone by the global-stack walk: accelerations
+    and the interaction count equal the unconstrained walk's, and the
+    accelerations equal the oracle's to FP64 rounding (FP64)."""
+    from galaxy_formation_amd import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(22)
+    pos = np.ascontiguousarray(np.repeat(rng.normal(size=(60, 3)), 100, axis=0))
+    mass = rng.uniform(0.5, 1.5, len(pos)) / len(pos)
+    ref = run(eng, pos, mass, 0.5, 0.05, prec)
+    inter = eng.last_interactions()
+    assert eng.tree_info()[2] >= 60
+    _lib.check(L.gf_set_test_limits(8, 0))
+    try:
+        acc = run(eng, pos, mass, 0.5, 0.05, prec)
+        assert eng.last_interactions() == inter
+    finally:
+        _lib.check(L.gf_set_test_limits(0, 0))
+    np.testing.assert_array_equal(acc, ref)
+    if prec == "fp64":
+        oref, _ = O.accel(pos, mass, 0.5, 0.05, mode=1)
+        assert relerr(acc, oref).max() < 1e-12
+
+
 @pytest.mark.parametrize("limit", [1, 64, 300])
 def test_global_stack_walk_equals_lds_walk(eng, limit):
     """With the LDS stack cut to `limit` entries, groups overflow and are

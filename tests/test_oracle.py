@@ -338,6 +338,21 @@ def test_sph_closed_forms_at_rest():
     assert np.abs(one[1]).max() == 0
 
 
+def test_density_uses_each_particles_own_h():
+    """Two particles 1.5 apart, h = 1 and 2, masses 1 and 3 (sph.py:72-77:
+    rho_i = sum_j m_j W(r_ij, h_i), the self term included).  By hand, from
+    the M4 spline: W(1.5, 1) = 0.25 (2 - 1.5)^3 / pi = 0.03125 / pi and
+    W(1.5, 2) = (1 - 1.5 0.75^2 + 0.75 0.75^3) / (8 pi) = 0.47265625 / (8 pi),
+    so rho_0 = (1 + 3 x 0.03125) / pi and rho_1 = (3 + 0.47265625) / (8 pi).
+    A neighbour's h in place of the particle's own gives other values."""
+    pos = np.array([[0.0, 0.0, 0.0], [1.5, 0.0, 0.0]])
+    nb = np.array([[0, 1], [1, 0]], np.int64)
+    m = np.array([1.0, 3.0])
+    h = np.array([1.0, 2.0])
+    rho = O.sph_forces(pos, np.zeros((2, 3)), m, nb, h, np.ones(2), GAMMA, 1.0, 2.0)[0]
+    np.testing.assert_allclose(rho, [1.09375 / np.pi, 3.47265625 / (8 * np.pi)], rtol=1e-15)
+
+
 def test_divv_of_linear_flows():
     """div v of v = s x inside a lattice is close to 3 s (the continuum limit:
     -(1/rho) sum_j m_j r_ij^2 (1/r) dW/dr -> -int 4 pi r^3 W'(r) dr = 3); a

@@ -156,6 +156,55 @@ def test_neighbour_finder_full_and_verlet():
     np.testing.assert_allclose(d2, od2, rtol=1e-15)
 
 
+def test_neighbour_finder_after_star_formation():
+    """The cache keeps global indices: after some gas particles turn into
+    stars (they leave g), the re-selection drops them from the candidate
+    lists and equals the oracle's knn_from_candidates over those lists
+    (sph.py:155-163), and the exact neighbours where the true ones are still
+    candidates, without a new search.  A gas list with a particle the cache
+    does not know, or rows left with fewer than k candidates, fall back to a
+    full search."""
+    c = ics.make_cloud(3000, 3000, seed=31)
+    g = c.gas
+    nf = sph.NeighbourFinder(c.n)
+    p = np.ascontiguousarray(c.pos[g])
+    nf.find(g, p, True)
+    owner, cand = nf.cache.export()
+    assert (owner[g] == np.arange(len(g))).all() and (np.delete(owner, g) == -1).all()
+    _, onb48 = O.knn(p, 48)
+    np.testing.assert_array_equal(cand, g[onb48])
+    # 3 % of the gas becomes stars
+    rng = np.random.default_rng(1)
+    keep = np.sort(rng.choice(len(g), int(0.97 * len(g)), replace=False))
+    g2 = g[keep]
+    p2 = np.ascontiguousarray(c.pos[g2] + 1e-4 * c.vel[g2])
+    d2, nb2 = nf.find(g2, p2, False)
+    assert nf.full_searches == 1
+    loc = np.full(c.n, -1, np.int64)
+    loc[g2] = np.arange(len(g2))
+    ok, onb, od = O.knn_from_candidates(p2, loc[cand[owner[g2]]], 32)
+    assert ok
+    np.testing.assert_array_equal(nb2, onb)
+    np.testing.assert_array_equal(d2, od)
+    ed, enb = O.knn(p2, 32)
+    served = (np.sort(onb, 1) == np.sort(enb, 1)).all(1)
+    assert served.mean() > 0.99
+    # a particle that was not gas at the search: full search
+    g3 = np.concatenate([g2, [c.n - 1]])
+    p3 = np.ascontiguousarray(c.pos[g3])
+    d3, nb3 = nf.find(g3, p3, False)
+    assert nf.full_searches == 2
+    ed3, enb3 = O.knn(p3, 32)
+    np.testing.assert_array_equal(nb3, enb3)
+    np.testing.assert_array_equal(d3, ed3)
+    # most of the gas gone: rows with fewer than 32 candidates left
+    g4 = g3[::3]
+    p4 = np.ascontiguousarray(c.pos[g4])
+    d4, nb4 = nf.find(g4, p4, False)
+    assert nf.full_searches == 3
+    np.testing.assert_array_equal(nb4, O.knn(p4, 32)[1])
+
+
 def test_hydro_pipeline_equals_oracle():
     """Device-resident pipeline (gas tree, kNN, h, density, forces) against
     the oracle's brute-force kNN + h rule + sph_forces."""
@@ -187,6 +236,8 @@ def test_hydro_pipeline_equals_oracle():
     np.testing.assert_allclose(out["acc"].cpu().numpy(), oacc, rtol=1e-10,
                                atol=1e-12 * np.abs(oacc).max())
     np.testing.assert_allclose(out["dt"].cpu().numpy(), odt, rtol=1e-13)
+    np.testing.assert_allclose(out["dudt"].cpu().numpy(), odudt, rtol=1e-10,
+                               atol=1e-12 * np.abs(odudt).max())
     np.testing.assert_allclose(out["divv"].cpu().numpy(), odivv, rtol=1e-10,
                                atol=1e-12 * np.abs(odivv).max())
 

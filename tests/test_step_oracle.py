@@ -1,7 +1,7 @@
 """CPU pins of the step-loop oracle (oracle/step_oracle.py) by mathematics:
 the leapfrog's closed form on linear forces, its exactness for constant
 forces whatever the rungs do, hand-worked rung ladders, and the reference
-caller's formulas for dt_ind and dt_grav (simulation.py:67-70)."""
+caller's formulas for dt_ind and dt_grav (simulation.py:68-71)."""
 
 import numpy as np
 
@@ -127,7 +127,7 @@ def test_gas_energy_kicks_and_floor():
 def test_forces_glue_is_simulation_py_formulas():
     """acc = acc_grav + acc_sph; dt_ind = min(0.3 sqrt(eps / |a|),
     dt_courant) with dt_courant = inf off the gas; dt_grav from the largest
-    |a|; no hydro at 40 gas particles or fewer (simulation.py:67-80)."""
+    |a|; no hydro at 40 gas particles or fewer (simulation.py:68-80)."""
     rng = np.random.default_rng(2)
     n, ng, eps = 300, 120, 0.05
     pos = rng.normal(size=(n, 3))
