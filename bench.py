@@ -505,11 +505,10 @@ def main():
                                               "inside each timed step"},
                    "gates": {"acc_rms_vs_direct": GATE_ACC_RMS, "rho_vs_plain": GATE_RHO},
                    "meets_gates": fast["meets_accuracy_gates"]},
-        # compute-bound ("mfma" names the compute roof): the walk evaluates
-        # on the FP32 vector units, whose published peak is also 157.3
-        # TFLOP/s (packed FP32; 78.6 without packing, which this build does
-        # not use); no MFMA, pair terms are not a contraction
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+        # compute-bound on the FP32 vector units (VALU), published peak 157.3
+        # TFLOP/s with packed FP32 (78.6 without packing, which this build
+        # does not use); no MFMA: pair terms are not a contraction
+        "roofline": {"bound": "fp32-vector", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
                      "traffic": WALK_TRAFFIC_BYTES[prec_h] if a.workload == "disk1m" else None,
                      "kernel": {0: "k_walk<false, false, false> (FP32 group walk, monopoles)",
@@ -536,6 +535,8 @@ def main():
         "accuracy": accuracy,
     }
     out["gas_physics"] = gas_physics(B)
+    if a.workload == "disk1m":
+        out["host_abi_end_to_end"] = host_abi_end_to_end(B)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(B)
     print(json.dumps(out))
@@ -686,6 +687,58 @@ def sph_roofline(ng, k, ms, workload):
             "bytes_algorithmic": b, "kernels": "k_pack32 + k_density32 + k_force32", "ms": ms}
 
 
+def host_abi_end_to_end(B, runs=3):
+    """The drop-in through the reference's host-array interfaces, as
+    simulation.py:61-95 calls them: Engine.gravity (nbody_accel_active,
+    acc_grav holding the previous step's gravity, as simulation.py:64 passes
+    it), NeighbourFinder.find (a full search, fresh = True), h = max(d_k / 2,
+    eps / 2) in numpy, and sph.sph_forces; every call copies its numpy inputs
+    to the GPU and its outputs back (PCIe included).  Wall time, median of
+    `runs` after one warm-up, for the plain path (FP64 gravity) and the fast
+    path's gravity (FP32 + quadrupoles, relative rule); the host interface
+    has no learned first radius (sph.py's signatures carry no history)."""
+    import torch
+
+    from galaxy_formation_amd import sph
+    from galaxy_formation_amd.engine import Engine
+    c, ng, k = B.c, B.ng, B.a.k
+    torch.cuda.synchronize()
+    acc_prev = B.acc_prev.cpu().numpy() if B.acc_prev is not None else np.zeros((c.n, 3))
+    g = np.arange(ng, dtype=np.int64)
+    gp = np.ascontiguousarray(c.pos[:ng])
+    gv = np.ascontiguousarray(c.vel[:ng])
+    gm = np.ascontiguousarray(c.mass[:ng])
+    gu = np.ascontiguousarray(c.u[:ng])
+    out = {}
+    for name, prec, rel in (("plain_fp64", "fp64", 0.0), ("fast_fp32_quad", "fp32q", QUAD_REL_TOL)):
+        eng = Engine(prec, quad_relative_tolerance=rel)
+        nf = sph.NeighbourFinder(c.n, k=k, n_candidates=k)
+        rho, dudt, dt_i = np.empty(ng), np.empty(ng), np.empty(ng)
+        a_sph = np.empty((ng, 3))
+
+        def one():
+            acc = acc_prev.copy()
+            t = [time.perf_counter()]
+            eng.gravity(c.pos, c.mass, acc, B.a.theta, c.eps)
+            t.append(time.perf_counter())
+            d, nb = nf.find(g, gp, True)
+            h = np.maximum(0.5 * d[:, -1], 0.5 * c.eps)
+            t.append(time.perf_counter())
+            sph.sph_forces(gp, gv, gm, nb, h, gu, GAMMA, ALPHA, BETA, rho, a_sph, dudt, dt_i)
+            t.append(time.perf_counter())
+            return np.diff(t)
+        one()
+        r = np.array([one() for _ in range(runs)])
+        tm = r[int(np.argsort(r.sum(1))[len(r) // 2])]
+        out[name] = {"ms": 1e3 * float(tm.sum()), "gravity_ms": 1e3 * float(tm[0]),
+                     "neighbours_ms": 1e3 * float(tm[1]), "sph_forces_ms": 1e3 * float(tm[2]),
+                     "force_evals_per_s": c.n / float(tm.sum())}
+        eng.quad_relative_tolerance = 0.0
+    out["note"] = (f"host arrays through the C ABI, k = {k}: nbody_accel_active, "
+                   "gf_nbcache_search, gf_sph_forces; H2D and D2H copies inside the times")
+    return out
+
+
 def cpu_baseline(B):
     """The reference path on the host cores, one full evaluation of the same
     workload: cpp/nbody.cpp built from source (oracle/_ref; nbody_accel_active,
@@ -719,15 +772,30 @@ def cpu_baseline(B):
         return np.diff(t)
 
     one()
-    runs = np.array([one() for _ in range(2)])
-    tm = np.median(runs, axis=0)
+    runs = np.array([one() for _ in range(5)])
+    totals = runs.sum(axis=1)
+    tm = runs[int(np.argsort(totals)[len(totals) // 2])]
     total = float(tm.sum())
     return {"value": c.n / total, "unit": "force-evals/s", "cores": int(L.nbody_num_threads()),
-            "kind": "reference",
+            "kind": "reference", "cpu": cpu_model(),
             "sample": f"full {B.a.workload} evaluation ({c.n} particles, {ng} gas, k={k}), "
-                      "median of 2 after 1 warm-up; gravity = reference nbody.cpp, kNN = "
+                      "median of 5 after 1 warm-up; gravity = reference nbody.cpp, kNN = "
                       "scipy cKDTree, SPH sums = C restatement (Numba absent)",
-            "phases_s": {"gravity": float(tm[0]), "knn": float(tm[1]), "sph": float(tm[2])}}
+            "build_flags": "-O3 -march=x86-64-v3 -ffast-math -fopenmp (the reference's "
+                           "cpp/Makefile:5 flags, x86-64-v3 in place of -march=native: the "
+                           "library is built once, before the run, for any x86-64-v3 host)",
+            "phases_s": {"gravity": float(tm[0]), "knn": float(tm[1]), "sph": float(tm[2])},
+            "totals_s": [float(x) for x in totals]}
+
+
+def cpu_model():
+    """The host CPU's model name and logical CPU count, from /proc/cpuinfo."""
+    try:
+        names = [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo")
+                 if ln.startswith("model name")]
+        return f"{names[0]} ({len(names)} logical CPUs)" if names else None
+    except OSError:
+        return None
 
 
 if __name__ == "__main__":

@@ -588,9 +588,11 @@ int gf_nbcache_search(int n_total, int n, const int64_t* g, const double* pos, i
     if (gf_ctx_init()) return -1;
     hipStream_t s = g_ctx.stream;
     const size_t b1 = sizeof(double) * size_t(n), b3 = 3 * b1;
+    // d_nb / d_dist: the search's n x m table, then its first k columns
+    // behind it
     if (STAGE(g_ctx.d_pos, b3) || STAGE(g_ctx.d_mass, b1) || STAGE(g_ctx.d_g, sizeof(int64_t) * n) ||
-        STAGE(g_ctx.d_nb, sizeof(int64_t) * size_t(n) * m) ||
-        STAGE(g_ctx.d_dist, sizeof(double) * size_t(n) * m))
+        STAGE(g_ctx.d_nb, sizeof(int64_t) * size_t(n) * (m + k)) ||
+        STAGE(g_ctx.d_dist, sizeof(double) * size_t(n) * (m + k)))
         return -1;
     GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
     GF_HIP(hipMemcpyAsync(g_ctx.d_g, g, sizeof(int64_t) * size_t(n), hipMemcpyHostToDevice, s));
@@ -601,11 +603,13 @@ int gf_nbcache_search(int n_total, int n, const int64_t* g, const double* pos, i
     g_ctx.sph_k = 0;   // the held table holds input indices: not usable for forces
     if (gf_sph_export_knn(S, m, g_ctx.d_nb, g_ctx.d_dist, s)) return -1;
     if (gf_verlet_store(g_ctx.verlet, n_total, n, m, g_ctx.d_g, g_ctx.d_nb, s)) return -1;
-    // the first k columns of each row
-    GF_HIP(hipMemcpy2DAsync(nb, sizeof(int64_t) * k, g_ctx.d_nb, sizeof(int64_t) * m,
-                            sizeof(int64_t) * k, size_t(n), hipMemcpyDeviceToHost, s));
-    GF_HIP(hipMemcpy2DAsync(dist, sizeof(double) * k, g_ctx.d_dist, sizeof(double) * m,
-                            sizeof(double) * k, size_t(n), hipMemcpyDeviceToHost, s));
+    // the first k columns of each row, compacted on the device (a pitched
+    // copy into pageable host memory goes row by row)
+    int64_t* knb = g_ctx.d_nb + size_t(n) * m;
+    double* kdist = g_ctx.d_dist + size_t(n) * m;
+    if (gf_first_columns(n, m, k, g_ctx.d_nb, g_ctx.d_dist, knb, kdist, s)) return -1;
+    GF_HIP(hipMemcpyAsync(nb, knb, sizeof(int64_t) * size_t(n) * k, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipMemcpyAsync(dist, kdist, sizeof(double) * size_t(n) * k, hipMemcpyDeviceToHost, s));
     GF_HIP(hipStreamSynchronize(s));
     return 0;
 }

@@ -35,4 +35,7 @@ int gf_verlet_reselect(GfVerlet& V, int n_total, int n, int k, const int64_t* g,
 // The cache as host arrays: owner (n_total), cand (rows x m) global indices
 // (either may be null).  Returns the row count.
 int gf_verlet_export(const GfVerlet& V, int64_t* owner, int64_t* cand, hipStream_t s);
+// nb, dist (n x k) = the first k columns of src_nb, src_dist (n x m), device.
+int gf_first_columns(int n, int m, int k, const int64_t* src_nb, const double* src_dist,
+                     int64_t* nb, double* dist, hipStream_t s);
 void gf_verlet_free(GfVerlet& V);

@@ -110,6 +110,19 @@ __global__ __launch_bounds__(kRowsPerBlock * 64) void k_reselect(
     }
 }
 
+// dst (n x k) = the first k columns of src (n x m), one thread per entry
+__global__ __launch_bounds__(kTB) void k_first_columns(int n, int m, int k,
+                                                       const int64_t* __restrict__ src_nb,
+                                                       const double* __restrict__ src_dist,
+                                                       int64_t* __restrict__ nb,
+                                                       double* __restrict__ dist) {
+    const size_t t = size_t(blockIdx.x) * kTB + threadIdx.x;
+    if (t >= size_t(n) * k) return;
+    const size_t row = t / k, c = t % k;
+    nb[t] = src_nb[row * m + c];
+    dist[t] = src_dist[row * m + c];
+}
+
 template <typename T>
 int reserve(T*& p, size_t& cap, size_t count) {
     if (count <= cap && p) return 0;
@@ -192,6 +205,15 @@ int gf_verlet_export(const GfVerlet& V, int64_t* owner, int64_t* cand, hipStream
     if (cand)
         for (size_t i = 0; i < c.size(); ++i) cand[i] = c[i];
     return V.rows;
+}
+
+int gf_first_columns(int n, int m, int k, const int64_t* src_nb, const double* src_dist,
+                     int64_t* nb, double* dist, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_first_columns, dim3(gf_div_up(int(size_t(n) * k), kTB)), dim3(kTB), 0, s,
+                       n, m, k, src_nb, src_dist, nb, dist);
+    GF_LAUNCH_CHECK();
+    return 0;
 }
 
 void gf_verlet_free(GfVerlet& V) {
