@@ -203,24 +203,53 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restri
         if (i < n) atomicAdd(&h[(key[i] >> shift) & 0xFF], 1);
     }
     __syncthreads();
-    if (threadIdx.x < 256) hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    if (threadIdx.x < 256) hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];  // digit-major
 }
 
-// Stable scatter: wave w of block b owns keys [b*TILE + w*SPAN, +SPAN), read
-// as chunks of 64.  Inside a chunk, lanes holding the same digit are ranked
-// by lane order (8 ballots); chunks are ranked in order through a per-wave
-// digit counter; waves are ranked in order through an LDS prefix.
+// One wave per digit: the exclusive prefix of the digit's row of block
+// counts (in place) and the row total (tot[d]).
+__global__ __launch_bounds__(64) void k_rs_rowscan(int* __restrict__ hist, int nblocks,
+                                                   int* __restrict__ tot) {
+    int* row = hist + size_t(blockIdx.x) * nblocks;
+    const int lane = threadIdx.x;
+    const int per = (nblocks + 63) / 64;
+    const int b0 = min(nblocks, lane * per), b1 = min(nblocks, b0 + per);
+    int sum = 0;
+    for (int b = b0; b < b1; ++b) sum += row[b];
+    const int incl = gf_wave_incl_scan(sum);
+    int run = incl - sum;
+    for (int b = b0; b < b1; ++b) {
+        const int v = row[b];
+        row[b] = run;
+        run += v;
+    }
+    if (lane == 63) tot[blockIdx.x] = incl;
+}
+
+// Stable scatter of one tile of RS_TILE keys.  Ranking: wave w owns keys
+// [tile + w*SPAN, +SPAN), read as chunks of 64; lanes holding the same digit
+// are ranked by lane order (8 ballots), chunks in order through a per-wave
+// digit counter, waves in order through an LDS prefix.  The tile is first
+// written to LDS in digit order, then copied out in that order: consecutive
+// threads write consecutive slots of one digit's output run, so the global
+// stores coalesce.  A digit's run starts at the keys of smaller digits
+// (scan of tot) plus its keys in earlier tiles (the row prefix of hist).
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const uint64_t* __restrict__ key, const uint32_t* __restrict__ val, int n, int shift,
-    int nblocks, const int* __restrict__ offs, uint64_t* __restrict__ key_out,
-    uint32_t* __restrict__ val_out) {
+    int nblocks, const int* __restrict__ hist, const int* __restrict__ tot,
+    uint64_t* __restrict__ key_out, uint32_t* __restrict__ val_out) {
     __shared__ int cnt[RS_WAVES][256];
-    __shared__ int wbase[RS_WAVES][256];
+    __shared__ int lbase[256];                   // tile-local start of each digit
+    __shared__ int gbase[256];                   // global start of the tile's run of each digit
+    __shared__ int s_wtot[2][4];
+    __shared__ uint64_t skey[RS_TILE];
+    __shared__ uint32_t sval[RS_TILE];
     const int lane = __lane_id(), w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_THREADS) (&cnt[0][0])[i] = 0;
     __syncthreads();
 
-    const long long base = (long long)blockIdx.x * RS_TILE + (long long)w * RS_WAVE_SPAN;
+    const long long tile0 = (long long)blockIdx.x * RS_TILE;
+    const long long base = tile0 + (long long)w * RS_WAVE_SPAN;
     uint64_t k_[RS_ITEMS];
     uint32_t v_[RS_ITEMS];
     int r_[RS_ITEMS];
@@ -251,23 +280,54 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
         r_[c] = ok ? pre + rank : -1;
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < 256; d += RS_THREADS) {   // one digit per thread
-        int run = offs[d * nblocks + blockIdx.x];
+    // per digit (first 4 waves, one digit per thread): the waves' prefixes,
+    // the tile's count, and both scans over the 256 digits
+    int dcnt = 0, gt = 0, lincl = 0, gincl = 0;
+    if (threadIdx.x < 256) {
+        const int d = threadIdx.x;
 #pragma unroll
         for (int ww = 0; ww < RS_WAVES; ++ww) {
-            wbase[ww][d] = run;
-            run += cnt[ww][d];
+            const int c = cnt[ww][d];
+            cnt[ww][d] = dcnt;                   // from here on: the wave's prefix in the tile
+            dcnt += c;
         }
+        gt = tot[d];
+        lincl = gf_wave_incl_scan(dcnt);
+        gincl = gf_wave_incl_scan(gt);
+        if (lane == 63) {
+            s_wtot[0][w] = lincl;
+            s_wtot[1][w] = gincl;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 256) {
+        const int d = threadIdx.x;
+        int lo = lincl - dcnt, go = gincl - gt;
+        for (int q = 0; q < w; ++q) {
+            lo += s_wtot[0][q];
+            go += s_wtot[1][q];
+        }
+        lbase[d] = lo;
+        gbase[d] = go + hist[size_t(d) * nblocks + blockIdx.x];
     }
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < RS_ITEMS; ++c) {
         if (r_[c] >= 0) {
             const int d = int((k_[c] >> shift) & 0xFF);
-            const int dst = wbase[w][d] + r_[c];
-            key_out[dst] = k_[c];
-            val_out[dst] = v_[c];
+            const int lp = lbase[d] + cnt[w][d] + r_[c];
+            skey[lp] = k_[c];
+            sval[lp] = v_[c];
         }
+    }
+    __syncthreads();
+    const int m = int(min((long long)RS_TILE, (long long)n - tile0));
+    for (int i = threadIdx.x; i < m; i += RS_THREADS) {
+        const uint64_t kk = skey[i];
+        const int d = int((kk >> shift) & 0xFF);
+        const int dst = gbase[d] + (i - lbase[d]);
+        key_out[dst] = kk;
+        val_out[dst] = sval[i];
     }
 }
 
@@ -294,27 +354,22 @@ int gf_scan_excl(const int* in, int* out, int n, int* partial, hipStream_t s, co
     return 0;
 }
 
-size_t gf_sort_temp_ints(int n) {
-    const size_t nh = size_t(256) * gf_div_up(n, RS_TILE);
-    return nh + 1 + gf_scan_temp_ints(int(nh));
-}
+size_t gf_sort_temp_ints(int n) { return size_t(256) * gf_div_up(n, RS_TILE) + 256; }
 
 int gf_radix_sort_pairs(uint64_t* key, uint32_t* val, uint64_t* key_alt, uint32_t* val_alt, int n,
                         int begin_bit, int end_bit, int* temp, hipStream_t s, int* in_alt) {
     *in_alt = 0;
     if (n <= 1) return 0;
     const int nb = gf_div_up(n, RS_TILE);
-    const int nh = 256 * nb;
-    int* hist = temp;
-    int* partial = temp + nh + 1;   // hist[nh] receives the total; partials follow
+    int* hist = temp;               // digit-major per-tile digit counts
+    int* tot = temp + size_t(256) * nb;
     uint64_t *ka = key, *kb = key_alt;
     uint32_t *va = val, *vb = val_alt;
     for (int shift = begin_bit; shift < end_bit; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, s, ka, n, shift, nb, hist);
-        // exclusive scan of the bin-major histogram, in place
-        if (gf_scan_excl(hist, hist, nh, partial, s)) return -1;
+        hipLaunchKernelGGL(k_rs_rowscan, dim3(256), dim3(64), 0, s, hist, nb, tot);
         hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_THREADS), 0, s, ka, va, n, shift, nb,
-                           hist, kb, vb);
+                           hist, tot, kb, vb);
         GF_LAUNCH_CHECK();
         uint64_t* tk = ka; ka = kb; kb = tk;
         uint32_t* tv = va; va = vb; vb = tv;

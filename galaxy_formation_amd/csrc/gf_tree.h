@@ -25,8 +25,8 @@ struct GfTree {
     int *start = nullptr, *count = nullptr, *next = nullptr, *parent = nullptr;
     int8_t *leaf = nullptr, *depth = nullptr;
     int *child8 = nullptr;                         // 8 slots per node, octant order, -1 = empty
-    int *dlist = nullptr;                          // node ids grouped by depth
-    int *dmeta = nullptr;                          // per depth: count, offset (+1), cursor
+    int *dlist = nullptr;                          // nodes above the bottom-up subtrees
+    int *dmeta = nullptr;                          // [0] their count (0 between passes)
     double4 *com = nullptr;                        // centre of mass and mass (FP64)
     double *open_r = nullptr;                      // opening radius (FP64)
     double4 *box = nullptr;                        // geometric centre and half-width (FP64)

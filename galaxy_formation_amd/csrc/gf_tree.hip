@@ -325,96 +325,65 @@ __device__ __forceinline__ void quad_add(double* q, double m, double dx, double 
     q[5] += m * (3 * dy * dz);
 }
 
-// Nodes listed by depth (deepest level first is the bottom-up order).
-__global__ __launch_bounds__(TB) void k_depth_count(const int* __restrict__ counters,
-                                                    const int8_t* __restrict__ depth,
-                                                    int* __restrict__ dcount) {
-    __shared__ int h[GF_KEY_LEVELS + 1];
-    if (threadIdx.x <= GF_KEY_LEVELS) h[threadIdx.x] = 0;
-    __syncthreads();
-    const int nn = counters[0];
-    for (int id = blockIdx.x * TB + threadIdx.x; id < nn; id += gridDim.x * TB)
-        atomicAdd(&h[depth[id]], 1);
-    __syncthreads();
-    if (threadIdx.x <= GF_KEY_LEVELS && h[threadIdx.x]) atomicAdd(&dcount[threadIdx.x], h[threadIdx.x]);
-}
+// Arrays of the bottom-up pass over one tree.
+struct MomentArgs {
+    const int* start;
+    const int* count;
+    const int* next;
+    const int* parent;
+    const int8_t* leaf;
+    const int8_t* depth;
+    const int* child8;
+    const double4* box;
+    const double* tx;
+    const double* ty;
+    const double* tz;
+    const double* tm;
+    const double* tdisp;
+    double4* com;
+    double* open_r;
+    double* ndisp;
+    double* quad;
+    double theta_d;
+};
 
-// dcount[0..L] -> doff[0..L+1] (exclusive); cursors reset.
-__global__ void k_depth_offsets(int* __restrict__ dcount, int* __restrict__ doff,
-                                int* __restrict__ dcur) {
-    if (threadIdx.x != 0) return;
-    int run = 0;
-    for (int d = 0; d <= GF_KEY_LEVELS; ++d) {
-        doff[d] = run;
-        run += dcount[d];
-        dcur[d] = 0;
-    }
-    doff[GF_KEY_LEVELS + 1] = run;
-}
-
-// Chunks of TB nodes: ranks inside a chunk from LDS counters, one global
-// atomic per depth and chunk reserves the chunk's slots (order inside a depth
-// does not matter).
-__global__ __launch_bounds__(TB) void k_depth_scatter(const int* __restrict__ counters,
-                                                      const int8_t* __restrict__ depth,
-                                                      const int* __restrict__ doff,
-                                                      int* __restrict__ dcur,
-                                                      int* __restrict__ dlist) {
-    __shared__ int cnt[GF_KEY_LEVELS + 1];
-    __shared__ int base[GF_KEY_LEVELS + 1];
-    const int nn = counters[0];
-    for (int c0 = blockIdx.x * TB; c0 < nn; c0 += gridDim.x * TB) {
-        if (threadIdx.x <= GF_KEY_LEVELS) cnt[threadIdx.x] = 0;
-        __syncthreads();
-        const int id = c0 + threadIdx.x;
-        int d = -1, r = 0;
-        if (id < nn) {
-            d = depth[id];
-            r = atomicAdd(&cnt[d], 1);
-        }
-        __syncthreads();
-        if (threadIdx.x <= GF_KEY_LEVELS && cnt[threadIdx.x] > 0)
-            base[threadIdx.x] = doff[threadIdx.x] + atomicAdd(&dcur[threadIdx.x], cnt[threadIdx.x]);
-        __syncthreads();
-        if (d >= 0) dlist[base[d] + r] = id;
-        __syncthreads();
-    }
-}
-
-// Centres of mass, opening radii and largest displacements of the nodes of
-// one depth; the deeper levels are done.  Leaves sum their particles in tree
+// Moments of node id from its particles (leaf) or its children (done
+// before it).  MODE 0: centre of mass, opening radius and largest
+// displacement; 1: those and the traceless quadrupole; 2: the quadrupole
+// alone, about the held centre of mass.  Leaves sum their particles in tree
 // order (nbody.cpp:124-131), internal nodes their children in octant order
 // (nbody.cpp:136-147), both in FP64.
-template <bool QUAD>
-__global__ __launch_bounds__(TB) void k_level_com(
-    int d, const int* __restrict__ doff, const int* __restrict__ dlist,
-    const int* __restrict__ start, const int* __restrict__ count, const int8_t* __restrict__ leaf,
-    const int* __restrict__ child8, const double4* __restrict__ box,
-    const double* __restrict__ tx, const double* __restrict__ ty, const double* __restrict__ tz,
-    const double* __restrict__ tm, const double* __restrict__ tdisp, double4* __restrict__ com,
-    double* __restrict__ open_r, double* __restrict__ ndisp, double theta_d,
-    double* __restrict__ quad) {
+template <int MODE>
+__device__ void node_moments(const MomentArgs& a, int id) {
 #pragma clang fp contract(off)
-    const int b0 = doff[d], b1 = doff[d + 1];
-    for (int k = b0 + blockIdx.x * TB + threadIdx.x; k < b1; k += gridDim.x * TB) {
-        const int id = dlist[k];
-        double M = 0, sx = 0, sy = 0, sz = 0, dm = 0;
-        const bool lf = leaf[id] != 0;
-        const int b = start[id], cnt = count[id];
-        // a leaf of at most GF_LEAF_SIZE particles, and a node's children,
-        // are loaded in one round (slots past the count read an in-range
-        // entry that is not used), then summed in their order
-        double pm[GF_LEAF_SIZE], px[GF_LEAF_SIZE], py[GF_LEAF_SIZE], pz[GF_LEAF_SIZE];
-        const bool small = lf && cnt <= GF_LEAF_SIZE;
+    const bool lf = a.leaf[id] != 0;
+    const int b = a.start[id], cnt = a.count[id];
+    const bool small = lf && cnt <= GF_LEAF_SIZE;
+    // a leaf of at most GF_LEAF_SIZE particles, and a node's children, are
+    // loaded in one round (slots past the count read an in-range entry that
+    // is not used), then summed in their order
+    double pm[GF_LEAF_SIZE], px[GF_LEAF_SIZE], py[GF_LEAF_SIZE], pz[GF_LEAF_SIZE];
+    double4 c;
+    if constexpr (MODE == 2) {
+        c = a.com[id];
         if (small) {
 #pragma unroll
             for (int u = 0; u < GF_LEAF_SIZE; ++u) {
                 const int i = b + (u < cnt ? u : 0);
-                pm[u] = tm[i];
-                px[u] = tx[i];
-                py[u] = ty[i];
-                pz[u] = tz[i];
-                const double dd = tdisp[i];
+                pm[u] = a.tm[i]; px[u] = a.tx[i]; py[u] = a.ty[i]; pz[u] = a.tz[i];
+            }
+        }
+    } else {
+        double M = 0, sx = 0, sy = 0, sz = 0, dm = 0;
+        if (small) {
+#pragma unroll
+            for (int u = 0; u < GF_LEAF_SIZE; ++u) {
+                const int i = b + (u < cnt ? u : 0);
+                pm[u] = a.tm[i];
+                px[u] = a.tx[i];
+                py[u] = a.ty[i];
+                pz[u] = a.tz[i];
+                const double dd = a.tdisp[i];
                 if (u < cnt) {
                     M += pm[u];
                     sx += pm[u] * px[u];
@@ -425,24 +394,24 @@ __global__ __launch_bounds__(TB) void k_level_com(
             }
         } else if (lf) {
             for (int i = b; i < b + cnt; ++i) {
-                const double m = tm[i];
+                const double m = a.tm[i];
                 M += m;
-                sx += m * tx[i];
-                sy += m * ty[i];
-                sz += m * tz[i];
-                dm = fmax(dm, tdisp[i]);
+                sx += m * a.tx[i];
+                sy += m * a.ty[i];
+                sz += m * a.tz[i];
+                dm = fmax(dm, a.tdisp[i]);
             }
         } else {
-            const int4* sl = reinterpret_cast<const int4*>(child8 + 8 * size_t(id));
+            const int4* sl = reinterpret_cast<const int4*>(a.child8 + 8 * size_t(id));
             const int4 s0 = sl[0], s1 = sl[1];
             const int cs[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
             double4 cm[8];
             double nd[8];
 #pragma unroll
             for (int o = 0; o < 8; ++o) {
-                const int c = cs[o] >= 0 ? cs[o] : id;
-                cm[o] = com[c];
-                nd[o] = ndisp[c];
+                const int ch = cs[o] >= 0 ? cs[o] : id;
+                cm[o] = a.com[ch];
+                nd[o] = a.ndisp[ch];
             }
 #pragma unroll
             for (int o = 0; o < 8; ++o) {
@@ -454,62 +423,116 @@ __global__ __launch_bounds__(TB) void k_level_com(
                 dm = fmax(dm, nd[o]);
             }
         }
-        const double4 c = finalize_node(id, M, sx, sy, sz, dm, theta_d, box, com, open_r, ndisp);
-        if constexpr (QUAD) {
-            // the traceless quadrupole about the new centre of mass, as
-            // k_level_quad forms it (same terms, same order)
-            double q[6] = {0, 0, 0, 0, 0, 0};
-            if (small) {
+        c = finalize_node(id, M, sx, sy, sz, dm, a.theta_d, a.box, a.com, a.open_r, a.ndisp);
+    }
+    if constexpr (MODE >= 1) {
+        // the traceless quadrupole about the centre of mass: the particles
+        // of a leaf, or the children's tensors moved by the parallel-axis rule
+        double q[6] = {0, 0, 0, 0, 0, 0};
+        if (small) {
 #pragma unroll
-                for (int u = 0; u < GF_LEAF_SIZE; ++u)
-                    if (u < cnt) quad_add(q, pm[u], px[u] - c.x, py[u] - c.y, pz[u] - c.z);
-            } else if (lf) {
-                for (int i = b; i < b + cnt; ++i) quad_add(q, tm[i], tx[i] - c.x, ty[i] - c.y, tz[i] - c.z);
+            for (int u = 0; u < GF_LEAF_SIZE; ++u)
+                if (u < cnt) quad_add(q, pm[u], px[u] - c.x, py[u] - c.y, pz[u] - c.z);
+        } else if (lf) {
+            for (int i = b; i < b + cnt; ++i)
+                quad_add(q, a.tm[i], a.tx[i] - c.x, a.ty[i] - c.y, a.tz[i] - c.z);
+        } else {
+            const int* slots = a.child8 + 8 * size_t(id);
+            for (int o = 0; o < 8; ++o) {
+                const int ch = slots[o];
+                if (ch < 0) continue;
+                const double4 cm = a.com[ch];
+                for (int j = 0; j < 6; ++j) q[j] += a.quad[6 * size_t(ch) + j];
+                quad_add(q, cm.w, cm.x - c.x, cm.y - c.y, cm.z - c.z);
+            }
+        }
+        for (int j = 0; j < 6; ++j) a.quad[6 * size_t(id) + j] = q[j];
+    }
+}
+
+// Bottom-up pass, first half.  A subtree root is a node of at most
+// MOMENT_SUB particles whose parent holds more (or the root itself); its
+// subtree is the DFS node range [root, next[root]).  Each block scans a
+// chunk of node ids for subtree roots.  The thread that finds a root of at
+// most MOMENT_SMALL nodes finishes that subtree alone in reverse DFS order
+// (a node's children follow it in DFS order); the block finishes each larger
+// one together, deepest level first, a barrier between levels.  Nodes of
+// more than MOMENT_SUB particles are listed in top[] (*ntop) for the second
+// half.  No node is touched by two threads.
+constexpr int MOMENT_SUB = 2048;
+constexpr int MOMENT_SMALL = 32;
+
+template <int MODE>
+__global__ __launch_bounds__(TB) void k_moments_subtrees(MomentArgs a,
+                                                         const int* __restrict__ counters,
+                                                         int* __restrict__ top,
+                                                         int* __restrict__ ntop) {
+    __shared__ int roots[TB];
+    __shared__ int nroots, maxd;
+    const int nn = counters[0];
+    for (int c0 = blockIdx.x * TB; c0 < nn; c0 += gridDim.x * TB) {
+        if (threadIdx.x == 0) nroots = 0;
+        __syncthreads();
+        const int id = c0 + threadIdx.x;
+        if (id < nn) {
+            if (a.count[id] > MOMENT_SUB) {
+                top[atomicAdd(ntop, 1)] = id;
             } else {
-                const int* slots = child8 + 8 * size_t(id);
-                for (int o = 0; o < 8; ++o) {
-                    const int ch = slots[o];
-                    if (ch < 0) continue;
-                    const double4 cm = com[ch];
-                    for (int j = 0; j < 6; ++j) q[j] += quad[6 * size_t(ch) + j];
-                    quad_add(q, cm.w, cm.x - c.x, cm.y - c.y, cm.z - c.z);
+                const int p = a.parent[id];
+                if (p < 0 || a.count[p] > MOMENT_SUB) {
+                    const int e = min(a.next[id], nn);
+                    if (e - id <= MOMENT_SMALL) {
+                        for (int k = e - 1; k >= id; --k) node_moments<MODE>(a, k);
+                    } else {
+                        roots[atomicAdd(&nroots, 1)] = id;
+                    }
                 }
             }
-            for (int j = 0; j < 6; ++j) quad[6 * size_t(id) + j] = q[j];
+        }
+        __syncthreads();
+        const int nr = nroots;
+        for (int q = 0; q < nr; ++q) {
+            const int r = roots[q];
+            const int e = min(a.next[r], nn);
+            const int d0 = a.depth[r];
+            if (threadIdx.x == 0) maxd = d0;
+            __syncthreads();
+            int md = d0;
+            for (int k = r + threadIdx.x; k < e; k += TB) md = max(md, int(a.depth[k]));
+            atomicMax(&maxd, md);
+            __syncthreads();
+            const int dmax = maxd;
+            for (int d = dmax; d >= d0; --d) {
+                for (int k = r + threadIdx.x; k < e; k += TB)
+                    if (a.depth[k] == d) node_moments<MODE>(a, k);
+                __syncthreads();
+            }
         }
     }
 }
 
-// Traceless quadrupoles about the centres of mass, bottom-up by depth like
-// the centres of mass: from the particles of a leaf, or the children's
-// tensors moved by the parallel-axis rule (FP64).
-__global__ __launch_bounds__(TB) void k_level_quad(
-    int d, const int* __restrict__ doff, const int* __restrict__ dlist,
-    const int* __restrict__ start, const int* __restrict__ count, const int8_t* __restrict__ leaf,
-    const int* __restrict__ child8, const double* __restrict__ tx, const double* __restrict__ ty,
-    const double* __restrict__ tz, const double* __restrict__ tm, const double4* __restrict__ com,
-    double* __restrict__ quad) {
-#pragma clang fp contract(off)
-    const int b0 = doff[d], b1 = doff[d + 1];
-    for (int k = b0 + blockIdx.x * TB + threadIdx.x; k < b1; k += gridDim.x * TB) {
-        const int id = dlist[k];
-        const double4 c = com[id];
-        double q[6] = {0, 0, 0, 0, 0, 0};
-        if (leaf[id]) {
-            const int b = start[id], e = b + count[id];
-            for (int i = b; i < e; ++i) quad_add(q, tm[i], tx[i] - c.x, ty[i] - c.y, tz[i] - c.z);
-        } else {
-            const int* slots = child8 + 8 * size_t(id);
-            for (int o = 0; o < 8; ++o) {
-                const int ch = slots[o];
-                if (ch < 0) continue;
-                const double4 cm = com[ch];
-                for (int j = 0; j < 6; ++j) q[j] += quad[6 * size_t(ch) + j];
-                quad_add(q, cm.w, cm.x - c.x, cm.y - c.y, cm.z - c.z);
-            }
+// Second half: one block finishes the listed nodes above the subtrees,
+// deepest first, and clears the list's counter for the next pass.
+template <int MODE>
+__global__ __launch_bounds__(1024) void k_moments_top(MomentArgs a, const int* __restrict__ top,
+                                                      int* __restrict__ ntop) {
+    __shared__ int maxd;
+    const int nt = *ntop;
+    if (threadIdx.x == 0) maxd = 0;
+    __syncthreads();
+    int md = 0;
+    for (int k = threadIdx.x; k < nt; k += 1024) md = max(md, int(a.depth[top[k]]));
+    atomicMax(&maxd, md);
+    __syncthreads();
+    const int dmax = maxd;
+    for (int d = dmax; d >= 0; --d) {
+        for (int k = threadIdx.x; k < nt; k += 1024) {
+            const int id = top[k];
+            if (a.depth[id] == d) node_moments<MODE>(a, id);
         }
-        for (int j = 0; j < 6; ++j) quad[6 * size_t(id) + j] = q[j];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *ntop = 0;
 }
 
 // Groups (nbody.cpp:234-243): the outermost nodes holding <= 64 particles, or
@@ -610,38 +633,48 @@ void gf_tree_free(GfTree& T) {
     T = GfTree();
 }
 
-// Bottom-up centre-of-mass pass over the held topology (build and refit):
-// one launch per depth, deepest first.  A level of depth d holds at most 8^d
-// nodes, which sizes the grids of the shallow levels.
-static int com_pass(GfTree& T, hipStream_t s) {
-    const int* doff = T.dmeta + 32;
-    // the quadrupoles ride along when the last walk of this tree wanted them
-    const bool fuse = T.want_quad != 0;
-    for (int d = GF_KEY_LEVELS; d >= 0; --d) {
-        const int g = d >= 6 ? 1024 : std::max(1, 1 << std::max(0, 3 * d - 8));
-        if (fuse)
-            hipLaunchKernelGGL(k_level_com<true>, dim3(g), dim3(TB), 0, s, d, doff, T.dlist,
-                               T.start, T.count, T.leaf, T.child8, T.box, T.tx, T.ty, T.tz, T.tm,
-                               T.tdisp, T.com, T.open_r, T.ndisp, double(T.theta), T.quad);
-        else
-            hipLaunchKernelGGL(k_level_com<false>, dim3(g), dim3(TB), 0, s, d, doff, T.dlist,
-                               T.start, T.count, T.leaf, T.child8, T.box, T.tx, T.ty, T.tz, T.tm,
-                               T.tdisp, T.com, T.open_r, T.ndisp, double(T.theta), nullptr);
+static MomentArgs moment_args(const GfTree& T) {
+    MomentArgs a;
+    a.start = T.start; a.count = T.count; a.next = T.next; a.parent = T.parent;
+    a.leaf = T.leaf; a.depth = T.depth; a.child8 = T.child8; a.box = T.box;
+    a.tx = T.tx; a.ty = T.ty; a.tz = T.tz; a.tm = T.tm; a.tdisp = T.tdisp;
+    a.com = T.com; a.open_r = T.open_r; a.ndisp = T.ndisp; a.quad = T.quad;
+    a.theta_d = double(T.theta);
+    return a;
+}
+
+// The bottom-up pass over the held topology in two launches (subtrees, then
+// the nodes above them); mode as node_moments.
+static int moments_pass(const GfTree& T, int mode, hipStream_t s) {
+    const MomentArgs a = moment_args(T);
+    const int ng = node_grid(T.cap_nodes);
+    int* ntop = T.dmeta;
+    if (mode == 0) {
+        hipLaunchKernelGGL(k_moments_subtrees<0>, dim3(ng), dim3(TB), 0, s, a, T.counters, T.dlist, ntop);
+        hipLaunchKernelGGL(k_moments_top<0>, dim3(1), dim3(1024), 0, s, a, T.dlist, ntop);
+    } else if (mode == 1) {
+        hipLaunchKernelGGL(k_moments_subtrees<1>, dim3(ng), dim3(TB), 0, s, a, T.counters, T.dlist, ntop);
+        hipLaunchKernelGGL(k_moments_top<1>, dim3(1), dim3(1024), 0, s, a, T.dlist, ntop);
+    } else {
+        hipLaunchKernelGGL(k_moments_subtrees<2>, dim3(ng), dim3(TB), 0, s, a, T.counters, T.dlist, ntop);
+        hipLaunchKernelGGL(k_moments_top<2>, dim3(1), dim3(1024), 0, s, a, T.dlist, ntop);
     }
     GF_LAUNCH_CHECK();
+    return 0;
+}
+
+// Centres of mass (build and refit); the quadrupoles ride along when the
+// last walk of this tree wanted them.
+static int com_pass(GfTree& T, hipStream_t s) {
+    const bool fuse = T.want_quad != 0;
+    if (moments_pass(T, fuse ? 1 : 0, s)) return -1;
     T.quad_ok = fuse ? 1 : 0;
     return 0;
 }
 
 int gf_tree_quadrupoles(const GfTree& T, hipStream_t s) {
     if (T.quad_ok || T.n == 0) return 0;
-    const int* doff = T.dmeta + 32;
-    for (int d = GF_KEY_LEVELS; d >= 0; --d) {
-        const int g = d >= 6 ? 1024 : std::max(1, 1 << std::max(0, 3 * d - 8));
-        hipLaunchKernelGGL(k_level_quad, dim3(g), dim3(TB), 0, s, d, doff, T.dlist, T.start,
-                           T.count, T.leaf, T.child8, T.tx, T.ty, T.tz, T.tm, T.com, T.quad);
-    }
-    GF_LAUNCH_CHECK();
+    if (moments_pass(T, 2, s)) return -1;
     T.quad_ok = 1;
     return 0;
 }
@@ -682,11 +715,6 @@ int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float
     GF_HIP(hipMemcpyAsync(T.xb, pos, sizeof(double) * 3 * size_t(n), hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_gather, dim3(g), dim3(TB), 0, s, pos, mass, T.idx, n, nullptr, T.tx,
                        T.ty, T.tz, T.tm, T.tdisp);
-    hipLaunchKernelGGL(k_depth_count, dim3(ng), dim3(TB), 0, s, T.counters, T.depth, T.dmeta);
-    hipLaunchKernelGGL(k_depth_offsets, dim3(1), dim3(64), 0, s, T.dmeta, T.dmeta + 32,
-                       T.dmeta + 64);
-    hipLaunchKernelGGL(k_depth_scatter, dim3(ng), dim3(TB), 0, s, T.counters, T.depth,
-                       T.dmeta + 32, T.dmeta + 64, T.dlist);
     GF_LAUNCH_CHECK();
     if (with_com && com_pass(T, s)) return -1;
     // group offsets: the node count lives on the device, so the scan runs over
