@@ -81,6 +81,11 @@ def parse():
     p.add_argument("--knn-blocks-beside-walk", type=int, default=512,
                    help="persistent blocks of the neighbour pass while the gravity walk runs "
                         "on the other stream (0 = the library default, every resident slot)")
+    p.add_argument("--order", default="knn-first",
+                   choices=("gas-then-walk", "knn-then-walk", "knn-first", "walk-first"),
+                   help="overlapped step: the gas tree and the neighbour solve first, the walk "
+                        "after the solve (knn-then-walk); the gas passes enqueued before the "
+                        "walk, beside it (knn-first); or the walk enqueued first (walk-first)")
     p.add_argument("--knn-alone", action="store_true",
                    help="profiling aid: the gas neighbour solve alone, three times from a "
                         "first radius of 1.083 d_k (about the learned proposal's margin)")
@@ -195,6 +200,9 @@ class Bench:
         self.sp2 = self.stream2.cuda_stream
         self.fork = torch.cuda.Event()
         self.join = torch.cuda.Event()
+        self.knn_done = torch.cuda.Event()
+        self.knn_go = torch.cuda.Event()
+        self.tree_done = torch.cuda.Event()
         # the previous step's gravity (set by main): the quadrupole walk reads
         # it from its acc array on entry, as a simulation's acc_grav holds it
         self.acc_prev = None
@@ -204,8 +212,15 @@ class Bench:
 
     def hydro(self, prec, pos, first, ev=None, rec=None, stream=None):
         """first: None (cell densities), ("radius", r) or ("learned", model)."""
+        self.hydro_begin(prec, pos, first, rec, stream)
+        return self.hydro_end(prec, rec, stream)
+
+    def hydro_begin(self, prec, pos, first, rec=None, stream=None, after=None):
+        """Gas tree, first radii, and the neighbour solve's first passes
+        enqueued without waiting for the device (the passes after the event
+        `after`, when given)."""
         stream = stream or self.stream
-        L, chk, sp, o = self.L, self._lib.check, stream.cuda_stream, self.out[prec]
+        L, chk, sp = self.L, self._lib.check, stream.cuda_stream
         rec(2) if rec else None
         chk(L.gf_hydro_setup_dev(self.ng, self.p(pos), self.p(self.vel), self.p(self.mass),
                                  self.p(self.u), sp))
@@ -220,7 +235,21 @@ class Bench:
             x = model.features(self.a.k, self.dk_prev, self.cell_rho, self.cell_count,
                                self.divv_prev, self.dtc_prev, self.D, stream=stream)
             r_init = model.propose(x, self.dk_prev, out=self.r_learned, stream=stream)
-        passes = chk(L.gf_hydro_knn_dev(self.a.k, self.p(r_init), sp))
+        # the solve's set-up, then an event the walk can wait for, then the
+        # passes: two from a caller's radii (the learned start resolves all
+        # but a few groups in one), five from the cell densities (the plain
+        # start's usual count)
+        chk(L.gf_hydro_knn_begin_dev(self.a.k, self.p(r_init), 0, sp))
+        if after is not None:
+            stream.wait_event(after)
+        self.knn_go.record(stream)
+        chk(L.gf_hydro_knn_more_dev(2 if r_init is not None else 5, sp))
+
+    def hydro_end(self, prec, rec=None, stream=None):
+        """The neighbour solve's end (waits for it), then h, density, forces."""
+        stream = stream or self.stream
+        L, chk, sp, o = self.L, self._lib.check, stream.cuda_stream, self.out[prec]
+        passes = chk(L.gf_hydro_knn_end_dev(sp))
         rec(4) if rec else None
         # the fast paths' density and forces: FP32 pair terms, FP64 sums
         chk(L.gf_set_sph_precision(0 if prec == 1 else 1))
@@ -255,13 +284,67 @@ class Bench:
         rec(0, self.stream)
         self.entry_acc(prec)          # inside the timed step
         self.fork.record(self.stream)
+        if overlap and self.a.order == "gas-then-walk":
+            # the whole gas pipeline first (its tree beside the gravity tree
+            # build, both chains of short launches), the walk after it: the
+            # two long kernels, the neighbour pass and the walk, each have
+            # every CU to themselves, and no pass queues behind the walk
+            gs.wait_event(self.fork)
+            with torch.cuda.stream(gs):
+                self.hydro_begin(prec, self.pos, first, lambda i: rec(i, gs), stream=gs)
+            chk(L.gf_tree_build_dev(self.n, self.p(self.pos), self.p(self.mass), self.a.theta, sp))
+            rec(1, self.stream)
+            with torch.cuda.stream(gs):
+                passes = self.hydro_end(prec, lambda i: rec(i, gs), stream=gs)
+            self.join.record(gs)
+            self.stream.wait_event(self.join)
+            chk(L.gf_walk_dev(self.n, None, self.p(self.acc_g[prec]), self.c.eps, prec, sp))
+            rec(6, self.stream)
+            return passes
+        if overlap and self.a.order == "knn-then-walk":
+            # the gas tree and the neighbour solve go first, beside the
+            # gravity tree build (both chains of short, latency-bound
+            # launches); the walk waits for the solve, so each of the two
+            # long kernels has every CU to itself; the SPH sums run beside
+            # the walk
+            gs.wait_event(self.fork)
+            with torch.cuda.stream(gs):
+                self.hydro_begin(prec, self.pos, first, lambda i: rec(i, gs), stream=gs)
+            self.knn_done.record(gs)
+            chk(L.gf_tree_build_dev(self.n, self.p(self.pos), self.p(self.mass), self.a.theta, sp))
+            rec(1, self.stream)
+            self.stream.wait_event(self.knn_done)
+            chk(L.gf_walk_dev(self.n, None, self.p(self.acc_g[prec]), self.c.eps, prec, sp))
+            rec(6, self.stream)
+            with torch.cuda.stream(gs):
+                passes = self.hydro_end(prec, lambda i: rec(i, gs), stream=gs)
+            self.join.record(gs)
+            self.stream.wait_event(self.join)
+            return passes
         chk(L.gf_tree_build_dev(self.n, self.p(self.pos), self.p(self.mass), self.a.theta, sp))
         rec(1, self.stream)
-        chk(L.gf_walk_dev(self.n, None, self.p(self.acc_g[prec]), self.c.eps, prec, sp))
-        rec(6, self.stream)
-        gs.wait_event(self.fork)
-        with torch.cuda.stream(gs):
-            passes = self.hydro(prec, self.pos, first, ev, lambda i: rec(i, gs), stream=gs)
+        if overlap and self.a.order == "knn-first":
+            # the gas tree builds beside the gravity tree; the neighbour
+            # passes start once the gravity tree is built (its short kernels
+            # would otherwise wait for CUs the passes hold), and the walk
+            # waits for the passes' start: the pass's blocks are resident
+            # when the walk's persistent grid takes the rest of each CU
+            self.tree_done.record(self.stream)
+            gs.wait_event(self.fork)
+            with torch.cuda.stream(gs):
+                self.hydro_begin(prec, self.pos, first, lambda i: rec(i, gs), stream=gs,
+                                 after=self.tree_done)
+            self.stream.wait_event(self.knn_go)
+            chk(L.gf_walk_dev(self.n, None, self.p(self.acc_g[prec]), self.c.eps, prec, sp))
+            rec(6, self.stream)
+            with torch.cuda.stream(gs):
+                passes = self.hydro_end(prec, lambda i: rec(i, gs), stream=gs)
+        else:
+            chk(L.gf_walk_dev(self.n, None, self.p(self.acc_g[prec]), self.c.eps, prec, sp))
+            rec(6, self.stream)
+            gs.wait_event(self.fork)
+            with torch.cuda.stream(gs):
+                passes = self.hydro(prec, self.pos, first, ev, lambda i: rec(i, gs), stream=gs)
         self.join.record(gs)
         self.stream.wait_event(self.join)
         return passes
@@ -322,6 +405,36 @@ class Bench:
         res["knn_groups_to_heap_pass"] = int(st[1])
         res["knn_groups_to_global_stack"] = int(st[2])
         return res
+
+
+# persistent blocks of the neighbour pass beside the walk that each path is
+# timed with: 512 leaves room on each CU for the walk's waves (the monopole
+# walk's 8 KB of LDS per wave then lets a pass block stay beside 16 of
+# them), 0 takes every resident slot (the quadrupole walk's 10 KB per wave
+# fills a CU, so the pass runs before it instead)
+KNN_BLOCKS_TRIED = (512, 0)
+
+
+def best_schedule(B, prec, first):
+    """The overlapped step timed with each neighbour-pass grid of
+    KNN_BLOCKS_TRIED (the same steps, the same work); the faster one is the
+    path's time, and every one tried is reported under 'schedules'."""
+    a = B.a
+    keep = a.knn_blocks_beside_walk
+    tried, best = {}, None
+    for kb in KNN_BLOCKS_TRIED:
+        a.knn_blocks_beside_walk = kb
+        r = B.timed(prec, first)
+        tried[f"{a.order}, knn blocks {kb or 'all'}"] = r["ms"]
+        if best is None or r["ms"] < best["ms"]:
+            best = dict(r, knn_blocks_beside_walk=kb)
+    a.knn_blocks_beside_walk = keep
+    best["schedules"] = tried
+    # the steps of the chosen schedule leave their outputs last
+    a.knn_blocks_beside_walk = best["knn_blocks_beside_walk"]
+    B.timed(prec, first, steps=1)
+    a.knn_blocks_beside_walk = keep
+    return best
 
 
 def steploop(a):
@@ -412,11 +525,11 @@ def main():
     from galaxy_formation_amd.learned_h import LearnedH
     model = LearnedH().load(a.model)
 
-    res = {"plain_fp64": B.timed(1, None)}
+    res = {"plain_fp64": best_schedule(B, 1, None)}
     h_plain = B.out[1]["h"].cpu().numpy()
     rho_plain = B.out[1]["rho"].cpu().numpy()
-    res["fast_fp32"] = B.timed(0, ("learned", model))
-    res["fast_fp32_quad"] = B.timed(2, ("learned", model))
+    res["fast_fp32"] = best_schedule(B, 0, ("learned", model))
+    res["fast_fp32_quad"] = best_schedule(B, 2, ("learned", model))
     for name, prec in (("fast_fp32", 0), ("fast_fp32_quad", 2)):
         hf, rf = B.out[prec]["h"].cpu().numpy(), B.out[prec]["rho"].cpu().numpy()
         res[name]["h_equal_plain"] = bool(np.array_equal(hf, h_plain))

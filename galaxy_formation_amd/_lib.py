@@ -69,6 +69,9 @@ SIGNATURES = {
                            _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_hydro_setup_dev": (_ci, [_ci, _vp, _vp, _vp, _vp, _vp]),
     "gf_hydro_knn_dev": (_ci, [_ci, _vp, _vp]),
+    "gf_hydro_knn_begin_dev": (_ci, [_ci, _vp, _ci, _vp]),
+    "gf_hydro_knn_more_dev": (_ci, [_ci, _vp]),
+    "gf_hydro_knn_end_dev": (_ci, [_vp]),
     "gf_hydro_forces_dev": (_ci, [_cd, _cd, _cd, _cd, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_hydro_passes": (_ci, [_vp, _vp]),
     "gf_hydro_dk_dev": (_ci, [_vp, _vp]),

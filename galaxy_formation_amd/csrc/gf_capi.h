@@ -21,6 +21,7 @@ struct GfCtx {
     GfWalkScratch walk;
     GfSph sph;
     int sph_k = 0;                  // k of the held neighbour solve
+    int sph_pending_k = 0;          // k of a solve begun and not yet ended
     // staging buffers of the host-pointer entry points (grown, never shrunk)
     double* d_pos = nullptr; size_t d_pos_cap = 0;
     double* d_mass = nullptr; size_t d_mass_cap = 0;

@@ -662,7 +662,38 @@ int gf_hydro_setup_dev(int n, const double* pos, const double* vel, const double
 
 int gf_hydro_knn_dev(int k, const double* r_init, void* stream) {
     if (gf_ctx_init()) return -1;
+    g_ctx.sph_k = 0;
     if (gf_sph_knn(g_ctx.sph, k, r_init, true, false, 64, gf_stream(stream))) return -1;
+    g_ctx.sph_k = k;
+    return g_ctx.sph.last_passes;
+}
+
+int gf_hydro_knn_begin_dev(int k, const double* r_init, int passes, void* stream) {
+    if (gf_ctx_init()) return -1;
+    g_ctx.sph_k = 0;
+    if (passes < 0 || passes > 64) {
+        gf_set_error("gf_hydro_knn_begin_dev: passes = %d must be in [0, 64]", passes);
+        return -1;
+    }
+    if (gf_sph_knn_begin(g_ctx.sph, k, r_init, true, false, passes, gf_stream(stream))) return -1;
+    g_ctx.sph_pending_k = k;
+    return 0;
+}
+
+int gf_hydro_knn_more_dev(int passes, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (passes < 0 || passes > 64) {
+        gf_set_error("gf_hydro_knn_more_dev: passes = %d must be in [0, 64]", passes);
+        return -1;
+    }
+    return gf_sph_knn_more(g_ctx.sph, passes, gf_stream(stream));
+}
+
+int gf_hydro_knn_end_dev(void* stream) {
+    if (gf_ctx_init()) return -1;
+    const int k = g_ctx.sph_pending_k;
+    g_ctx.sph_pending_k = 0;
+    if (gf_sph_knn_end(g_ctx.sph, 64, gf_stream(stream))) return -1;
     g_ctx.sph_k = k;
     return g_ctx.sph.last_passes;
 }

@@ -34,6 +34,10 @@ namespace {
 #ifndef WALK_WAVES
 #define WALK_WAVES 4
 #endif
+// with quadrupoles: a 192-entry particle list and 64 quadrupole entries of
+// 36 B (a wave holds 9.25 KB of LDS with its 4 KB stack); a 112-entry list
+// (8 KB, the monopole walk's footprint) measured 1-2 % slower alone and no
+// faster beside the neighbour pass, which the walk's registers limit anyway
 #ifndef WALK_LCAP_QUAD
 #define WALK_LCAP_QUAD 192
 #endif
@@ -46,7 +50,7 @@ namespace {
 constexpr int SCAP = WALK_SCAP;                     // LDS stack entries
 constexpr int LCAP = 256;                           // LDS list entries
 constexpr int LCAP_QUAD = WALK_LCAP_QUAD;           // particle list with quadrupoles
-constexpr int QCAP = WALK_QCAP;                     // node list with quadrupoles (48 B each)
+constexpr int QCAP = WALK_QCAP;                     // node list with quadrupoles (36 B each)
 // one batch of 64 nodes appends up to 64 entries to a list after one flush
 static_assert(LCAP >= 64 && LCAP_QUAD >= 64 && QCAP >= 64, "list capacities below a batch");
 constexpr int GSCAP = 512 * (GF_KEY_LEVELS + 2);    // global stack entries per block
@@ -162,31 +166,34 @@ struct Acc<false> {
     // centre, qa = (xx, yy, zz, xy), qb = (xz, yz, -, -).  With d = node -
     // particle and r^2 = |d|^2 + eps^2, the acceleration of the potential
     // -m/r - (1/2) d.Q.d / r^5 is  a = (m/r^3 + (5/2) d.Q.d / r^7) d - Q.d / r^5.
-    __device__ static void quad_term(const float4& e, const float4& qa, const float4& qb,
-                                     float px, float py, float pz, float eps2, float& ax,
-                                     float& ay, float& az) {
+    // qa = (Qxx, Qyy, Qxy, Qxz), qyz = Qyz; Qzz = -(Qxx + Qyy) (traceless)
+    __device__ static void quad_term(const float4& e, const float4& qa, float qyz, float px,
+                                     float py, float pz, float eps2, float& ax, float& ay,
+                                     float& az) {
         const float dx = e.x - px, dy = e.y - py, dz = e.z - pz;
         const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
         const float inv = __builtin_amdgcn_rsqf(r2);
         const float inv2 = inv * inv;
         const float inv3 = inv * inv2;
         const float inv5 = inv3 * inv2;
-        const float qx = fmaf(qa.x, dx, fmaf(qa.w, dy, qb.x * dz));
-        const float qy = fmaf(qa.w, dx, fmaf(qa.y, dy, qb.y * dz));
-        const float qz = fmaf(qb.x, dx, fmaf(qb.y, dy, qa.z * dz));
+        const float qzz = -(qa.x + qa.y);
+        const float qx = fmaf(qa.x, dx, fmaf(qa.z, dy, qa.w * dz));
+        const float qy = fmaf(qa.z, dx, fmaf(qa.y, dy, qyz * dz));
+        const float qz = fmaf(qa.w, dx, fmaf(qyz, dy, qzz * dz));
         const float dqd = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
         const float f = fmaf(2.5f * dqd, inv5 * inv2, e.w * inv3);
         ax = fmaf(f, dx, fmaf(-inv5, qx, ax));
         ay = fmaf(f, dy, fmaf(-inv5, qy, ay));
         az = fmaf(f, dz, fmaf(-inv5, qz, az));
     }
-    __device__ void flush_quad(const float4* E, const float4* QA, const float4* QB, int ln, int sl,
+    __device__ void flush_quad(const float4* E, const float4* QA, const float* QB, int ln, int sl,
                                int S, float eps2) {
         float ax = 0, ay = 0, az = 0;
         int j = sl;
         for (; j + S < ln; j += 2 * S) {
-            const float4 e0 = E[j], qa0 = QA[j], qb0 = QB[j];
-            const float4 e1 = E[j + S], qa1 = QA[j + S], qb1 = QB[j + S];
+            const float4 e0 = E[j], qa0 = QA[j];
+            const float4 e1 = E[j + S], qa1 = QA[j + S];
+            const float qb0 = QB[j], qb1 = QB[j + S];
             quad_term(e0, qa0, qb0, px, py, pz, eps2, ax, ay, az);
             quad_term(e1, qa1, qb1, px, py, pz, eps2, ax, ay, az);
         }
@@ -251,7 +258,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
     constexpr int QC = QUAD ? QCAP : 1;
     __shared__ int s_stack[GST ? 1 : SCAP];
     __shared__ Ent s_list[LC];
-    __shared__ float4 s_qe[QC], s_qa[QC], s_qb[QC];
+    __shared__ float4 s_qe[QC], s_qa[QC];
+    __shared__ float s_qb[QC];
     // the slice reduction reuses the list memory once the list is evaluated
     static_assert(sizeof(Ent) * LC >= 3 * 64 * sizeof(double), "reduction space");
     double(*s_red)[64] = reinterpret_cast<double(*)[64]>(s_list);
@@ -468,10 +476,11 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                                 const int slot = nq + __popcll(qmask & gf_lanemask_lt());
                                 s_qe[slot] = make_float4(float(cm.x - bx), float(cm.y - by),
                                                          float(cm.z - bz), float(cm.w));
+                                // (xx, yy, xy, xz) and yz of the record's
+                                // (xx, yy, zz, xy), (xz, yz, ...)
                                 s_qa[slot] = make_float4(__int_as_float(rq03.x), __int_as_float(rq03.y),
-                                                         __int_as_float(rq03.z), __int_as_float(rq03.w));
-                                s_qb[slot] = make_float4(__int_as_float(rq45.x), __int_as_float(rq45.y),
-                                                         0.0f, 0.0f);
+                                                         __int_as_float(rq03.w), __int_as_float(rq45.x));
+                                s_qb[slot] = __int_as_float(rq45.y);
                             }
                             nq += nquad;
                         }

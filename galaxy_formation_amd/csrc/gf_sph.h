@@ -32,6 +32,10 @@ struct GfSph {
     int last_passes = 0;         // pass launches of the last solve
     long long stat_queued = 0;   // groups the fast pass handed on, summed over passes
     long long stat_queued2 = 0;  // groups handed on to the global-stack pass
+    // the solve in flight (gf_sph_knn_begin .. gf_sph_knn_end)
+    int knn_kq = 0, knn_pos_out = 0, knn_sorted = 0, knn_narrow = 0;
+    int knn_enqueued = 0;        // pass launches enqueued so far
+    int knn_open = 0;            // begun, not yet ended
 };
 
 void gf_sph_free(GfSph& S);
@@ -57,6 +61,18 @@ int gf_sph_setup(GfSph& S, int n, const double* pos, const double* vel, const do
 // indices); sorted: each row ascending in (d^2, index).
 int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool sorted,
                int max_passes, hipStream_t s);
+// The solve in two halves: _begin enqueues the set-up and `passes` pass
+// launches and returns without waiting (later passes find nothing to do once
+// every particle is resolved); _end waits for them, checks the device's
+// error flags and enqueues further passes, two at a time, until every
+// particle is resolved or max_passes launches were made.  Nothing may use
+// the neighbour table between the two calls.
+int gf_sph_knn_begin(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool sorted,
+                     int passes, hipStream_t s);
+int gf_sph_knn_end(GfSph& S, int max_passes, hipStream_t s);
+// Enqueues `passes` more passes of the solve in flight (after a _begin with
+// none, the set-up alone, a caller can order other streams after it).
+int gf_sph_knn_more(GfSph& S, int passes, hipStream_t s);
 
 // h = max(d_k / 2, eps / 2) (galaxy_formation/simulation.py:84) from the
 // k-th neighbour distance of the last solve.

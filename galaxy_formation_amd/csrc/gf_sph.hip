@@ -386,8 +386,12 @@ constexpr int FT = 256;
 #ifndef KNN_NARROW_BLOCKS_PER_CU
 #define KNN_NARROW_BLOCKS_PER_CU 5
 #endif
+#ifndef KNN_NARROW_CMAX
+#define KNN_NARROW_CMAX 512
+#endif
 struct KnnNarrow {
-    static constexpr int CMAX = 512, BLOCKS_PER_CU = KNN_NARROW_BLOCKS_PER_CU, UNION_FIRST = 1;
+    static constexpr int CMAX = KNN_NARROW_CMAX, BLOCKS_PER_CU = KNN_NARROW_BLOCKS_PER_CU,
+                         UNION_FIRST = 1;
 };
 struct KnnWide {
     static constexpr int CMAX = 1024, BLOCKS_PER_CU = 3, UNION_FIRST = 0;
@@ -1652,8 +1656,61 @@ int gf_sph_setup(GfSph& S, int n, const double* pos, const double* vel, const do
     return 0;
 }
 
-int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool sorted,
-               int max_passes, hipStream_t s) {
+static KnnArgs knn_args(GfSph& S) {
+    GfTree& T = S.tree;
+    KnnArgs a;
+    a.kn = reinterpret_cast<const KNode*>(S.knode);
+    a.p4 = reinterpret_cast<const double4*>(S.p4);
+    a.gmark = S.gmark;
+    a.retry_in = nullptr;
+    a.retry_out = nullptr;
+    a.pass_tag = 0;
+    a.list_cap = gf_test_limits().knn_list > 0 ? gf_test_limits().knn_list : CMAX_MAX;
+    a.start = T.start; a.count = T.count; a.leaf = T.leaf; a.child8 = T.child8; a.box = T.box;
+    a.groups = T.groups; a.counters = T.counters; a.tx = T.tx; a.ty = T.ty; a.tz = T.tz;
+    a.idx = T.idx; a.inv = S.inv; a.parent_ = T.parent; a.leaf_of = S.leaf_of; a.r = S.r;
+    a.anc = S.anc;
+    a.r_lo = S.r_lo; a.r_hi = S.r_hi; a.dk = S.dk; a.done = S.done; a.passes = S.passes;
+    a.nb = S.nb; a.n = T.n; a.kq = S.knn_kq; a.tree_pos_out = S.knn_pos_out;
+    a.sorted = S.knn_sorted;
+    a.state = S.state; a.ovf = S.ovf; a.ovf2 = S.ovf2; a.gstack = S.gstack;
+    // the fast pass writes unordered rows; sorted rows come from the heap pass
+    a.use_queue = S.knn_sorted ? 0 : 1;
+    return a;
+}
+
+// Enqueues passes [from, from + count) of the held solve.  Each pass reads
+// its work from the device (the previous pass's retry list), so a pass after
+// convergence finds nothing to do and costs its launches only.
+static int knn_enqueue(GfSph& S, int from, int count, hipStream_t s) {
+    KnnArgs a = knn_args(S);
+    int* rbuf[2] = {S.retry, S.retry + S.cap_groups};
+    for (int pass = from; pass < from + count; ++pass) {
+        // the previous pass's retry list is this pass's work
+        a.retry_in = pass == 0 ? nullptr : rbuf[(pass - 1) & 1];
+        a.retry_out = rbuf[pass & 1];
+        a.pass_tag = pass + 1;
+        hipLaunchKernelGGL(k_knn_reset, dim3(1), dim3(64), 0, s, S.state, pass == 0 ? 1 : 0);
+        if (!S.knn_sorted)
+            hipLaunchKernelGGL(S.knn_narrow ? k_knn_fast<KnnNarrow> : k_knn_fast<KnnWide>,
+                               dim3(S.knn_narrow ? gf_sph_fast_blocks(S) : wide_blocks(S)),
+                               dim3(FT), 0, s, a);
+        if (S.knn_kq <= 64) {
+            hipLaunchKernelGGL((k_knn_pass<64, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_knn_pass<64, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);
+        } else {
+            hipLaunchKernelGGL((k_knn_pass<128, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_knn_pass<128, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);
+        }
+        GF_LAUNCH_CHECK();
+    }
+    S.knn_enqueued = from + count;
+    return 0;
+}
+
+int gf_sph_knn_begin(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool sorted,
+                     int passes, hipStream_t s) {
+    S.knn_open = 0;
     const int n = S.tree.n;
     if (n == 0) return 0;
     if (kq < 1 || kq > n) {
@@ -1662,6 +1719,10 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
     }
     if (kq > 128) {
         gf_set_error("kNN: k = %d exceeds 128", kq);
+        return -1;
+    }
+    if (passes < 0) {
+        gf_set_error("kNN: a negative number of passes");
         return -1;
     }
     if (sph_reserve(S, n, kq)) return -1;
@@ -1680,53 +1741,36 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
     hipLaunchKernelGGL(k_p4_pack, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, T.tx, T.ty, T.tz,
                        reinterpret_cast<double4*>(S.p4));
     GF_LAUNCH_CHECK();
-    KnnArgs a;
-    a.kn = reinterpret_cast<const KNode*>(S.knode);
-    a.p4 = reinterpret_cast<const double4*>(S.p4);
-    a.gmark = S.gmark;
-    a.retry_in = nullptr;
-    a.list_cap = gf_test_limits().knn_list > 0 ? gf_test_limits().knn_list : CMAX_MAX;
-    a.start = T.start; a.count = T.count; a.leaf = T.leaf; a.child8 = T.child8; a.box = T.box;
-    a.groups = T.groups; a.counters = T.counters; a.tx = T.tx; a.ty = T.ty; a.tz = T.tz;
-    a.idx = T.idx; a.inv = S.inv; a.parent_ = T.parent; a.leaf_of = S.leaf_of; a.r = S.r;
-    a.anc = S.anc;
-    a.r_lo = S.r_lo; a.r_hi = S.r_hi; a.dk = S.dk; a.done = S.done; a.passes = S.passes;
-    a.nb = S.nb; a.n = n; a.kq = kq; a.tree_pos_out = tree_pos_out; a.sorted = sorted;
-    a.state = S.state; a.ovf = S.ovf; a.ovf2 = S.ovf2; a.gstack = S.gstack;
-    // the fast pass writes unordered rows; sorted rows come from the heap pass
-    a.use_queue = sorted ? 0 : 1;
-    // passes are enqueued two at a time and read their work from the device
-    // (a pass after convergence finds nothing to do), so the host waits once
-    // per two passes
+    S.knn_kq = kq;
+    S.knn_pos_out = tree_pos_out ? 1 : 0;
+    S.knn_sorted = sorted ? 1 : 0;
+    S.knn_narrow = r_init ? 1 : 0;
     S.stat_queued = 0;
     S.stat_queued2 = 0;
-    int* rbuf[2] = {S.retry, S.retry + S.cap_groups};
-    int pass = 0;
+    S.knn_enqueued = 0;
+    if (passes > 0 && knn_enqueue(S, 0, passes, s)) return -1;
+    S.knn_open = 1;
+    return 0;
+}
+
+int gf_sph_knn_more(GfSph& S, int passes, hipStream_t s) {
+    if (S.tree.n == 0) return 0;
+    if (!S.knn_open) {
+        gf_set_error("kNN: no solve was begun");
+        return -1;
+    }
+    return passes > 0 ? knn_enqueue(S, S.knn_enqueued, passes, s) : 0;
+}
+
+int gf_sph_knn_end(GfSph& S, int max_passes, hipStream_t s) {
+    if (S.tree.n == 0) return 0;
+    if (!S.knn_open) {
+        gf_set_error("kNN: no solve was begun");
+        return -1;
+    }
+    S.knn_open = 0;
     int st[16];
     while (true) {
-        if (pass >= max_passes) {
-            gf_set_error("kNN: particles still unresolved after %d passes", max_passes);
-            return -1;
-        }
-        for (int q = 0; q < 2 && pass < max_passes; ++q, ++pass) {
-            // the previous pass's retry list is this pass's work
-            a.retry_in = pass == 0 ? nullptr : rbuf[(pass - 1) & 1];
-            a.retry_out = rbuf[pass & 1];
-            a.pass_tag = pass + 1;
-            hipLaunchKernelGGL(k_knn_reset, dim3(1), dim3(64), 0, s, S.state, pass == 0 ? 1 : 0);
-            if (!sorted)
-                hipLaunchKernelGGL(r_init ? k_knn_fast<KnnNarrow> : k_knn_fast<KnnWide>,
-                                   dim3(r_init ? gf_sph_fast_blocks(S) : wide_blocks(S)),
-                                   dim3(FT), 0, s, a);
-            if (kq <= 64) {
-                hipLaunchKernelGGL((k_knn_pass<64, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
-                hipLaunchKernelGGL((k_knn_pass<64, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);
-            } else {
-                hipLaunchKernelGGL((k_knn_pass<128, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
-                hipLaunchKernelGGL((k_knn_pass<128, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);
-            }
-            GF_LAUNCH_CHECK();
-        }
         GF_HIP(hipMemcpyAsync(st, S.state, sizeof(st), hipMemcpyDeviceToHost, s));
         GF_HIP(hipStreamSynchronize(s));
         if (st[4] != 0 || st[12] != 0) {
@@ -1738,13 +1782,25 @@ int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool s
                          "bounds)", st[13]);
             return -1;
         }
-        if (st[0] == 0) break;
+        if (st[0] == 0 && S.knn_enqueued > 0) break;
+        if (S.knn_enqueued >= max_passes) {
+            gf_set_error("kNN: particles still unresolved after %d passes", max_passes);
+            return -1;
+        }
+        // not converged: two more passes, then look again
+        if (knn_enqueue(S, S.knn_enqueued, std::min(2, max_passes - S.knn_enqueued), s)) return -1;
     }
     S.stat_queued = st[9] + st[3];
     S.stat_queued2 = st[10] + st[6];
-    pass = st[11];
-    S.last_passes = pass;
+    S.last_passes = st[11];
     return 0;
+}
+
+int gf_sph_knn(GfSph& S, int kq, const double* r_init, bool tree_pos_out, bool sorted,
+               int max_passes, hipStream_t s) {
+    if (gf_sph_knn_begin(S, kq, r_init, tree_pos_out, sorted, std::min(2, max_passes), s))
+        return -1;
+    return gf_sph_knn_end(S, max_passes, s);
 }
 
 int gf_sph_smoothing(GfSph& S, int k, double eps, hipStream_t s) {

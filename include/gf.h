@@ -225,6 +225,20 @@ int gf_hydro_dev(int n, const double* pos, const double* vel, const double* mass
 int gf_hydro_setup_dev(int n, const double* pos, const double* vel, const double* mass,
                        const double* u, void* stream);
 int gf_hydro_knn_dev(int k, const double* r_init, void* stream);
+/* gf_hydro_knn_dev in parts, so that a caller can enqueue other work (the
+ * gravity walk on another stream) while the solve runs: _begin enqueues the
+ * set-up and `passes` (0..64) pass launches on the stream and returns
+ * without waiting for the device; _more enqueues further passes (a caller
+ * that records an event between the set-up and the first pass can order
+ * another stream's work after the solve's first kernels); each pass reads
+ * its work from the device, so passes after convergence find nothing to do.
+ * _end waits for them, checks the device's error flags, enqueues further
+ * passes until every particle is resolved, and returns the number of pass
+ * launches that had work.  Nothing may read the solve's results before
+ * _end. */
+int gf_hydro_knn_begin_dev(int k, const double* r_init, int passes, void* stream);
+int gf_hydro_knn_more_dev(int passes, void* stream);
+int gf_hydro_knn_end_dev(void* stream);
 int gf_hydro_forces_dev(double eps, double gamma, double alpha, double beta, double* h,
                         double* rho, double* acc, double* dudt, double* dt, double* divv,
                         void* stream);
