@@ -16,8 +16,9 @@
 // of nodes that start at each position, their DFS indices (an exclusive
 // scan: preorder is the order of (start, depth)) and `next` (the DFS index
 // of the first node starting at the cell's end) are all computed in parallel.
-// Centres of mass are summed bottom-up in FP64, one launch per depth,
-// children in octant order.
+// Centres of mass are summed bottom-up in FP64, children in octant order,
+// in two launches: subtrees of at most 256 nodes, each finished by one block
+// in LDS, then the nodes above them.
 #include "gf_prim.h"
 #include "gf_tree.h"
 
@@ -299,12 +300,11 @@ __global__ __launch_bounds__(TB) void k_gather(const double* __restrict__ pos,
 // 2 (half + dm) / theta + |com - centre| (dm = 0 right after a build).
 __device__ __forceinline__ double4 finalize_node(int id, double M, double sx, double sy,
                                                  double sz, double dm, double theta_d,
-                                                 const double4* __restrict__ box, double4* com,
+                                                 const double4& bx, double4* com,
                                                  double* open_r, double* ndisp) {
 #pragma clang fp contract(off)
     const double inv = M > 0 ? 1.0 / M : 0.0;
     const double cx = sx * inv, cy = sy * inv, cz = sz * inv;
-    const double4 bx = box[id];
     const double dx = cx - bx.x, dy = cy - bx.y, dz = cz - bx.z;
     const double4 c = make_double4(cx, cy, cz, M);
     com[id] = c;
@@ -347,25 +347,62 @@ struct MomentArgs {
     double theta_d;
 };
 
-// Moments of node id from its particles (leaf) or its children (done
-// before it).  MODE 0: centre of mass, opening radius and largest
-// displacement; 1: those and the traceless quadrupole; 2: the quadrupole
-// alone, about the held centre of mass.  Leaves sum their particles in tree
-// order (nbody.cpp:124-131), internal nodes their children in octant order
-// (nbody.cpp:136-147), both in FP64.
-template <int MODE>
-__device__ void node_moments(const MomentArgs& a, int id) {
-#pragma clang fp contract(off)
-    const bool lf = a.leaf[id] != 0;
-    const int b = a.start[id], cnt = a.count[id];
-    const bool small = lf && cnt <= GF_LEAF_SIZE;
-    // a leaf of at most GF_LEAF_SIZE particles, and a node's children, are
-    // loaded in one round (slots past the count read an in-range entry that
-    // is not used), then summed in their order
-    double pm[GF_LEAF_SIZE], px[GF_LEAF_SIZE], py[GF_LEAF_SIZE], pz[GF_LEAF_SIZE];
+// What a node's moments leave for its parent: the centre of mass and mass,
+// the largest displacement and the quadrupole.
+struct NodeMoments {
     double4 c;
+    double dm;
+    double q[6];
+};
+
+// The children of an internal node by octant slot: from the tree's arrays
+// (node ids), or from the block's LDS window of a subtree (entries, 16 bits
+// a slot, 0xFFFF = empty).  An empty slot reads the node itself (entry 0 of
+// the window), so the loads of all slots go out together, unconditionally.
+struct KidsGlobal {
+    static constexpr int QBATCH = 8;     // children's quadrupoles loaded at once
+    const MomentArgs& a;
+    int self;
+    int cs[8];
+    __device__ bool has(int o) const { return cs[o] >= 0; }
+    __device__ int at(int o) const { return cs[o] >= 0 ? cs[o] : self; }
+    __device__ double4 com(int o) const { return a.com[at(o)]; }
+    __device__ double nd(int o) const { return a.ndisp[at(o)]; }
+    __device__ double q(int o, int j) const { return a.quad[6 * size_t(at(o)) + j]; }
+};
+
+struct KidsLds {
+    static constexpr int QBATCH = 2;
+    const double4* s_com;
+    const double* s_nd;
+    const double* s_q;
+    uint32_t e[4];
+    __device__ int entry(int o) const { return int((e[o >> 1] >> (16 * (o & 1))) & 0xFFFFu); }
+    __device__ bool has(int o) const { return entry(o) != 0xFFFF; }
+    __device__ int at(int o) const { return has(o) ? entry(o) : 0; }
+    __device__ double4 com(int o) const { return s_com[at(o)]; }
+    __device__ double nd(int o) const { return s_nd[at(o)]; }
+    __device__ double q(int o, int j) const { return s_q[6 * at(o) + j]; }
+};
+
+// Moments of leaf id (particles [b, b + cnt), box bx) in tree order
+// (nbody.cpp:124-131), FP64.  MODE 0: centre of mass, opening radius and
+// largest displacement; 1: those and the traceless quadrupole; 2: the
+// quadrupole alone, about the held centre of mass `held`.  Results go to the
+// tree's arrays and are returned.
+template <int MODE>
+__device__ NodeMoments leaf_moments(const MomentArgs& a, int id, int b, int cnt,
+                                    const double4& bx, const double4& held) {
+#pragma clang fp contract(off)
+    NodeMoments r;
+    const bool small = cnt <= GF_LEAF_SIZE;
+    // a leaf of at most GF_LEAF_SIZE particles is loaded in one round (slots
+    // past the count read an in-range entry that is not used), then summed
+    // in order
+    double pm[GF_LEAF_SIZE], px[GF_LEAF_SIZE], py[GF_LEAF_SIZE], pz[GF_LEAF_SIZE];
+    double4 c = held;
+    r.dm = 0;
     if constexpr (MODE == 2) {
-        c = a.com[id];
         if (small) {
 #pragma unroll
             for (int u = 0; u < GF_LEAF_SIZE; ++u) {
@@ -392,7 +429,7 @@ __device__ void node_moments(const MomentArgs& a, int id) {
                     dm = fmax(dm, dd);
                 }
             }
-        } else if (lf) {
+        } else {
             for (int i = b; i < b + cnt; ++i) {
                 const double m = a.tm[i];
                 M += m;
@@ -401,138 +438,346 @@ __device__ void node_moments(const MomentArgs& a, int id) {
                 sz += m * a.tz[i];
                 dm = fmax(dm, a.tdisp[i]);
             }
-        } else {
-            const int4* sl = reinterpret_cast<const int4*>(a.child8 + 8 * size_t(id));
-            const int4 s0 = sl[0], s1 = sl[1];
-            const int cs[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
-            double4 cm[8];
-            double nd[8];
-#pragma unroll
-            for (int o = 0; o < 8; ++o) {
-                const int ch = cs[o] >= 0 ? cs[o] : id;
-                cm[o] = a.com[ch];
-                nd[o] = a.ndisp[ch];
-            }
-#pragma unroll
-            for (int o = 0; o < 8; ++o) {
-                if (cs[o] < 0) continue;
-                M += cm[o].w;
-                sx += cm[o].w * cm[o].x;
-                sy += cm[o].w * cm[o].y;
-                sz += cm[o].w * cm[o].z;
-                dm = fmax(dm, nd[o]);
-            }
         }
-        c = finalize_node(id, M, sx, sy, sz, dm, a.theta_d, a.box, a.com, a.open_r, a.ndisp);
+        c = finalize_node(id, M, sx, sy, sz, dm, a.theta_d, bx, a.com, a.open_r, a.ndisp);
+        r.dm = dm;
     }
+    r.c = c;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) r.q[j] = 0;
     if constexpr (MODE >= 1) {
-        // the traceless quadrupole about the centre of mass: the particles
-        // of a leaf, or the children's tensors moved by the parallel-axis rule
-        double q[6] = {0, 0, 0, 0, 0, 0};
+        double* q = r.q;
         if (small) {
 #pragma unroll
             for (int u = 0; u < GF_LEAF_SIZE; ++u)
                 if (u < cnt) quad_add(q, pm[u], px[u] - c.x, py[u] - c.y, pz[u] - c.z);
-        } else if (lf) {
+        } else {
             for (int i = b; i < b + cnt; ++i)
                 quad_add(q, a.tm[i], a.tx[i] - c.x, a.ty[i] - c.y, a.tz[i] - c.z);
-        } else {
-            const int* slots = a.child8 + 8 * size_t(id);
-            for (int o = 0; o < 8; ++o) {
-                const int ch = slots[o];
-                if (ch < 0) continue;
-                const double4 cm = a.com[ch];
-                for (int j = 0; j < 6; ++j) q[j] += a.quad[6 * size_t(ch) + j];
-                quad_add(q, cm.w, cm.x - c.x, cm.y - c.y, cm.z - c.z);
+        }
+        for (int j = 0; j < 6; ++j) a.quad[6 * size_t(id) + j] = q[j];
+    }
+    return r;
+}
+
+// Moments of internal node id from its children, in octant order
+// (nbody.cpp:136-147); the quadrupole sums the children's tensors moved by
+// the parallel-axis rule.  Modes as leaf_moments.
+template <int MODE, typename Kids>
+__device__ NodeMoments internal_moments(const MomentArgs& a, int id, const Kids& k,
+                                        const double4& bx, const double4& held) {
+#pragma clang fp contract(off)
+    NodeMoments r;
+    double4 c = held;
+    double4 cm[8];
+    r.dm = 0;
+    if constexpr (MODE != 2) {
+        double M = 0, sx = 0, sy = 0, sz = 0, dm = 0;
+        double nd[8];
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+            cm[o] = k.com(o);
+            nd[o] = k.nd(o);
+        }
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+            if (!k.has(o)) continue;
+            M += cm[o].w;
+            sx += cm[o].w * cm[o].x;
+            sy += cm[o].w * cm[o].y;
+            sz += cm[o].w * cm[o].z;
+            dm = fmax(dm, nd[o]);
+        }
+        c = finalize_node(id, M, sx, sy, sz, dm, a.theta_d, bx, a.com, a.open_r, a.ndisp);
+        r.dm = dm;
+    } else {
+#pragma unroll
+        for (int o = 0; o < 8; ++o) cm[o] = k.com(o);
+    }
+    r.c = c;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) r.q[j] = 0;
+    if constexpr (MODE >= 1) {
+        // QB children's tensors loaded at a time, summed in octant order
+        constexpr int QB = Kids::QBATCH;
+        double* q = r.q;
+#pragma unroll
+        for (int o0 = 0; o0 < 8; o0 += QB) {
+            double qq[QB][6];
+#pragma unroll
+            for (int u = 0; u < QB; ++u)
+#pragma unroll
+                for (int j = 0; j < 6; ++j) qq[u][j] = k.q(o0 + u, j);
+#pragma unroll
+            for (int u = 0; u < QB; ++u) {
+                const int o = o0 + u;
+                if (!k.has(o)) continue;
+                for (int j = 0; j < 6; ++j) q[j] += qq[u][j];
+                quad_add(q, cm[o].w, cm[o].x - c.x, cm[o].y - c.y, cm[o].z - c.z);
             }
         }
         for (int j = 0; j < 6; ++j) a.quad[6 * size_t(id) + j] = q[j];
     }
+    return r;
 }
 
-// Bottom-up pass, first half.  A subtree root is a node of at most
-// MOMENT_SUB particles whose parent holds more (or the root itself); its
-// subtree is the DFS node range [root, next[root]).  Each block scans a
-// chunk of node ids for subtree roots.  The thread that finds a root of at
-// most MOMENT_SMALL nodes finishes that subtree alone in reverse DFS order
-// (a node's children follow it in DFS order); the block finishes each larger
-// one together, deepest level first, a barrier between levels.  Nodes of
-// more than MOMENT_SUB particles are listed in top[] (*ntop) for the second
-// half.  No node is touched by two threads.
-constexpr int MOMENT_SUB = 2048;
-constexpr int MOMENT_SMALL = 32;
+// Bottom-up pass, first half.  The subtree of node id is its DFS range
+// [id, next[id]); a subtree root is a node whose subtree has at most
+// MOMENT_NCAP nodes while its parent's has more (or the root itself).  The
+// block of chunk [c0, c0 + MOMENT_NCAP) finishes the subtrees whose roots lie
+// in the chunk: they are disjoint and inside [c0, c0 + 2 MOMENT_NCAP), so one
+// LDS window of 2 MOMENT_NCAP entries holds their moments.  A node belongs to
+// one of them iff the largest subtree end among the chunk's roots at or
+// before it lies past it (a prefix maximum).  Each thread owns entries t and
+// t + MOMENT_NCAP of the window and loads what they need first; then the
+// leaves (their particles), then the internal nodes one depth at a time,
+// deepest first, children read from LDS, so a depth waits on LDS only.
+// Nodes whose subtree is larger are listed in top[] (*ntop) for the second
+// half.  Three blocks a CU (two with the quadrupoles, whose window takes
+// 61 KB of LDS).
+constexpr int MOMENT_NCAP = TB;
+
+__device__ __forceinline__ int wave_incl_max(int v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v = max(v, u);
+    }
+    return v;
+}
 
 template <int MODE>
-__global__ __launch_bounds__(TB) void k_moments_subtrees(MomentArgs a,
-                                                         const int* __restrict__ counters,
-                                                         int* __restrict__ top,
-                                                         int* __restrict__ ntop) {
-    __shared__ int roots[TB];
-    __shared__ int nroots, maxd;
+__global__ __launch_bounds__(TB, MODE == 1 ? 2 : 3) void k_moments_subtrees(
+    MomentArgs a, const int* __restrict__ counters, int* __restrict__ top, int* __restrict__ ntop) {
+    constexpr int W = 2 * MOMENT_NCAP;
+    constexpr bool Q = MODE >= 1;
+    __shared__ double4 s_com[W];
+    __shared__ double s_nd[MODE == 2 ? 1 : W];
+    __shared__ double s_q[Q ? 6 * W : 1];
+    __shared__ double4 s_box[MODE == 2 ? 1 : W];
+    __shared__ int s_wmax[TB / 64];
+    __shared__ int s_dlo, s_dhi;
     const int nn = counters[0];
-    for (int c0 = blockIdx.x * TB; c0 < nn; c0 += gridDim.x * TB) {
-        if (threadIdx.x == 0) nroots = 0;
+    const int t = threadIdx.x, lane = __lane_id(), w = t >> 6;
+    for (int c0 = blockIdx.x * MOMENT_NCAP; c0 < nn; c0 += gridDim.x * MOMENT_NCAP) {
+        if (t == 0) {
+            s_dlo = 1 << 20;
+            s_dhi = -1;
+        }
+        int endm = 0;
+        {
+            const int id = c0 + t;
+            if (id < nn) {
+                const int sz = min(a.next[id], nn) - id;
+                if (sz > MOMENT_NCAP) {
+                    top[atomicAdd(ntop, 1)] = id;
+                } else {
+                    const int p = a.parent[id];
+                    if (p < 0 || min(a.next[p], nn) - p > MOMENT_NCAP) endm = id + sz;
+                }
+            }
+        }
+        int pmax = wave_incl_max(endm);
+        if (lane == 63) s_wmax[w] = pmax;
         __syncthreads();
-        const int id = c0 + threadIdx.x;
-        if (id < nn) {
-            if (a.count[id] > MOMENT_SUB) {
-                top[atomicAdd(ntop, 1)] = id;
-            } else {
-                const int p = a.parent[id];
-                if (p < 0 || a.count[p] > MOMENT_SUB) {
-                    const int e = min(a.next[id], nn);
-                    if (e - id <= MOMENT_SMALL) {
-                        for (int k = e - 1; k >= id; --k) node_moments<MODE>(a, k);
-                    } else {
-                        roots[atomicAdd(&nroots, 1)] = id;
+        int all = 0;
+        for (int q = 0; q < TB / 64; ++q) {
+            if (q < w) pmax = max(pmax, s_wmax[q]);
+            all = max(all, s_wmax[q]);
+        }
+        // what the owned entries need, loaded up front
+        bool own[2], leaf[2];
+        int dep[2], pb[2], pc[2];
+        KidsLds kids[2];
+        for (int h = 0; h < 2; ++h) {
+            const int id = c0 + t + h * MOMENT_NCAP;
+            own[h] = id < nn && (h == 0 ? pmax : all) > id;
+            leaf[h] = false;
+            dep[h] = pb[h] = pc[h] = 0;
+            kids[h] = KidsLds{s_com, s_nd, s_q, {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}};
+            if (own[h]) {
+                // one round of loads (a leaf's child slots are all empty)
+                leaf[h] = a.leaf[id] != 0;
+                dep[h] = a.depth[id];
+                pb[h] = a.start[id];
+                pc[h] = a.count[id];
+                if constexpr (MODE != 2) s_box[t + h * MOMENT_NCAP] = a.box[id];
+                if constexpr (MODE == 2) s_com[t + h * MOMENT_NCAP] = a.com[id];
+                {
+                    // the children lie in the same subtree, so inside the window
+                    const int4* sl = reinterpret_cast<const int4*>(a.child8 + 8 * size_t(id));
+                    const int4 s0 = sl[0], s1 = sl[1];
+                    const int cs[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+#pragma unroll
+                    for (int o = 0; o < 8; ++o) {
+                        const uint32_t ent = cs[o] >= 0 ? uint32_t(cs[o] - c0) : 0xFFFFu;
+                        const uint32_t sh = 16u * uint32_t(o & 1);
+                        kids[h].e[o >> 1] = (kids[h].e[o >> 1] & ~(0xFFFFu << sh)) | (ent << sh);
                     }
                 }
             }
         }
         __syncthreads();
-        const int nr = nroots;
-        for (int q = 0; q < nr; ++q) {
-            const int r = roots[q];
-            const int e = min(a.next[r], nn);
-            const int d0 = a.depth[r];
-            if (threadIdx.x == 0) maxd = d0;
-            __syncthreads();
-            int md = d0;
-            for (int k = r + threadIdx.x; k < e; k += TB) md = max(md, int(a.depth[k]));
-            atomicMax(&maxd, md);
-            __syncthreads();
-            const int dmax = maxd;
-            for (int d = dmax; d >= d0; --d) {
-                for (int k = r + threadIdx.x; k < e; k += TB)
-                    if (a.depth[k] == d) node_moments<MODE>(a, k);
-                __syncthreads();
+        int dlo = 1 << 20, dhi = -1;
+        for (int h = 0; h < 2; ++h) {
+            if (!own[h]) continue;
+            const int e = t + h * MOMENT_NCAP;
+            if (leaf[h]) {
+                const int id = c0 + e;
+                const double4 held = MODE == 2 ? s_com[e] : make_double4(0, 0, 0, 0);
+                const double4 bx = MODE != 2 ? s_box[e] : make_double4(0, 0, 0, 0);
+                const NodeMoments r = leaf_moments<MODE>(a, id, pb[h], pc[h], bx, held);
+                if constexpr (MODE != 2) {
+                    s_com[e] = r.c;
+                    s_nd[e] = r.dm;
+                }
+                if constexpr (Q)
+                    for (int j = 0; j < 6; ++j) s_q[6 * e + j] = r.q[j];
+            } else {
+                dlo = min(dlo, dep[h]);
+                dhi = max(dhi, dep[h]);
             }
         }
+        if (dhi >= 0) {
+            atomicMin(&s_dlo, dlo);
+            atomicMax(&s_dhi, dhi);
+        }
+        __syncthreads();
+        const int d_lo = s_dlo, d_hi = s_dhi;
+        for (int d = d_hi; d >= d_lo; --d) {
+            for (int h = 0; h < 2; ++h) {
+                if (!own[h] || leaf[h] || dep[h] != d) continue;
+                const int e = t + h * MOMENT_NCAP;
+                const double4 held = MODE == 2 ? s_com[e] : make_double4(0, 0, 0, 0);
+                const double4 bx = MODE != 2 ? s_box[e] : make_double4(0, 0, 0, 0);
+                const NodeMoments r = internal_moments<MODE>(a, c0 + e, kids[h], bx, held);
+                if constexpr (MODE != 2) {
+                    s_com[e] = r.c;
+                    s_nd[e] = r.dm;
+                }
+                if constexpr (Q)
+                    for (int j = 0; j < 6; ++j) s_q[6 * e + j] = r.q[j];
+            }
+            __syncthreads();
+        }
+        // the window is reused by the next chunk
+        __syncthreads();
     }
 }
 
 // Second half: one block finishes the listed nodes above the subtrees,
-// deepest first, and clears the list's counter for the next pass.
+// deepest first (their children are subtree roots, done by the first half,
+// or listed nodes of the depth below), and clears the list's counter for
+// the next pass.  Up to MOMENT_TOP_MAX listed nodes are sorted by depth in
+// LDS (a counting sort) with their children and boxes, so a depth is one
+// node a thread and one round of loads (the children's moments); a longer
+// list is read from the tree's arrays at every depth.
 template <int MODE>
-__global__ __launch_bounds__(1024) void k_moments_top(MomentArgs a, const int* __restrict__ top,
-                                                      int* __restrict__ ntop) {
-    __shared__ int maxd;
+__device__ __forceinline__ NodeMoments top_node(const MomentArgs& a, int id) {
+    KidsGlobal kg{a, id, {}};
+    const int4* sl = reinterpret_cast<const int4*>(a.child8 + 8 * size_t(id));
+    const int4 s0 = sl[0], s1 = sl[1];
+    kg.cs[0] = s0.x; kg.cs[1] = s0.y; kg.cs[2] = s0.z; kg.cs[3] = s0.w;
+    kg.cs[4] = s1.x; kg.cs[5] = s1.y; kg.cs[6] = s1.z; kg.cs[7] = s1.w;
+    const double4 bx = MODE != 2 ? a.box[id] : make_double4(0, 0, 0, 0);
+    const double4 held = MODE == 2 ? a.com[id] : make_double4(0, 0, 0, 0);
+    return internal_moments<MODE>(a, id, kg, bx, held);
+}
+
+constexpr int MOMENT_TOP_TB = 512, MOMENT_TOP_MAX = 2048;
+constexpr int MOMENT_DEPTHS = 32;            // node depths are at most GF_KEY_LEVELS
+static_assert(GF_KEY_LEVELS < MOMENT_DEPTHS, "depth bins");
+
+template <int MODE>
+__global__ __launch_bounds__(MOMENT_TOP_TB) void k_moments_top(MomentArgs a,
+                                                               const int* __restrict__ top,
+                                                               int* __restrict__ ntop) {
+    constexpr int K = MOMENT_TOP_MAX / MOMENT_TOP_TB;
+    __shared__ int s_id[MOMENT_TOP_MAX];         // listed nodes by depth
+    __shared__ int4 s_cs[MOMENT_TOP_MAX][2];     // their children, octant order
+    __shared__ double4 s_bx[MOMENT_TOP_MAX];     // box (modes 0, 1) or held centre of mass
+    __shared__ int s_cnt[MOMENT_DEPTHS], s_off[MOMENT_DEPTHS], s_pos[MOMENT_DEPTHS];
+    __shared__ int s_dmax;
     const int nt = *ntop;
-    if (threadIdx.x == 0) maxd = 0;
+    const int t = threadIdx.x;
+    if (t < MOMENT_DEPTHS) s_cnt[t] = 0;
+    if (t == 0) s_dmax = 0;
     __syncthreads();
-    int md = 0;
-    for (int k = threadIdx.x; k < nt; k += 1024) md = max(md, int(a.depth[top[k]]));
-    atomicMax(&maxd, md);
-    __syncthreads();
-    const int dmax = maxd;
-    for (int d = dmax; d >= 0; --d) {
-        for (int k = threadIdx.x; k < nt; k += 1024) {
-            const int id = top[k];
-            if (a.depth[id] == d) node_moments<MODE>(a, id);
+    if (nt == 0) {
+        // nothing above the subtrees
+    } else if (nt <= MOMENT_TOP_MAX) {
+        // counting sort by depth; every load of a round goes out together
+        int id[K], dep[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) id[k] = top[min(t + MOMENT_TOP_TB * k, nt - 1)];
+#pragma unroll
+        for (int k = 0; k < K; ++k) dep[k] = a.depth[id[k]];
+        int md = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (t + MOMENT_TOP_TB * k < nt) {
+                atomicAdd(&s_cnt[dep[k]], 1);
+                md = max(md, dep[k]);
+            }
+        atomicMax(&s_dmax, md);
+        __syncthreads();
+        if (t == 0) {
+            int o = 0;
+            for (int d = 0; d < MOMENT_DEPTHS; ++d) {
+                s_off[d] = s_pos[d] = o;
+                o += s_cnt[d];
+            }
         }
         __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (t + MOMENT_TOP_TB * k < nt) s_id[atomicAdd(&s_pos[dep[k]], 1)] = id[k];
+        __syncthreads();
+        int4 c0[K], c1[K];
+        double4 bx[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int nid = s_id[min(t + MOMENT_TOP_TB * k, nt - 1)];
+            const int4* sl = reinterpret_cast<const int4*>(a.child8 + 8 * size_t(nid));
+            c0[k] = sl[0];
+            c1[k] = sl[1];
+            bx[k] = MODE != 2 ? a.box[nid] : a.com[nid];
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int e = t + MOMENT_TOP_TB * k;
+            if (e < nt) {
+                s_cs[e][0] = c0[k];
+                s_cs[e][1] = c1[k];
+                s_bx[e] = bx[k];
+            }
+        }
+        __syncthreads();
+        for (int d = s_dmax; d >= 0; --d) {
+            const int e1 = s_off[d] + s_cnt[d];
+            for (int e = s_off[d] + t; e < e1; e += MOMENT_TOP_TB) {
+                const int4 q0 = s_cs[e][0], q1 = s_cs[e][1];
+                const KidsGlobal kg{a, s_id[e], {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w}};
+                const double4 v = s_bx[e];
+                (void)internal_moments<MODE>(a, s_id[e], kg, v, v);
+            }
+            __syncthreads();
+        }
+    } else {
+        int md = 0;
+        for (int k = t; k < nt; k += MOMENT_TOP_TB) md = max(md, int(a.depth[top[k]]));
+        atomicMax(&s_dmax, md);
+        __syncthreads();
+        for (int d = s_dmax; d >= 0; --d) {
+            for (int k = t; k < nt; k += MOMENT_TOP_TB) {
+                const int id = top[k];
+                if (a.depth[id] == d) (void)top_node<MODE>(a, id);
+            }
+            __syncthreads();
+        }
     }
-    if (threadIdx.x == 0) *ntop = 0;
+    if (t == 0) *ntop = 0;
 }
 
 // Groups (nbody.cpp:234-243): the outermost nodes holding <= 64 particles, or
@@ -644,20 +889,20 @@ static MomentArgs moment_args(const GfTree& T) {
 }
 
 // The bottom-up pass over the held topology in two launches (subtrees, then
-// the nodes above them); mode as node_moments.
+// the nodes above them); mode as leaf_moments.
 static int moments_pass(const GfTree& T, int mode, hipStream_t s) {
     const MomentArgs a = moment_args(T);
     const int ng = node_grid(T.cap_nodes);
     int* ntop = T.dmeta;
     if (mode == 0) {
         hipLaunchKernelGGL(k_moments_subtrees<0>, dim3(ng), dim3(TB), 0, s, a, T.counters, T.dlist, ntop);
-        hipLaunchKernelGGL(k_moments_top<0>, dim3(1), dim3(1024), 0, s, a, T.dlist, ntop);
+        hipLaunchKernelGGL(k_moments_top<0>, dim3(1), dim3(MOMENT_TOP_TB), 0, s, a, T.dlist, ntop);
     } else if (mode == 1) {
         hipLaunchKernelGGL(k_moments_subtrees<1>, dim3(ng), dim3(TB), 0, s, a, T.counters, T.dlist, ntop);
-        hipLaunchKernelGGL(k_moments_top<1>, dim3(1), dim3(1024), 0, s, a, T.dlist, ntop);
+        hipLaunchKernelGGL(k_moments_top<1>, dim3(1), dim3(MOMENT_TOP_TB), 0, s, a, T.dlist, ntop);
     } else {
         hipLaunchKernelGGL(k_moments_subtrees<2>, dim3(ng), dim3(TB), 0, s, a, T.counters, T.dlist, ntop);
-        hipLaunchKernelGGL(k_moments_top<2>, dim3(1), dim3(1024), 0, s, a, T.dlist, ntop);
+        hipLaunchKernelGGL(k_moments_top<2>, dim3(1), dim3(MOMENT_TOP_TB), 0, s, a, T.dlist, ntop);
     }
     GF_LAUNCH_CHECK();
     return 0;
