@@ -62,28 +62,31 @@ constexpr int GSCAP = 512 * (GF_KEY_LEVELS + 2);    // global stack entries per 
 #endif
 constexpr int GST_BLOCKS = 256;
 
-// A tree node packed for the walk (128 bytes): everything one visit reads,
-// so a batch of 64 nodes costs one round of loads.
+// A tree node packed for the walk (128 bytes).  A visit reads the first 96
+// bytes, everything the opening test, the quadrupole rule and the descent
+// need, in one round of loads for a batch of 64 nodes; only a node accepted
+// with its quadrupole reads the tensor in the last 32.
 struct GNode {
     double4 com;          // centre of mass, mass (FP64)
     double open_r;        // opening radius (FP64)
     int start, count;     // particle range in tree order
-    int leaf, nchild, pad1, pad2;
-    int4 c_lo, c_hi;      // the nchild non-empty children in octant order, then -1
-    float q[6];           // traceless quadrupole (xx, yy, zz, xy, xz, yz), FP32
+    int leaf, nchild;
     float qlen;           // sqrt(3 max |Q_ij| / m): the quadrupole term's share of
                           // the monopole at distance d is at most about (qlen / d)^2
     float qabs4;          // (2.858 |Q|_F)^(1/4): the quadrupole term's acceleration at
                           // distance d is at most (qabs4 / d)^4 (3.5 |Q|_2 / d^4 with
                           // |Q|_2 <= sqrt(2/3) |Q|_F for a traceless symmetric Q)
+    int4 c_lo, c_hi;      // the nchild non-empty children in octant order, then -1
+    float q[6];           // traceless quadrupole (xx, yy, zz, xy, xz, yz), FP32
+    float pad[2];
 };
 static_assert(sizeof(GNode) == 128, "GNode layout");
 static_assert(offsetof(GNode, open_r) == 32 && offsetof(GNode, start) == 40 &&
                   offsetof(GNode, leaf) == 48 && offsetof(GNode, nchild) == 52 &&
+                  offsetof(GNode, qlen) == 56 && offsetof(GNode, qabs4) == 60 &&
                   offsetof(GNode, c_lo) == 64 && offsetof(GNode, c_hi) == 80 &&
-                  offsetof(GNode, q) == 96 && offsetof(GNode, qlen) == 120 &&
-                  offsetof(GNode, qabs4) == 124,
-              "the walk reads GNode as eight 16-byte words");
+                  offsetof(GNode, q) == 96,
+              "the walk reads GNode as 16-byte words");
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 struct WalkArgs {
@@ -395,21 +398,13 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 __syncthreads();
                 // every lane loads a record (the root where it has no node),
                 // so the visit is free of branches; the empty asm keeps the
-                // eight loads together, where the compiler would otherwise
-                // move the leaf flag, children and quadrupole into the paths
-                // that use them, one dependent round trip each
+                // six loads together, where the compiler would otherwise
+                // move the leaf flag and children into the paths that use
+                // them, one dependent round trip each
                 const bool valid = node >= 0;
                 const v4i* r = reinterpret_cast<const v4i*>(a.gn + (valid ? node : 0));
                 v4i w0 = r[0], w1 = r[1], w2 = r[2], w3 = r[3], w4 = r[4], w5 = r[5];
-                v4i rq03 = {0, 0, 0, 0}, rq45 = {0, 0, 0, 0};   // QUAD: q[0..3]; q[4], q[5], qlen
-                if constexpr (QUAD) {
-                    rq03 = r[6];
-                    rq45 = r[7];
-                    asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5),
-                                 "+v"(rq03), "+v"(rq45));
-                } else {
-                    asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5));
-                }
+                asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5));
                 const double4 cm = make_double4(__hiloint2double(w0.y, w0.x), __hiloint2double(w0.w, w0.z),
                                                 __hiloint2double(w1.y, w1.x), __hiloint2double(w1.w, w1.z));
                 const double orr = __hiloint2double(w2.y, w2.x);
@@ -430,7 +425,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                     // relative rule, its bound (qabs4 / (d - br))^4 is below
                     // quad_atol |acc|_min iff qabs4 qfac + br < d (estimates:
                     // FP32 is enough)
-                    const float fld = __int_as_float(qrel ? rq45.w : rq45.z);
+                    const float fld = __int_as_float(qrel ? w3.w : w3.z);
                     const float x = fmaf(fld, qfac, float(br));
                     small_q = x * x < float(d2);
                 }
@@ -454,6 +449,9 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 // lanes), then the opened leaves' particles, one per lane
                 const uint64_t mmask = __ballot(mono);
                 const int nmono = __popcll(mmask);
+                int qslot = -1;                                // QUAD: this lane's tensor slot
+                v4i rq03 = {0, 0, 0, 0};                       // q[0..3], then q[4], q[5]
+                int2 rq45 = make_int2(0, 0);
                 if (nmono > 0) {
                     if constexpr (QUAD) {
                         // nodes whose quadrupole term is negligible join the
@@ -473,14 +471,14 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                                 __syncthreads();
                             }
                             if (quad) {
-                                const int slot = nq + __popcll(qmask & gf_lanemask_lt());
-                                s_qe[slot] = make_float4(float(cm.x - bx), float(cm.y - by),
-                                                         float(cm.z - bz), float(cm.w));
-                                // (xx, yy, xy, xz) and yz of the record's
-                                // (xx, yy, zz, xy), (xz, yz, ...)
-                                s_qa[slot] = make_float4(__int_as_float(rq03.x), __int_as_float(rq03.y),
-                                                         __int_as_float(rq03.w), __int_as_float(rq45.x));
-                                s_qb[slot] = __int_as_float(rq45.y);
+                                // the tensor is loaded now and lands in its
+                                // slot after this batch's particles: no flush
+                                // of the quadrupole list comes in between
+                                rq03 = r[6];
+                                rq45 = *reinterpret_cast<const int2*>(r + 7);
+                                qslot = nq + __popcll(qmask & gf_lanemask_lt());
+                                s_qe[qslot] = make_float4(float(cm.x - bx), float(cm.y - by),
+                                                          float(cm.z - bz), float(cm.w));
                             }
                             nq += nquad;
                         }
@@ -551,6 +549,14 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                     }
                     ln += m;
                     e0 += m;
+                }
+                if constexpr (QUAD) {
+                    if (qslot >= 0) {
+                        // (xx, yy, xy, xz) and yz of the record's (xx, yy, zz, xy), (xz, yz)
+                        s_qa[qslot] = make_float4(__int_as_float(rq03.x), __int_as_float(rq03.y),
+                                                  __int_as_float(rq03.w), __int_as_float(rq45.x));
+                        s_qb[qslot] = __int_as_float(rq45.y);
+                    }
                 }
                 __syncthreads();
             }
@@ -630,7 +636,7 @@ __global__ __launch_bounds__(256) void k_gnode_pack(const int* __restrict__ coun
         g.start = start[id];
         g.count = count[id];
         g.leaf = leaf[id];
-        g.pad1 = g.pad2 = 0;
+        g.pad[0] = g.pad[1] = 0.0f;
         // the non-empty children first, in octant order
         int cc[8], nc = 0;
         for (int k = 0; k < 8; ++k) cc[k] = -1;
