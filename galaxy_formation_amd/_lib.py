@@ -31,7 +31,7 @@ SIGNATURES = {
     "gf_version": (_ci, []),
     "gf_device_info": (_ci, [ctypes.c_char_p, _ci]),
     "gf_set_precision": (_ci, [_ci]),
-    "gf_set_test_limits": (_ci, [_ci, _ci]),
+    "gf_set_test_limits": (_ci, [_ci, _ci, _ci]),
     "gf_set_quad_tolerance": (_ci, [_cd]),
     "gf_get_quad_tolerance": (_cd, []),
     "gf_set_quad_relative_tolerance": (_ci, [_cd]),

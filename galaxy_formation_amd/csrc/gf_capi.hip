@@ -171,13 +171,16 @@ int gf_get_walk_blocks(void) {
     return g_ctx.walk.walk_blocks > 0 ? g_ctx.walk.walk_blocks : gf_walk_default_blocks();
 }
 
-int gf_set_test_limits(int walk_stack, int knn_list) {
-    if (walk_stack < 0 || walk_stack > 1024 || knn_list < 0 || knn_list > 1024) {
-        gf_set_error("test limits: walk_stack and knn_list must be in [0, 1024] (0 = default)");
+int gf_set_test_limits(int walk_stack, int knn_list, int moment_top) {
+    if (walk_stack < 0 || walk_stack > 1024 || knn_list < 0 || knn_list > 1024 ||
+        moment_top < 0 || moment_top > 2048) {
+        gf_set_error("test limits: walk_stack and knn_list must be in [0, 1024], moment_top in "
+                     "[0, 2048] (0 = default)");
         return -1;
     }
     gf_test_limits().walk_stack = walk_stack;
     gf_test_limits().knn_list = knn_list;
+    gf_test_limits().moment_top = moment_top;
     return 0;
 }
 

@@ -130,9 +130,11 @@ __device__ __forceinline__ int gf_xcd_block() {
 }
 
 // Capacities that tests lower to drive the fallback paths (0 = default):
-// the walk's LDS stack entries and the neighbour solve's candidate list.
+// the walk's LDS stack entries, the neighbour solve's candidate list and
+// the nodes above the bottom-up subtrees finished from LDS.
 struct GfTestLimits {
     int walk_stack = 0;
     int knn_list = 0;
+    int moment_top = 0;
 };
 GfTestLimits& gf_test_limits();

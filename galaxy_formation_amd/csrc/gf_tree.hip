@@ -669,10 +669,10 @@ __global__ __launch_bounds__(TB, MODE == 1 ? 2 : 3) void k_moments_subtrees(
 // Second half: one block finishes the listed nodes above the subtrees,
 // deepest first (their children are subtree roots, done by the first half,
 // or listed nodes of the depth below), and clears the list's counter for
-// the next pass.  Up to MOMENT_TOP_MAX listed nodes are sorted by depth in
-// LDS (a counting sort) with their children and boxes, so a depth is one
-// node a thread and one round of loads (the children's moments); a longer
-// list is read from the tree's arrays at every depth.
+// the next pass.  Up to top_max (<= MOMENT_TOP_MAX) listed nodes are sorted
+// by depth in LDS (a counting sort) with their children and boxes, so a
+// depth is one node a thread and one round of loads (the children's
+// moments); a longer list is read from the tree's arrays at every depth.
 template <int MODE>
 __device__ __forceinline__ NodeMoments top_node(const MomentArgs& a, int id) {
     KidsGlobal kg{a, id, {}};
@@ -692,7 +692,8 @@ static_assert(GF_KEY_LEVELS < MOMENT_DEPTHS, "depth bins");
 template <int MODE>
 __global__ __launch_bounds__(MOMENT_TOP_TB) void k_moments_top(MomentArgs a,
                                                                const int* __restrict__ top,
-                                                               int* __restrict__ ntop) {
+                                                               int* __restrict__ ntop,
+                                                               int top_max) {
     constexpr int K = MOMENT_TOP_MAX / MOMENT_TOP_TB;
     __shared__ int s_id[MOMENT_TOP_MAX];         // listed nodes by depth
     __shared__ int4 s_cs[MOMENT_TOP_MAX][2];     // their children, octant order
@@ -706,7 +707,7 @@ __global__ __launch_bounds__(MOMENT_TOP_TB) void k_moments_top(MomentArgs a,
     __syncthreads();
     if (nt == 0) {
         // nothing above the subtrees
-    } else if (nt <= MOMENT_TOP_MAX) {
+    } else if (nt <= top_max) {
         // counting sort by depth; every load of a round goes out together
         int id[K], dep[K];
 #pragma unroll
@@ -894,15 +895,20 @@ static int moments_pass(const GfTree& T, int mode, hipStream_t s) {
     const MomentArgs a = moment_args(T);
     const int ng = node_grid(T.cap_nodes);
     int* ntop = T.dmeta;
+    const int tl = gf_test_limits().moment_top;
+    const int top_max = tl > 0 ? std::min(tl, MOMENT_TOP_MAX) : MOMENT_TOP_MAX;
     if (mode == 0) {
         hipLaunchKernelGGL(k_moments_subtrees<0>, dim3(ng), dim3(TB), 0, s, a, T.counters, T.dlist, ntop);
-        hipLaunchKernelGGL(k_moments_top<0>, dim3(1), dim3(MOMENT_TOP_TB), 0, s, a, T.dlist, ntop);
+        hipLaunchKernelGGL(k_moments_top<0>, dim3(1), dim3(MOMENT_TOP_TB), 0, s, a, T.dlist, ntop,
+                           top_max);
     } else if (mode == 1) {
         hipLaunchKernelGGL(k_moments_subtrees<1>, dim3(ng), dim3(TB), 0, s, a, T.counters, T.dlist, ntop);
-        hipLaunchKernelGGL(k_moments_top<1>, dim3(1), dim3(MOMENT_TOP_TB), 0, s, a, T.dlist, ntop);
+        hipLaunchKernelGGL(k_moments_top<1>, dim3(1), dim3(MOMENT_TOP_TB), 0, s, a, T.dlist, ntop,
+                           top_max);
     } else {
         hipLaunchKernelGGL(k_moments_subtrees<2>, dim3(ng), dim3(TB), 0, s, a, T.counters, T.dlist, ntop);
-        hipLaunchKernelGGL(k_moments_top<2>, dim3(1), dim3(MOMENT_TOP_TB), 0, s, a, T.dlist, ntop);
+        hipLaunchKernelGGL(k_moments_top<2>, dim3(1), dim3(MOMENT_TOP_TB), 0, s, a, T.dlist, ntop,
+                           top_max);
     }
     GF_LAUNCH_CHECK();
     return 0;

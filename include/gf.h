@@ -110,11 +110,13 @@ int gf_get_knn_blocks(void);
 int gf_set_walk_blocks(int blocks);
 int gf_get_walk_blocks(void);
 /* Testing hook, no counterpart in the reference: lowers the capacity of
- * the gravity walk's LDS stack (walk_stack entries, 1..1024) and of the
- * neighbour solve's candidate lists (knn_list, 1..1024) so that tests can
- * drive the global-memory and heap fallbacks on small inputs; 0 restores
- * the default.  Results do not depend on these capacities. */
-int gf_set_test_limits(int walk_stack, int knn_list);
+ * the gravity walk's LDS stack (walk_stack entries, 1..1024), of the
+ * neighbour solve's candidate lists (knn_list, 1..1024) and of the tree
+ * build's LDS list of nodes above its bottom-up subtrees (moment_top,
+ * 1..2048) so that tests can drive the global-memory, heap and list
+ * fallbacks on small inputs; 0 restores the default.  Results do not depend
+ * on these capacities. */
+int gf_set_test_limits(int walk_stack, int knn_list, int moment_top);
 /* Interactions (sum over evaluated particles of their list length) of the
  * last nbody_accel_active call (the count nbody.cpp:397 loops over). */
 long long gf_last_interactions(void);

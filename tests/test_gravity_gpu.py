@@ -99,6 +99,38 @@ def test_tree_fp64_centres_equal_oracle(eng):
     np.testing.assert_array_equal(t["open_r"], o["dopen_r"])
 
 
+@pytest.mark.parametrize("top", [1, 16])
+def test_moments_above_subtrees_fallback(eng, top):
+    """The build finishes the nodes above its LDS subtrees (41 of them for
+    this cloud) from a depth-sorted LDS list, or, past the list's capacity
+    (lowered here to `top`), from the tree's arrays at every depth: the same
+    centres of mass and opening radii, bit for bit and equal to the
+    oracle's, and the same quadrupoles (through the quadrupole walk)."""
+    from galaxy_formation_amd import _lib
+    L = _lib.load()
+    c = ics.make_cloud(20000, 20000, seed=7)
+    run(eng, c.pos, c.mass, 0.5, c.eps, "fp64")
+    ref_t = eng.tree_export()
+    ref_q = run(eng, c.pos, c.mass, 0.5, c.eps, "fp32q")
+    _lib.check(L.gf_set_test_limits(0, 0, top))
+    try:
+        run(eng, c.pos, c.mass, 0.5, c.eps, "fp64")
+        t = eng.tree_export()
+        q = run(eng, c.pos, c.mass, 0.5, c.eps, "fp32q")
+    finally:
+        _lib.check(L.gf_set_test_limits(0, 0, 0))
+    nn = len(t["start"])
+    size = np.minimum(t["next"], nn) - np.arange(nn)
+    assert (size > 256).sum() > top
+    np.testing.assert_array_equal(t["com4"], ref_t["com4"])
+    np.testing.assert_array_equal(t["open_r"], ref_t["open_r"])
+    np.testing.assert_array_equal(q, ref_q)
+    O.accel(c.pos, c.mass, 0.5, c.eps, mode=1)
+    o = O.tree_export()
+    np.testing.assert_array_equal(t["com4"], o["dcom4"])
+    np.testing.assert_array_equal(t["open_r"], o["dopen_r"])
+
+
 @pytest.mark.parametrize("theta", [0.4, 0.5, 0.7])
 def test_fp64_equals_oracle(eng, theta):
     c = ics.config("cloud10k")
@@ -282,12 +314,12 @@ def test_groups_over_64_particles_through_the_global_stack(eng, prec):
     ref = run(eng, pos, mass, 0.5, 0.05, prec)
     inter = eng.last_interactions()
     assert eng.tree_info()[2] >= 60
-    _lib.check(L.gf_set_test_limits(8, 0))
+    _lib.check(L.gf_set_test_limits(8, 0, 0))
     try:
         acc = run(eng, pos, mass, 0.5, 0.05, prec)
         assert eng.last_interactions() == inter
     finally:
-        _lib.check(L.gf_set_test_limits(0, 0))
+        _lib.check(L.gf_set_test_limits(0, 0, 0))
     np.testing.assert_array_equal(acc, ref)
     if prec == "fp64":
         oref, _ = O.accel(pos, mass, 0.5, 0.05, mode=1)
@@ -304,14 +336,14 @@ def test_global_stack_walk_equals_lds_walk(eng, limit):
     c = ics.config("cloud10k")
     ref = {p: (run(eng, c.pos, c.mass, 0.5, c.eps, p), eng.last_interactions())
            for p in ("fp32", "fp64")}
-    _lib.check(L.gf_set_test_limits(limit, 0))
+    _lib.check(L.gf_set_test_limits(limit, 0, 0))
     try:
         for p in ("fp32", "fp64"):
             acc = run(eng, c.pos, c.mass, 0.5, c.eps, p)
             np.testing.assert_array_equal(acc, ref[p][0])
             assert eng.last_interactions() == ref[p][1]
     finally:
-        _lib.check(L.gf_set_test_limits(0, 0))
+        _lib.check(L.gf_set_test_limits(0, 0, 0))
 
 
 @pytest.mark.parametrize("theta", [0.4, 0.5, 0.7])

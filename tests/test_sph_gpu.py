@@ -389,13 +389,13 @@ def test_neighbours_with_small_candidate_lists(k, cap, first):
     L = _lib.load()
     pos, *_ = disk_gas(6000, seed=13)
     od, onb = O.knn(pos, k)
-    _lib.check(L.gf_set_test_limits(0, cap))
+    _lib.check(L.gf_set_test_limits(0, cap, 0))
     try:
         nb, dk, passes = _device_solve(pos, k, None if first is None else first * od[:, -1])
         st = np.zeros(6, np.int64)
         _lib.check(L.gf_hydro_stats(st, 6))
     finally:
-        _lib.check(L.gf_set_test_limits(0, 0))
+        _lib.check(L.gf_set_test_limits(0, 0, 0))
     np.testing.assert_array_equal(np.sort(nb, axis=1), np.sort(onb, axis=1))
     np.testing.assert_array_equal(dk, od[:, -1])
     if cap < 100:
