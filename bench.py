@@ -44,7 +44,7 @@ PEAK_FP64 = 78.6e12
 # 0 monopole, 2 quadrupole): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from
 # rocprofv3 --pmc (profiles/); counters cannot be read inside a run, so the
 # measured figures are carried here
-WALK_TRAFFIC_BYTES = {0: 3527188960, 2: 3668275230}
+WALK_TRAFFIC_BYTES = {0: 3485224288, 2: 3654067808}
 # the same for one evaluation's FP32-pair SPH sums on the 1M disk's gas
 # (k_pack32 + k_density32 + k_force32)
 SPH_TRAFFIC_BYTES = 641058784
@@ -86,6 +86,10 @@ def parse():
                    help="overlapped step: the gas tree and the neighbour solve first, the walk "
                         "after the solve (knn-then-walk); the gas passes enqueued before the "
                         "walk, beside it (knn-first); or the walk enqueued first (walk-first)")
+    p.add_argument("--build-alone", type=int, default=None, metavar="PREC",
+                   help="profiling aid: the workload's gravity tree (with the quadrupoles "
+                        "for PREC 2) and gas tree, one warm-up and five builds each, "
+                        "nothing else; prints their mean durations")
     p.add_argument("--knn-alone", action="store_true",
                    help="profiling aid: the gas neighbour solve alone, three times from a "
                         "first radius of 1.083 d_k (about the learned proposal's margin)")
@@ -165,6 +169,32 @@ def walk_alone_only(a):
     ms = B.walk_alone(a.walk_alone, launches=3)
     print(f"walk alone, precision {a.walk_alone}, {a.workload}: {ms:.3f} ms per launch "
           f"(mean of 3 after one warm-up)", flush=True)
+
+
+def build_alone_only(a):
+    """The gravity and gas tree builds of the workload and nothing else, for
+    trace runs.  For PREC 2 one quadrupole walk first makes the following
+    builds compute the quadrupoles with the centres of mass."""
+    B = Bench(a)
+    torch, L, chk = B.torch, B.L, B._lib.check
+    chk(L.gf_tree_build_dev(B.n, B.p(B.pos), B.p(B.mass), a.theta, B.sp))
+    if a.build_alone == 2:
+        chk(L.gf_walk_dev(B.n, None, B.p(B.acc_g[2]), B.c.eps, 2, B.sp))
+    ms = {"gravity tree": [], "gas tree": []}
+    for it in range(6):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(B.stream)
+        chk(L.gf_tree_build_dev(B.n, B.p(B.pos), B.p(B.mass), a.theta, B.sp))
+        e[1].record(B.stream)
+        chk(L.gf_hydro_setup_dev(B.ng, B.p(B.pos), B.p(B.vel), B.p(B.mass), B.p(B.u), B.sp))
+        e[2].record(B.stream)
+        torch.cuda.synchronize()
+        if it > 0:
+            ms["gravity tree"].append(e[0].elapsed_time(e[1]))
+            ms["gas tree"].append(e[1].elapsed_time(e[2]))
+    print(f"builds alone, precision {a.build_alone}, {a.workload}: " +
+          ", ".join(f"{k} {np.mean(v):.3f} ms" for k, v in ms.items()) +
+          " (mean of 5 after one warm-up)", flush=True)
 
 
 class Bench:
@@ -516,6 +546,9 @@ def main():
         return
     if a.knn_alone:
         knn_alone_only(a)
+        return
+    if a.build_alone is not None:
+        build_alone_only(a)
         return
     B = Bench(a)
     torch, n, ng, c = B.torch, B.n, B.ng, B.c

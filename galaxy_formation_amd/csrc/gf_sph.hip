@@ -10,8 +10,8 @@
 // nearest; otherwise r_i grows and the particle takes another pass.  The
 // fast pass (k_knn_fast) serves one group of <= 64 gas particles per block of
 // four waves; groups it cannot finish go to a heap pass (k_knn_pass: one wave
-// per group, a max-heap of the kq best (d^2, input index) per lane in LDS),
-// which also writes rows sorted when asked.  Distances are FP64 with the
+// per group, a max-heap of the kq best (d^2, input index) per lane in LDS).
+// A solve asked for sorted rows sorts each row at the end (k_sort_rows).  Distances are FP64 with the
 // operations of the oracle (no contraction), so neighbour sets and d_k are
 // bit-identical to the brute-force oracle whatever the first radius.
 //
@@ -69,7 +69,7 @@ struct KnnArgs {
     int* passes;
     int* nb;
     int n, kq;
-    int tree_pos_out, sorted;
+    int tree_pos_out;
     int* state;     // [0] unresolved, [1] cursor of the fast pass, [2] cursor of the
                     // heap pass, [3] queue 1 length, [4] errors, [5] cursor of the
                     // global-stack heap pass, [6] queue 2 length, [7] retry list
@@ -81,7 +81,6 @@ struct KnnArgs {
     int* ovf;       // queue 1: groups the fast pass hands to the heap pass
     int* ovf2;      // queue 2: groups the heap pass hands to its global-stack variant
     int* gstack;
-    int use_queue;  // the heap pass walks queue 1 (else every group)
     const int* retry_in;    // groups of this pass (null = all; count in state[8])
     int* retry_out;         // groups that need another pass (count in state[7])
     int* gmark;             // pass tag of the last retry request per group
@@ -118,8 +117,8 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
     const int lane = __lane_id();
     int* stack = GST ? a.gstack + size_t(blockIdx.x) * KGSCAP : s_stack;
     const int scap = GST ? KGSCAP : KSCAP;
-    const int* list = GST ? a.ovf2 : (a.use_queue ? a.ovf : nullptr);
-    const int ngroups = GST ? a.state[6] : (a.use_queue ? a.state[3] : a.counters[1]);
+    const int* list = GST ? a.ovf2 : a.ovf;
+    const int ngroups = GST ? a.state[6] : a.state[3];
     if (ngroups == 0) return;          // nothing queued (the common case)
     int* cursor = a.state + (GST ? 5 : 2);
     const int kq = a.kq;
@@ -129,7 +128,7 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
         if (lane == 0) gi = atomicAdd(cursor, 1);
         gi = __shfl(gi, 0, 64);
         if (gi >= ngroups) break;
-        const int gp = list ? list[gi] : gi;
+        const int gp = list[gi];
         const int G = a.groups[gp];
         const int b0 = a.start[G], np = a.count[G];
         for (int c0 = 0; c0 < np; c0 += 64) {
@@ -293,40 +292,7 @@ __global__ __launch_bounds__(64) void k_knn_pass(KnnArgs a) {
             if (mine) {
                 a.passes[t] += 1;
                 if (cnt >= kq) {
-                    if (a.sorted) {
-                        // heapsort in place: ascending (d^2, index)
-                        for (int end = kq - 1; end > 0; --end) {
-                            const double td = hd[end * 64 + lane];
-                            const int ti = hi[end * 64 + lane];
-                            hd[end * 64 + lane] = hd[lane];
-                            hi[end * 64 + lane] = hi[lane];
-                            int p = 0;
-                            while (true) {
-                                int q = 2 * p + 1;
-                                if (q >= end) break;
-                                double qd = hd[q * 64 + lane];
-                                int qi = hi[q * 64 + lane];
-                                if (q + 1 < end) {
-                                    const double rd = hd[(q + 1) * 64 + lane];
-                                    const int rr = hi[(q + 1) * 64 + lane];
-                                    if (key_less(qd, qi, rd, rr)) {
-                                        ++q;
-                                        qd = rd;
-                                        qi = rr;
-                                    }
-                                }
-                                if (!key_less(td, ti, qd, qi)) break;
-                                hd[p * 64 + lane] = qd;
-                                hi[p * 64 + lane] = qi;
-                                p = q;
-                            }
-                            hd[p * 64 + lane] = td;
-                            hi[p * 64 + lane] = ti;
-                        }
-                        a.dk[t] = sqrt(hd[(kq - 1) * 64 + lane]);
-                    } else {
-                        a.dk[t] = sqrt(hd[lane]);
-                    }
+                    a.dk[t] = sqrt(hd[lane]);            // the heap's top: the kq-th
                     for (int s = 0; s < kq; ++s) {
                         const int id = hi[s * 64 + lane];
                         a.nb[size_t(s) * a.n + t] = a.tree_pos_out ? a.inv[id] : id;
@@ -1035,6 +1001,79 @@ __global__ __launch_bounds__(FT, Shape::BLOCKS_PER_CU) void k_knn_fast(KnnArgs a
     }
 }
 
+// Sorts each row of a solved neighbour table by (d^2, input index),
+// ascending, the order the heap pass writes in its sorted mode (sph.py:165-166
+// order): one wave a row, its kq <= 64 NS entries on NS slots a lane, a
+// bitonic network of shuffles.  d^2 is the selection's FP64 value (member
+// minus neighbour, no contraction), so the order is that of the selection.
+template <int NS>
+__global__ __launch_bounds__(256) void k_sort_rows(int n, int kq, const double4* __restrict__ p4,
+                                                   const uint32_t* __restrict__ idx,
+                                                   const int* __restrict__ inv, int tree_pos_out,
+                                                   int* __restrict__ nb) {
+#pragma clang fp contract(off)
+    const int lane = __lane_id();
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= n) return;                    // whole waves leave together
+    const double4 q = p4[t];
+    double d[NS];
+    int id[NS], pos[NS];
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+        const int sl = lane + 64 * r;
+        d[r] = INFINITY;
+        id[r] = 0x7fffffff;
+        pos[r] = 0;
+        if (sl < kq) {
+            const int v = nb[size_t(sl) * n + t];
+            const int ps = tree_pos_out ? v : inv[v];
+            const double4 c = p4[ps];
+            const double dx = q.x - c.x, dy = q.y - c.y, dz = q.z - c.z;
+            d[r] = dx * dx + dy * dy + dz * dz;
+            id[r] = int(idx[ps]);
+            pos[r] = ps;
+        }
+    }
+    constexpr int N = 64 * NS;
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j == 64) {
+                // NS == 2, k == 128: slot 0 keeps the smaller of the lane's two
+                if constexpr (NS == 2) {
+                    if (key_less(d[1], id[1], d[0], id[0])) {
+                        const double td = d[0]; d[0] = d[1]; d[1] = td;
+                        const int ti = id[0]; id[0] = id[1]; id[1] = ti;
+                        const int tp = pos[0]; pos[0] = pos[1]; pos[1] = tp;
+                    }
+                }
+                continue;
+            }
+#pragma unroll
+            for (int r = 0; r < NS; ++r) {
+                const int i = lane + 64 * r;
+                const double od = __shfl_xor(d[r], j, 64);
+                const int oi = __shfl_xor(id[r], j, 64);
+                const int op = __shfl_xor(pos[r], j, 64);
+                const bool want_min = ((i & j) == 0) == ((i & k) == 0);
+                const bool take = want_min ? key_less(od, oi, d[r], id[r])
+                                           : key_less(d[r], id[r], od, oi);
+                if (take) {
+                    d[r] = od;
+                    id[r] = oi;
+                    pos[r] = op;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+        const int sl = lane + 64 * r;
+        if (sl < kq) nb[size_t(sl) * n + t] = tree_pos_out ? pos[r] : id[r];
+    }
+}
+
 // Packed gas-tree nodes and particle positions for the neighbour walks.
 __global__ __launch_bounds__(TB) void k_knode_pack(const int* __restrict__ counters,
                                                    const double4* __restrict__ box,
@@ -1672,10 +1711,7 @@ static KnnArgs knn_args(GfSph& S) {
     a.anc = S.anc;
     a.r_lo = S.r_lo; a.r_hi = S.r_hi; a.dk = S.dk; a.done = S.done; a.passes = S.passes;
     a.nb = S.nb; a.n = T.n; a.kq = S.knn_kq; a.tree_pos_out = S.knn_pos_out;
-    a.sorted = S.knn_sorted;
     a.state = S.state; a.ovf = S.ovf; a.ovf2 = S.ovf2; a.gstack = S.gstack;
-    // the fast pass writes unordered rows; sorted rows come from the heap pass
-    a.use_queue = S.knn_sorted ? 0 : 1;
     return a;
 }
 
@@ -1691,10 +1727,9 @@ static int knn_enqueue(GfSph& S, int from, int count, hipStream_t s) {
         a.retry_out = rbuf[pass & 1];
         a.pass_tag = pass + 1;
         hipLaunchKernelGGL(k_knn_reset, dim3(1), dim3(64), 0, s, S.state, pass == 0 ? 1 : 0);
-        if (!S.knn_sorted)
-            hipLaunchKernelGGL(S.knn_narrow ? k_knn_fast<KnnNarrow> : k_knn_fast<KnnWide>,
-                               dim3(S.knn_narrow ? gf_sph_fast_blocks(S) : wide_blocks(S)),
-                               dim3(FT), 0, s, a);
+        hipLaunchKernelGGL(S.knn_narrow ? k_knn_fast<KnnNarrow> : k_knn_fast<KnnWide>,
+                           dim3(S.knn_narrow ? gf_sph_fast_blocks(S) : wide_blocks(S)),
+                           dim3(FT), 0, s, a);
         if (S.knn_kq <= 64) {
             hipLaunchKernelGGL((k_knn_pass<64, false>), dim3(KNN_BLOCKS), dim3(64), 0, s, a);
             hipLaunchKernelGGL((k_knn_pass<64, true>), dim3(KNN_GST_BLOCKS), dim3(64), 0, s, a);
@@ -1793,6 +1828,20 @@ int gf_sph_knn_end(GfSph& S, int max_passes, hipStream_t s) {
     S.stat_queued = st[9] + st[3];
     S.stat_queued2 = st[10] + st[6];
     S.last_passes = st[11];
+    if (S.knn_sorted) {
+        // the fast pass writes a row's neighbours unordered
+        const GfTree& T = S.tree;
+        const int nblk = gf_div_up(T.n, 4);
+        if (S.knn_kq <= 64)
+            hipLaunchKernelGGL(k_sort_rows<1>, dim3(nblk), dim3(256), 0, s, T.n, S.knn_kq,
+                               reinterpret_cast<const double4*>(S.p4), T.idx, S.inv,
+                               S.knn_pos_out, S.nb);
+        else
+            hipLaunchKernelGGL(k_sort_rows<2>, dim3(nblk), dim3(256), 0, s, T.n, S.knn_kq,
+                               reinterpret_cast<const double4*>(S.p4), T.idx, S.inv,
+                               S.knn_pos_out, S.nb);
+        GF_LAUNCH_CHECK();
+    }
     return 0;
 }
 
