@@ -16,6 +16,7 @@
 #include "gf_capi.h"
 #include "gf_gravity.h"
 #include "gf_render.h"
+#include "gf_xfer.h"
 
 static char g_err[1024] = "";
 
@@ -209,15 +210,15 @@ int nbody_accel_active(int n, const double* pos, const double* mass, double* acc
         STAGE(g_ctx.d_act, size_t(n)))
         return -1;
     if (n > 0) {
-        GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
-        GF_HIP(hipMemcpyAsync(g_ctx.d_mass, mass, b1, hipMemcpyHostToDevice, s));
-        GF_HIP(hipMemcpyAsync(g_ctx.d_acc, acc, b3, hipMemcpyHostToDevice, s));
-        if (active) GF_HIP(hipMemcpyAsync(g_ctx.d_act, active, size_t(n), hipMemcpyHostToDevice, s));
+        if (gf_copy_h2d(g_ctx.d_pos, pos, b3, s)) return -1;
+        if (gf_copy_h2d(g_ctx.d_mass, mass, b1, s)) return -1;
+        if (gf_copy_h2d(g_ctx.d_acc, acc, b3, s)) return -1;
+        if (active && gf_copy_h2d(g_ctx.d_act, active, size_t(n), s)) return -1;
     }
     if (gravity_impl(n, g_ctx.d_pos, g_ctx.d_mass, g_ctx.d_acc, theta, eps,
                      active ? g_ctx.d_act : nullptr, rebuild, g_ctx.precision, s))
         return -1;
-    if (n > 0) GF_HIP(hipMemcpyAsync(acc, g_ctx.d_acc, b3, hipMemcpyDeviceToHost, s));
+    if (n > 0 && gf_copy_d2h(acc, g_ctx.d_acc, b3, s)) return -1;
     GF_HIP(hipStreamSynchronize(s));
     if (n > 0) {
         if (check_walk_errors()) return -1;
@@ -242,13 +243,13 @@ int gf_accel_direct_subset(int n, const double* pos, const double* mass, int nt,
         STAGE(g_ctx.d_tgt, sizeof(int64_t) * size_t(nt)) ||
         STAGE(g_ctx.d_partial, sizeof(double) * gf_direct_partial_doubles(n, nt)))
         return -1;
-    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(g_ctx.d_mass, mass, b1, hipMemcpyHostToDevice, s));
-    if (tgt) GF_HIP(hipMemcpyAsync(g_ctx.d_tgt, tgt, sizeof(int64_t) * nt, hipMemcpyHostToDevice, s));
+    if (gf_copy_h2d(g_ctx.d_pos, pos, b3, s)) return -1;
+    if (gf_copy_h2d(g_ctx.d_mass, mass, b1, s)) return -1;
+    if (tgt && gf_copy_h2d(g_ctx.d_tgt, tgt, sizeof(int64_t) * nt, s)) return -1;
     if (gf_direct(n, g_ctx.d_pos, g_ctx.d_mass, nt, tgt ? g_ctx.d_tgt : nullptr, g_ctx.d_acc, eps,
                   g_ctx.d_partial, s))
         return -1;
-    GF_HIP(hipMemcpyAsync(acc, g_ctx.d_acc, bt, hipMemcpyDeviceToHost, s));
+    if (gf_copy_d2h(acc, g_ctx.d_acc, bt, s)) return -1;
     GF_HIP(hipStreamSynchronize(s));
     return 0;
 }
@@ -400,18 +401,18 @@ int nbody_render(int n, const double* pos, const uint8_t* group, int ngroups, co
         STAGE(g_ctx.d_img, bimg) || STAGE(g_ctx.d_small, sizeof(double) * 12))
         return -1;
     if (n > 0) {
-        GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, sizeof(double) * 3 * size_t(n), hipMemcpyHostToDevice, s));
-        GF_HIP(hipMemcpyAsync(g_ctx.d_act, group, size_t(n), hipMemcpyHostToDevice, s));
+        if (gf_copy_h2d(g_ctx.d_pos, pos, sizeof(double) * 3 * size_t(n), s)) return -1;
+        if (gf_copy_h2d(g_ctx.d_act, group, size_t(n), s)) return -1;
     }
     double small[12];
     memcpy(small, rot, sizeof(double) * 9);
     memcpy(small + 9, center, sizeof(double) * 3);
-    GF_HIP(hipMemcpyAsync(g_ctx.d_small, small, sizeof(small), hipMemcpyHostToDevice, s));
+    if (gf_copy_h2d(g_ctx.d_small, small, sizeof(small), s)) return -1;
     double* dsm = reinterpret_cast<double*>(g_ctx.d_small);
     if (gf_render_deposit(n, g_ctx.d_pos, g_ctx.d_act, ngroups, dsm, dsm + 9, scale, W, H,
                           reinterpret_cast<float*>(g_ctx.d_img), s))
         return -1;
-    GF_HIP(hipMemcpyAsync(img, g_ctx.d_img, bimg, hipMemcpyDeviceToHost, s));
+    if (gf_copy_d2h(img, g_ctx.d_img, bimg, s)) return -1;
     GF_HIP(hipStreamSynchronize(s));
     return 0;
 }
@@ -432,15 +433,15 @@ int nbody_tonemap(int W, int H, int ngroups, const float* img, const float* glow
     if (STAGE(g_ctx.d_img, bimg) || STAGE(g_ctx.d_glow, bglow) || STAGE(g_ctx.d_out8, bout) ||
         STAGE(g_ctx.d_small, sizeof(float) * 3 * size_t(ngroups) + 96))
         return -1;
-    GF_HIP(hipMemcpyAsync(g_ctx.d_img, img, bimg, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(g_ctx.d_glow, glow, bglow, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(g_ctx.d_small, colors, sizeof(float) * 3 * size_t(ngroups), hipMemcpyHostToDevice, s));
+    if (gf_copy_h2d(g_ctx.d_img, img, bimg, s)) return -1;
+    if (gf_copy_h2d(g_ctx.d_glow, glow, bglow, s)) return -1;
+    if (gf_copy_h2d(g_ctx.d_small, colors, sizeof(float) * 3 * size_t(ngroups), s)) return -1;
     if (gf_render_tonemap(W, H, ngroups, reinterpret_cast<const float*>(g_ctx.d_img),
                           reinterpret_cast<const float*>(g_ctx.d_glow), gw, gh,
                           reinterpret_cast<const float*>(g_ctx.d_small), inv_ref, K, exposure,
                           reinterpret_cast<uint8_t*>(g_ctx.d_out8), s))
         return -1;
-    GF_HIP(hipMemcpyAsync(out, g_ctx.d_out8, bout, hipMemcpyDeviceToHost, s));
+    if (gf_copy_d2h(out, g_ctx.d_out8, bout, s)) return -1;
     GF_HIP(hipStreamSynchronize(s));
     return 0;
 }
@@ -453,13 +454,13 @@ int gf_sph_kernel(int n, const double* r, const double* h, double* w, double* g)
     hipStream_t s = g_ctx.stream;
     const size_t b = sizeof(double) * size_t(n);
     if (STAGE(g_ctx.d_pos, 2 * b) || STAGE(g_ctx.d_out1, 2 * b)) return -1;
-    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, r, b, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(g_ctx.d_pos + n, h, b, hipMemcpyHostToDevice, s));
+    if (gf_copy_h2d(g_ctx.d_pos, r, b, s)) return -1;
+    if (gf_copy_h2d(g_ctx.d_pos + n, h, b, s)) return -1;
     if (gf_sph_kernel_eval(n, g_ctx.d_pos, g_ctx.d_pos + n, w ? g_ctx.d_out1 : nullptr,
                            g ? g_ctx.d_out1 + n : nullptr, s))
         return -1;
-    if (w) GF_HIP(hipMemcpyAsync(w, g_ctx.d_out1, b, hipMemcpyDeviceToHost, s));
-    if (g) GF_HIP(hipMemcpyAsync(g, g_ctx.d_out1 + n, b, hipMemcpyDeviceToHost, s));
+    if (w && gf_copy_d2h(w, g_ctx.d_out1, b, s)) return -1;
+    if (g && gf_copy_d2h(g, g_ctx.d_out1 + n, b, s)) return -1;
     GF_HIP(hipStreamSynchronize(s));
     return 0;
 }
@@ -480,22 +481,22 @@ int gf_sph_forces(int n, int k, const double* pos, const double* vel, const doub
         STAGE(g_ctx.d_out1, 4 * b1) || STAGE(g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k) ||
         STAGE(g_ctx.d_scratch, gf_sph_table_scratch_bytes(n, k)))
         return -1;
-    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(g_ctx.d_vel, vel, b3, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(g_ctx.d_mass, mass, b1, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(g_ctx.d_h, h, b1, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(g_ctx.d_u, u, b1, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(g_ctx.d_nb, nb, sizeof(int64_t) * size_t(n) * k, hipMemcpyHostToDevice, s));
+    if (gf_copy_h2d(g_ctx.d_pos, pos, b3, s)) return -1;
+    if (gf_copy_h2d(g_ctx.d_vel, vel, b3, s)) return -1;
+    if (gf_copy_h2d(g_ctx.d_mass, mass, b1, s)) return -1;
+    if (gf_copy_h2d(g_ctx.d_h, h, b1, s)) return -1;
+    if (gf_copy_h2d(g_ctx.d_u, u, b1, s)) return -1;
+    if (gf_copy_h2d(g_ctx.d_nb, nb, sizeof(int64_t) * size_t(n) * k, s)) return -1;
     double* o = g_ctx.d_out1;
     if (gf_sph_forces_table(n, k, g_ctx.d_pos, g_ctx.d_vel, g_ctx.d_mass, g_ctx.d_nb, g_ctx.d_h,
                             g_ctx.d_u, gamma, alpha, beta, o, g_ctx.d_acc, o + n, o + 2 * size_t(n),
                             o + 3 * size_t(n), g_ctx.d_scratch, s))
         return -1;
-    if (rho) GF_HIP(hipMemcpyAsync(rho, o, b1, hipMemcpyDeviceToHost, s));
-    if (acc) GF_HIP(hipMemcpyAsync(acc, g_ctx.d_acc, b3, hipMemcpyDeviceToHost, s));
-    if (dudt) GF_HIP(hipMemcpyAsync(dudt, o + n, b1, hipMemcpyDeviceToHost, s));
-    if (dt_i) GF_HIP(hipMemcpyAsync(dt_i, o + 2 * size_t(n), b1, hipMemcpyDeviceToHost, s));
-    if (divv) GF_HIP(hipMemcpyAsync(divv, o + 3 * size_t(n), b1, hipMemcpyDeviceToHost, s));
+    if (rho && gf_copy_d2h(rho, o, b1, s)) return -1;
+    if (acc && gf_copy_d2h(acc, g_ctx.d_acc, b3, s)) return -1;
+    if (dudt && gf_copy_d2h(dudt, o + n, b1, s)) return -1;
+    if (dt_i && gf_copy_d2h(dt_i, o + 2 * size_t(n), b1, s)) return -1;
+    if (divv && gf_copy_d2h(divv, o + 3 * size_t(n), b1, s)) return -1;
     GF_HIP(hipStreamSynchronize(s));
     return 0;
 }
@@ -514,16 +515,16 @@ int gf_knn(int n, const double* pos, int k, const double* r_init, int64_t* nb, d
         STAGE(g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k) ||
         STAGE(g_ctx.d_dist, sizeof(double) * size_t(n) * k))
         return -1;
-    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
+    if (gf_copy_h2d(g_ctx.d_pos, pos, b3, s)) return -1;
     GF_HIP(hipMemsetAsync(g_ctx.d_mass, 0, b1, s));
-    if (r_init) GF_HIP(hipMemcpyAsync(g_ctx.d_h, r_init, b1, hipMemcpyHostToDevice, s));
+    if (r_init && gf_copy_h2d(g_ctx.d_h, r_init, b1, s)) return -1;
     GfSph& S = g_ctx.sph;
     if (gf_sph_setup(S, n, g_ctx.d_pos, nullptr, g_ctx.d_mass, nullptr, s)) return -1;
     if (gf_sph_knn(S, k, r_init ? g_ctx.d_h : nullptr, false, true, 64, s)) return -1;
     g_ctx.sph_k = 0;   // the held table holds input indices: not usable for forces
     if (gf_sph_export_knn(S, k, g_ctx.d_nb, g_ctx.d_dist, s)) return -1;
-    GF_HIP(hipMemcpyAsync(nb, g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k, hipMemcpyDeviceToHost, s));
-    GF_HIP(hipMemcpyAsync(dist, g_ctx.d_dist, sizeof(double) * size_t(n) * k, hipMemcpyDeviceToHost, s));
+    if (gf_copy_d2h(nb, g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k, s)) return -1;
+    if (gf_copy_d2h(dist, g_ctx.d_dist, sizeof(double) * size_t(n) * k, s)) return -1;
     GF_HIP(hipStreamSynchronize(s));
     if (mean_passes) {
         int* p = new int[size_t(n)];
@@ -546,19 +547,19 @@ int gf_knn_from_candidates_host(int n, int m, const double* pos, const int64_t* 
         STAGE(g_ctx.d_dist, sizeof(double) * size_t(n) * k) || STAGE(g_ctx.d_flag, sizeof(int)))
         return -1;
     int one = 1;
-    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
+    if (gf_copy_h2d(g_ctx.d_pos, pos, b3, s)) return -1;
     int64_t* dcand = g_ctx.d_nb;
     int64_t* dnb = g_ctx.d_nb + size_t(n) * m;
-    GF_HIP(hipMemcpyAsync(dcand, cand, sizeof(int64_t) * size_t(n) * m, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(dnb, nb, sizeof(int64_t) * size_t(n) * k, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(g_ctx.d_dist, dist, sizeof(double) * size_t(n) * k, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(g_ctx.d_flag, &one, sizeof(int), hipMemcpyHostToDevice, s));
+    if (gf_copy_h2d(dcand, cand, sizeof(int64_t) * size_t(n) * m, s)) return -1;
+    if (gf_copy_h2d(dnb, nb, sizeof(int64_t) * size_t(n) * k, s)) return -1;
+    if (gf_copy_h2d(g_ctx.d_dist, dist, sizeof(double) * size_t(n) * k, s)) return -1;
+    if (gf_copy_h2d(g_ctx.d_flag, &one, sizeof(int), s)) return -1;
     if (gf_knn_from_candidates(n, m, g_ctx.d_pos, dcand, k, dnb, g_ctx.d_dist, g_ctx.d_flag, s))
         return -1;
     int ok = 0;
-    GF_HIP(hipMemcpyAsync(nb, dnb, sizeof(int64_t) * size_t(n) * k, hipMemcpyDeviceToHost, s));
-    GF_HIP(hipMemcpyAsync(dist, g_ctx.d_dist, sizeof(double) * size_t(n) * k, hipMemcpyDeviceToHost, s));
-    GF_HIP(hipMemcpyAsync(&ok, g_ctx.d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (gf_copy_d2h(nb, dnb, sizeof(int64_t) * size_t(n) * k, s)) return -1;
+    if (gf_copy_d2h(dist, g_ctx.d_dist, sizeof(double) * size_t(n) * k, s)) return -1;
+    if (gf_copy_d2h(&ok, g_ctx.d_flag, sizeof(int), s)) return -1;
     GF_HIP(hipStreamSynchronize(s));
     return ok;
 }
@@ -597,8 +598,8 @@ int gf_nbcache_search(int n_total, int n, const int64_t* g, const double* pos, i
         STAGE(g_ctx.d_nb, sizeof(int64_t) * size_t(n) * (m + k)) ||
         STAGE(g_ctx.d_dist, sizeof(double) * size_t(n) * (m + k)))
         return -1;
-    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(g_ctx.d_g, g, sizeof(int64_t) * size_t(n), hipMemcpyHostToDevice, s));
+    if (gf_copy_h2d(g_ctx.d_pos, pos, b3, s)) return -1;
+    if (gf_copy_h2d(g_ctx.d_g, g, sizeof(int64_t) * size_t(n), s)) return -1;
     GF_HIP(hipMemsetAsync(g_ctx.d_mass, 0, b1, s));
     GfSph& S = g_ctx.sph;
     if (gf_sph_setup(S, n, g_ctx.d_pos, nullptr, g_ctx.d_mass, nullptr, s)) return -1;
@@ -611,8 +612,8 @@ int gf_nbcache_search(int n_total, int n, const int64_t* g, const double* pos, i
     int64_t* knb = g_ctx.d_nb + size_t(n) * m;
     double* kdist = g_ctx.d_dist + size_t(n) * m;
     if (gf_first_columns(n, m, k, g_ctx.d_nb, g_ctx.d_dist, knb, kdist, s)) return -1;
-    GF_HIP(hipMemcpyAsync(nb, knb, sizeof(int64_t) * size_t(n) * k, hipMemcpyDeviceToHost, s));
-    GF_HIP(hipMemcpyAsync(dist, kdist, sizeof(double) * size_t(n) * k, hipMemcpyDeviceToHost, s));
+    if (gf_copy_d2h(nb, knb, sizeof(int64_t) * size_t(n) * k, s)) return -1;
+    if (gf_copy_d2h(dist, kdist, sizeof(double) * size_t(n) * k, s)) return -1;
     GF_HIP(hipStreamSynchronize(s));
     return 0;
 }
@@ -630,14 +631,13 @@ int gf_nbcache_reselect(int n_total, int n, const int64_t* g, const double* pos,
         STAGE(g_ctx.d_dist, sizeof(double) * size_t(n) * k) ||
         STAGE(g_ctx.d_scratch, sizeof(int) * size_t(n)))
         return -1;
-    GF_HIP(hipMemcpyAsync(g_ctx.d_pos, pos, b3, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(g_ctx.d_g, g, sizeof(int64_t) * size_t(n), hipMemcpyHostToDevice, s));
+    if (gf_copy_h2d(g_ctx.d_pos, pos, b3, s)) return -1;
+    if (gf_copy_h2d(g_ctx.d_g, g, sizeof(int64_t) * size_t(n), s)) return -1;
     const int ok = gf_verlet_reselect(g_ctx.verlet, n_total, n, k, g_ctx.d_g, g_ctx.d_pos,
                                       g_ctx.d_scratch, g_ctx.d_nb, g_ctx.d_dist, s);
     if (ok <= 0) return ok;
-    GF_HIP(hipMemcpyAsync(nb, g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k, hipMemcpyDeviceToHost, s));
-    GF_HIP(hipMemcpyAsync(dist, g_ctx.d_dist, sizeof(double) * size_t(n) * k, hipMemcpyDeviceToHost,
-                          s));
+    if (gf_copy_d2h(nb, g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k, s)) return -1;
+    if (gf_copy_d2h(dist, g_ctx.d_dist, sizeof(double) * size_t(n) * k, s)) return -1;
     GF_HIP(hipStreamSynchronize(s));
     return 1;
 }
@@ -750,8 +750,8 @@ int gf_hydro_neighbours(int64_t* nb, double* dk) {
     GF_HIP(hipDeviceSynchronize());
     if (gf_sph_table_out(g_ctx.sph, k, g_ctx.d_nb, s) || gf_sph_dk_out(g_ctx.sph, g_ctx.d_out1, s))
         return -1;
-    GF_HIP(hipMemcpyAsync(nb, g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k, hipMemcpyDeviceToHost, s));
-    if (dk) GF_HIP(hipMemcpyAsync(dk, g_ctx.d_out1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (gf_copy_d2h(nb, g_ctx.d_nb, sizeof(int64_t) * size_t(n) * k, s)) return -1;
+    if (dk && gf_copy_d2h(dk, g_ctx.d_out1, sizeof(double) * n, s)) return -1;
     GF_HIP(hipStreamSynchronize(s));
     return k;
 }
@@ -877,12 +877,12 @@ int gf_mlp_grad_host(int n, const uint16_t* x, const float* target, const float*
     float* dl = dg + GF_MLP_PARAMS;
     float* dt = dl + 4;
     uint16_t* dx = reinterpret_cast<uint16_t*>(dt + n);
-    GF_HIP(hipMemcpyAsync(dp, params, bp, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(dt, target, bt, hipMemcpyHostToDevice, s));
-    GF_HIP(hipMemcpyAsync(dx, x, bx, hipMemcpyHostToDevice, s));
+    if (gf_copy_h2d(dp, params, bp, s)) return -1;
+    if (gf_copy_h2d(dt, target, bt, s)) return -1;
+    if (gf_copy_h2d(dx, x, bx, s)) return -1;
     if (gf_mlp_grad(n, n, dx, dt, dp, tau, dg, dl, g_ctx.d_mlp_part, s)) return -1;
-    GF_HIP(hipMemcpyAsync(grad, dg, bp, hipMemcpyDeviceToHost, s));
-    if (loss) GF_HIP(hipMemcpyAsync(loss, dl, sizeof(float), hipMemcpyDeviceToHost, s));
+    if (gf_copy_d2h(grad, dg, bp, s)) return -1;
+    if (loss && gf_copy_d2h(loss, dl, sizeof(float), s)) return -1;
     GF_HIP(hipStreamSynchronize(s));
     return 0;
 }
