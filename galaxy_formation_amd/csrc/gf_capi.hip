@@ -65,10 +65,14 @@ int gf_stage_reserve(void** p, size_t* cap, size_t bytes) {
 #define STAGE(ptr, bytes) gf_stage_reserve(reinterpret_cast<void**>(&(ptr)), &ptr##_cap, (bytes))
 
 static int check_walk_errors() {
-    int oc[2] = {0, 0};
+    int oc[3] = {0, 0, 0};
     GF_HIP(hipMemcpy(oc, g_ctx.walk.ovf_count, sizeof(oc), hipMemcpyDeviceToHost));
     if (oc[1] != 0) {
         gf_set_error("tree walk: %d groups overflowed the global stack", oc[1]);
+        return -1;
+    }
+    if (oc[2] != 0) {
+        gf_set_error("tree walk: %d nodes whose children do not hold consecutive records", oc[2]);
         return -1;
     }
     return 0;

@@ -13,7 +13,12 @@ struct GfWalkScratch {
     int* ovf_count = nullptr;
     int* gstack = nullptr;                 // global stacks for those groups
     int cap_groups = 0;
-    void* gnode = nullptr;                 // packed nodes (128 bytes each)
+    void* gnode = nullptr;                 // walk records (64 bytes each), level by level
+    void* gquad = nullptr;                 // their quadrupoles (32 bytes each)
+    void* rkey = nullptr;                  // the records' order: sort keys and node ids,
+    void* rval = nullptr;                  // two buffers each
+    void* rec_of = nullptr;                // record of each node
+    void* rsort_tmp = nullptr;
     void* p4 = nullptr;                    // packed particles (x, y, z, m), tree order
     int cap_gnode = 0, cap_p4 = 0;
     // precision 2: an accepted node whose quadrupole term is estimated below
@@ -25,7 +30,7 @@ struct GfWalkScratch {
     // hold on entry (the caller's previous accelerations) is evaluated as a
     // monopole; groups without that history use quad_tol
     float quad_rel_tol = 0.0f;
-    // persistent one-wave blocks of the walk (0 = the default, 8192)
+    // persistent one-wave blocks of the walk (0 = the default, gf_walk_default_blocks)
     int walk_blocks = 0;
 };
 

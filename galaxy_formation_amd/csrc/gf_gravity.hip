@@ -28,6 +28,7 @@
 #include <type_traits>
 
 #include "gf_gravity.h"
+#include "gf_prim.h"
 
 namespace {
 
@@ -62,35 +63,40 @@ constexpr int GSCAP = 512 * (GF_KEY_LEVELS + 2);    // global stack entries per 
 #endif
 constexpr int GST_BLOCKS = 256;
 
-// A tree node packed for the walk (128 bytes).  A visit reads the first 96
-// bytes, everything the opening test, the quadrupole rule and the descent
-// need, in one round of loads for a batch of 64 nodes; only a node accepted
-// with its quadrupole reads the tensor in the last 32.
+// A tree node packed for the walk (64 bytes).  The records are numbered level
+// by level (root first, each level in tree order), so the children of a node
+// hold consecutive records and a visit reads one 64-byte record, everything
+// the opening test, the quadrupole rule and the descent need, in one round of
+// loads for a batch of 64 nodes.
 struct GNode {
     double4 com;          // centre of mass, mass (FP64)
     double open_r;        // opening radius (FP64)
     int start, count;     // particle range in tree order
-    int leaf, nchild;
+    int kid;              // record of the first non-empty child (internal nodes)
+    int nkid;             // non-empty children; -1 for a leaf
     float qlen;           // sqrt(3 max |Q_ij| / m): the quadrupole term's share of
                           // the monopole at distance d is at most about (qlen / d)^2
     float qabs4;          // (2.858 |Q|_F)^(1/4): the quadrupole term's acceleration at
                           // distance d is at most (qabs4 / d)^4 (3.5 |Q|_2 / d^4 with
                           // |Q|_2 <= sqrt(2/3) |Q|_F for a traceless symmetric Q)
-    int4 c_lo, c_hi;      // the nchild non-empty children in octant order, then -1
-    float q[6];           // traceless quadrupole (xx, yy, zz, xy, xz, yz), FP32
-    float pad[2];
 };
-static_assert(sizeof(GNode) == 128, "GNode layout");
+static_assert(sizeof(GNode) == 64, "GNode layout");
 static_assert(offsetof(GNode, open_r) == 32 && offsetof(GNode, start) == 40 &&
-                  offsetof(GNode, leaf) == 48 && offsetof(GNode, nchild) == 52 &&
-                  offsetof(GNode, qlen) == 56 && offsetof(GNode, qabs4) == 60 &&
-                  offsetof(GNode, c_lo) == 64 && offsetof(GNode, c_hi) == 80 &&
-                  offsetof(GNode, q) == 96,
+                  offsetof(GNode, kid) == 48 && offsetof(GNode, nkid) == 52 &&
+                  offsetof(GNode, qlen) == 56 && offsetof(GNode, qabs4) == 60,
               "the walk reads GNode as 16-byte words");
+// The traceless quadrupole of a record, FP32, read only for a node accepted
+// with its quadrupole: a = (xx, yy, xy, xz), yz.
+struct GQuad {
+    float4 a;
+    float yz, pad[3];
+};
+static_assert(sizeof(GQuad) == 32, "GQuad layout");
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 struct WalkArgs {
-    const GNode* gn;
+    const GNode* gn;           // walk records, level by level (record 0 is the root)
+    const GQuad* gq;           // their quadrupoles (QUAD)
     const double4* p4;         // (x, y, z, m) per tree position
     const int* start;
     const int* count;
@@ -111,7 +117,8 @@ struct WalkArgs {
     int* work;                 // [0] LDS-stack launch, [1] global-stack launch
     unsigned long long* inter;
     int* ovf;
-    int* ovf_count;            // [0] queued groups, [1] global-stack overflows (error)
+    int* ovf_count;            // [0] queued groups, [1] global-stack overflows (error),
+                               // [2] record layout errors (error)
     int* gstack;
     int lds_scap;              // LDS stack entries used by the first launch (<= SCAP)
     unsigned long long* inter_nodes;   // the node (monopole or quadrupole) share of inter
@@ -398,13 +405,13 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 __syncthreads();
                 // every lane loads a record (the root where it has no node),
                 // so the visit is free of branches; the empty asm keeps the
-                // six loads together, where the compiler would otherwise
-                // move the leaf flag and children into the paths that use
-                // them, one dependent round trip each
+                // four loads together, where the compiler would otherwise
+                // move the children into the paths that use them, one
+                // dependent round trip each
                 const bool valid = node >= 0;
                 const v4i* r = reinterpret_cast<const v4i*>(a.gn + (valid ? node : 0));
-                v4i w0 = r[0], w1 = r[1], w2 = r[2], w3 = r[3], w4 = r[4], w5 = r[5];
-                asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5));
+                v4i w0 = r[0], w1 = r[1], w2 = r[2], w3 = r[3];
+                asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3));
                 const double4 cm = make_double4(__hiloint2double(w0.y, w0.x), __hiloint2double(w0.w, w0.z),
                                                 __hiloint2double(w1.y, w1.x), __hiloint2double(w1.w, w1.z));
                 const double orr = __hiloint2double(w2.y, w2.x);
@@ -415,8 +422,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 const bool mono = valid && far;                 // accepted: one list entry
                 const bool opened = valid && !far;
                 const int first = w2.z;
-                const int pc = opened && w3.x ? w2.w : 0;      // an opened leaf's particles
-                const int push = opened && !w3.x ? w3.y : 0;   // an opened node's children
+                const int pc = opened && w3.y < 0 ? w2.w : 0;  // an opened leaf's particles
+                const int push = opened && w3.y > 0 ? w3.y : 0;   // an opened node's children
                 bool small_q = false;
                 if constexpr (QUAD) {
                     // the quadrupole term's share at the group's nearest
@@ -437,11 +444,11 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                     break;
                 }
                 {
+                    // the children's records are consecutive, in tree order
                     const int o = sn + pinc - push;
-                    const int cc[8] = {w4.x, w4.y, w4.z, w4.w, w5.x, w5.y, w5.z, w5.w};
 #pragma unroll
                     for (int k = 0; k < 8; ++k)
-                        if (k < push) stack[o + k] = cc[k];
+                        if (k < push) stack[o + k] = w3.x + k;
                 }
                 sn += ptot;
                 // append this batch's entries, evaluating the list whenever it
@@ -450,8 +457,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 const uint64_t mmask = __ballot(mono);
                 const int nmono = __popcll(mmask);
                 int qslot = -1;                                // QUAD: this lane's tensor slot
-                v4i rq03 = {0, 0, 0, 0};                       // q[0..3], then q[4], q[5]
-                int2 rq45 = make_int2(0, 0);
+                float4 rqa = make_float4(0, 0, 0, 0);          // its (xx, yy, xy, xz), yz
+                float rqb = 0.0f;
                 if (nmono > 0) {
                     if constexpr (QUAD) {
                         // nodes whose quadrupole term is negligible join the
@@ -474,8 +481,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                                 // the tensor is loaded now and lands in its
                                 // slot after this batch's particles: no flush
                                 // of the quadrupole list comes in between
-                                rq03 = r[6];
-                                rq45 = *reinterpret_cast<const int2*>(r + 7);
+                                rqa = a.gq[node].a;
+                                rqb = a.gq[node].yz;
                                 qslot = nq + __popcll(qmask & gf_lanemask_lt());
                                 s_qe[qslot] = make_float4(float(cm.x - bx), float(cm.y - by),
                                                           float(cm.z - bz), float(cm.w));
@@ -552,10 +559,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 }
                 if constexpr (QUAD) {
                     if (qslot >= 0) {
-                        // (xx, yy, xy, xz) and yz of the record's (xx, yy, zz, xy), (xz, yz)
-                        s_qa[qslot] = make_float4(__int_as_float(rq03.x), __int_as_float(rq03.y),
-                                                  __int_as_float(rq03.w), __int_as_float(rq45.x));
-                        s_qb[qslot] = __int_as_float(rq45.y);
+                        s_qa[qslot] = rqa;
+                        s_qb[qslot] = rqb;
                     }
                 }
                 __syncthreads();
@@ -619,7 +624,35 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
     }
 }
 
+// Keys of the records' order: the node depth (255 past the node count), with
+// the node ids as values; one stable pass on the low byte sorts them level by
+// level, each level in tree order.
+__global__ __launch_bounds__(256) void k_rec_keys(const int* __restrict__ counters,
+                                                 const int8_t* __restrict__ depth, int cap,
+                                                 uint64_t* __restrict__ key,
+                                                 uint32_t* __restrict__ val) {
+    const int nn = counters[0];
+    for (int id = blockIdx.x * 256 + threadIdx.x; id < cap; id += gridDim.x * 256) {
+        key[id] = id < nn ? uint64_t(uint8_t(depth[id])) : 255u;
+        val[id] = uint32_t(id);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rec_of(const int* __restrict__ counters,
+                                               const uint32_t* __restrict__ order,
+                                               int* __restrict__ rec_of) {
+    const int nn = counters[0];
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < nn; r += gridDim.x * 256)
+        rec_of[order[r]] = r;
+}
+
+// Record r holds node order[r].  In tree order (depth first, children in
+// octant order) the non-empty children of a node are consecutive among the
+// nodes of their depth, so they hold consecutive records; a node for which
+// that fails is counted in *bad (an error the caller reports).
 __global__ __launch_bounds__(256) void k_gnode_pack(const int* __restrict__ counters,
+                                                   const uint32_t* __restrict__ order,
+                                                   const int* __restrict__ rec_of,
                                                    const double4* __restrict__ com,
                                                    const double* __restrict__ open_r,
                                                    const int* __restrict__ start,
@@ -627,41 +660,49 @@ __global__ __launch_bounds__(256) void k_gnode_pack(const int* __restrict__ coun
                                                    const int8_t* __restrict__ leaf,
                                                    const int* __restrict__ child8,
                                                    const double* __restrict__ quad,
-                                                   GNode* __restrict__ gn) {
+                                                   GNode* __restrict__ gn,
+                                                   GQuad* __restrict__ gq, int* bad) {
     const int nn = counters[0];
-    for (int id = blockIdx.x * 256 + threadIdx.x; id < nn; id += gridDim.x * 256) {
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < nn; r += gridDim.x * 256) {
+        const int id = int(order[r]);
         GNode g;
         g.com = com[id];
         g.open_r = open_r[id];
         g.start = start[id];
         g.count = count[id];
-        g.leaf = leaf[id];
-        g.pad[0] = g.pad[1] = 0.0f;
-        // the non-empty children first, in octant order
-        int cc[8], nc = 0;
-        for (int k = 0; k < 8; ++k) cc[k] = -1;
-        for (int k = 0; k < 8; ++k) {
-            const int c = child8[8 * size_t(id) + k];
-            if (c >= 0) cc[nc++] = c;
+        g.kid = 0;
+        g.nkid = -1;
+        if (!leaf[id]) {
+            int nc = 0, k0 = -1;
+            bool ok = true;
+            for (int k = 0; k < 8; ++k) {
+                const int c = child8[8 * size_t(id) + k];
+                if (c < 0) continue;
+                const int rc = rec_of[c];
+                if (nc == 0) k0 = rc;
+                ok &= rc == k0 + nc;
+                ++nc;
+            }
+            if (!ok) atomicAdd(bad, 1);
+            g.kid = nc > 0 ? k0 : 0;
+            g.nkid = nc;
         }
-        g.nchild = nc;
-        g.c_lo = make_int4(cc[0], cc[1], cc[2], cc[3]);
-        g.c_hi = make_int4(cc[4], cc[5], cc[6], cc[7]);
-        double qmax = 0.0;
-        for (int k = 0; k < 6; ++k) {
-            const double qk = quad ? quad[6 * size_t(id) + k] : 0.0;
-            g.q[k] = float(qk);
-            qmax = fmax(qmax, fabs(qk));
-        }
-        g.qlen = g.com.w > 0 ? float(sqrt(3.0 * qmax / g.com.w)) : 0.0f;
-        double qf2 = 0.0;
+        double qmax = 0.0, qf2 = 0.0;
+        GQuad q4;
+        q4.a = make_float4(0, 0, 0, 0);
+        q4.yz = q4.pad[0] = q4.pad[1] = q4.pad[2] = 0.0f;
         if (quad) {
-            const double* Q = quad + 6 * size_t(id);
+            const double* Q = quad + 6 * size_t(id);   // (xx, yy, zz, xy, xz, yz)
+            for (int k = 0; k < 6; ++k) qmax = fmax(qmax, fabs(Q[k]));
             qf2 = Q[0] * Q[0] + Q[1] * Q[1] + Q[2] * Q[2] +
                   2.0 * (Q[3] * Q[3] + Q[4] * Q[4] + Q[5] * Q[5]);
+            q4.a = make_float4(float(Q[0]), float(Q[1]), float(Q[3]), float(Q[4]));
+            q4.yz = float(Q[5]);
+            gq[r] = q4;
         }
+        g.qlen = g.com.w > 0 ? float(sqrt(3.0 * qmax / g.com.w)) : 0.0f;
         g.qabs4 = float(sqrt(sqrt(2.858 * sqrt(qf2))));
-        gn[id] = g;
+        gn[r] = g;
     }
 }
 
@@ -680,6 +721,7 @@ __global__ void k_walk_reset(int* work, int* ovf_count) {
         work[1] = 0;
         ovf_count[0] = 0;
         ovf_count[1] = 0;
+        ovf_count[2] = 0;
     }
 }
 
@@ -818,8 +860,18 @@ int gf_walk_reserve(GfWalkScratch& W, int max_groups) {
 // Packed copies of the tree for the walk (sized for cap_nodes nodes and cap_n particles).
 static int walk_pack_reserve(GfWalkScratch& W, int cap_nodes, int cap_n) {
     if (cap_nodes > W.cap_gnode) {
-        if (W.gnode) GF_HIP(hipFree(W.gnode));
-        GF_HIP(hipMalloc(&W.gnode, sizeof(GNode) * size_t(std::max(1, cap_nodes))));
+        void** bufs[] = {&W.gnode, &W.gquad, &W.rkey, &W.rval, &W.rec_of, &W.rsort_tmp};
+        for (void** b : bufs) {
+            if (*b) GF_HIP(hipFree(*b));
+            *b = nullptr;
+        }
+        const size_t c = size_t(std::max(1, cap_nodes));
+        GF_HIP(hipMalloc(&W.gnode, sizeof(GNode) * c));
+        GF_HIP(hipMalloc(&W.gquad, sizeof(GQuad) * c));
+        GF_HIP(hipMalloc(&W.rkey, 2 * sizeof(uint64_t) * c));
+        GF_HIP(hipMalloc(&W.rval, 2 * sizeof(uint32_t) * c));
+        GF_HIP(hipMalloc(&W.rec_of, sizeof(int) * c));
+        GF_HIP(hipMalloc(&W.rsort_tmp, sizeof(int) * gf_sort_temp_ints(int(c))));
         W.cap_gnode = cap_nodes;
     }
     if (cap_n > W.cap_p4) {
@@ -836,8 +888,9 @@ void gf_walk_free(GfWalkScratch& W) {
     if (W.ovf_count) (void)hipFree(W.ovf_count);
     if (W.gstack) (void)hipFree(W.gstack);
     if (W.ovf) (void)hipFree(W.ovf);
-    if (W.gnode) (void)hipFree(W.gnode);
-    if (W.p4) (void)hipFree(W.p4);
+    void* bufs[] = {W.gnode, W.gquad, W.rkey, W.rval, W.rec_of, W.rsort_tmp, W.p4};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
     W = GfWalkScratch();
 }
 
@@ -853,14 +906,29 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
     if (gf_walk_reserve(W, T.cap_nodes) || walk_pack_reserve(W, T.cap_nodes, T.n)) return -1;
     T.want_quad = precision == GF_FP32_QUAD;
     if (precision == GF_FP32_QUAD && gf_tree_quadrupoles(T, s)) return -1;
-    hipLaunchKernelGGL(k_gnode_pack, dim3(std::min(2048, gf_div_up(T.cap_nodes, 256))), dim3(256), 0,
-                       s, T.counters, T.com, T.open_r, T.start, T.count, T.leaf, T.child8,
-                       precision == GF_FP32_QUAD ? T.quad : nullptr,
-                       reinterpret_cast<GNode*>(W.gnode));
+    hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, W.work, W.ovf_count);
+    // the records' order: nodes by depth, each depth in tree order
+    const int cn = T.cap_nodes, gb = std::min(2048, gf_div_up(cn, 256));
+    uint64_t* rkey = static_cast<uint64_t*>(W.rkey);
+    uint32_t* rval = static_cast<uint32_t*>(W.rval);
+    hipLaunchKernelGGL(k_rec_keys, dim3(gb), dim3(256), 0, s, T.counters, T.depth, cn, rkey, rval);
+    int alt = 0;
+    if (gf_radix_sort_pairs(rkey, rval, rkey + cn, rval + cn, cn, 0, 8,
+                            static_cast<int*>(W.rsort_tmp), s, &alt))
+        return -1;
+    const uint32_t* order = alt ? rval + cn : rval;
+    hipLaunchKernelGGL(k_rec_of, dim3(gb), dim3(256), 0, s, T.counters, order,
+                       static_cast<int*>(W.rec_of));
+    hipLaunchKernelGGL(k_gnode_pack, dim3(gb), dim3(256), 0, s, T.counters, order,
+                       static_cast<const int*>(W.rec_of), T.com, T.open_r, T.start, T.count,
+                       T.leaf, T.child8, precision == GF_FP32_QUAD ? T.quad : nullptr,
+                       static_cast<GNode*>(W.gnode), static_cast<GQuad*>(W.gquad),
+                       W.ovf_count + 2);
     hipLaunchKernelGGL(k_gp4_pack, dim3(gf_div_up(T.n, 256)), dim3(256), 0, s, T.n, T.tx, T.ty,
                        T.tz, T.tm, reinterpret_cast<double4*>(W.p4));
     WalkArgs a;
-    a.gn = reinterpret_cast<const GNode*>(W.gnode);
+    a.gn = static_cast<const GNode*>(W.gnode);
+    a.gq = static_cast<const GQuad*>(W.gquad);
     a.p4 = reinterpret_cast<const double4*>(W.p4);
     a.start = T.start; a.count = T.count; a.leaf = T.leaf; a.child8 = T.child8;
     a.com = T.com; a.open_r = T.open_r; a.groups = T.groups; a.counters = T.counters;
@@ -872,7 +940,6 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
     a.lds_scap = gf_test_limits().walk_stack > 0 ? std::min(SCAP, gf_test_limits().walk_stack) : SCAP;
     a.quad_rtol = W.quad_tol > 0 ? float(1.0 / std::sqrt(double(W.quad_tol))) : INFINITY;
     a.quad_atol = W.quad_rel_tol;
-    hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, W.work, W.ovf_count);
     // the groups are drawn from a queue, so any grid walks them all
     const int WALK_BLOCKS = W.walk_blocks > 0 ? W.walk_blocks : WALK_GRID;
     if (precision == GF_FP64) {
