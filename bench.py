@@ -44,7 +44,7 @@ PEAK_FP64 = 78.6e12
 # 0 monopole, 2 quadrupole): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from
 # rocprofv3 --pmc (profiles/); counters cannot be read inside a run, so the
 # measured figures are carried here
-WALK_TRAFFIC_BYTES = {0: 3485224288, 2: 3654067808}
+WALK_TRAFFIC_BYTES = {0: 1659382464, 2: 2309621760}
 # the same for one evaluation's FP32-pair SPH sums on the 1M disk's gas
 # (k_pack32 + k_density32 + k_force32)
 SPH_TRAFFIC_BYTES = 641058784
