@@ -11,10 +11,11 @@
 //     centres of mass, so both evaluation precisions use one interaction set.
 //     Accepted nodes append their monopole to the list; opened leaves append
 //     their particles; opened internal nodes push their children.
-//   * The list lives in LDS and is evaluated whenever it fills.  Small groups
+//   * The list lives in LDS and is evaluated whenever it fills, for the
+//     group's active members only, compacted onto the lanes.  Small groups
 //     split the list over several lanes per particle ("slices"), so a group
-//     of P particles keeps 64 / P lanes busy per particle.  A group whose P
-//     leaves many lanes idle is evaluated as two parts (32, 16 or 8
+//     of P active particles keeps 64 / P lanes busy per particle.  A group
+//     whose P leaves many lanes idle is evaluated as two parts (32, 16 or 8
 //     particles, then the rest), each with its own slices.
 //   * FP32 evaluation works on coordinates relative to the group centre b,
 //     rounded to FP32 once per list entry and group; each flush's FP32 partial
@@ -255,6 +256,18 @@ __device__ __forceinline__ int chunk_first_part(int P) {
     return best;
 }
 
+// Position of set bit number s (counted from 0) of m, for s < popcount(m):
+// the largest p with fewer than s + 1 set bits below it.
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int s) {
+    int pos = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        const uint64_t below = m & ((uint64_t(1) << (pos + step)) - 1);   // pos + step <= 63
+        if (__popcll(below) <= s) pos += step;
+    }
+    return pos;
+}
+
 // QUAD (FP32 only): accepted nodes carry their quadrupole and go to a list of
 // their own; particles keep the monopole list.
 template <bool F64, bool GST, bool QUAD>
@@ -358,18 +371,29 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
         unsigned long long g_len = 0, g_nodes = 0;
         bool g_ovf = false;
         for (int c0 = 0; c0 < np; c0 += 64) {
-            const int P = min(64, np - c0);
-            // parts A (particles [0, Pa)) and B ([Pa, P), possibly empty):
-            // lane l evaluates slice l / Pa of particle l % Pa, and slice
-            // l / Pb of particle Pa + l % Pb
+            const int Pc = min(64, np - c0);
+            bool chunk_act = false;
+            if (lane < Pc) chunk_act = a.active ? a.active[a.idx[b + c0 + lane]] != 0 : true;
+            const uint64_t amask = __ballot(chunk_act);
+            const int n_act = __popcll(amask);
+            if (n_act == 0) continue;
+            // only the active members are evaluated: slot s holds the chunk's
+            // s-th active particle (slot s is particle s when all are active);
+            // lane s keeps slot s's position in the chunk
+            const int P = n_act;
+            const int slot_pos = nth_set_bit(amask, min(lane, P - 1));
+            // parts A (slots [0, Pa)) and B ([Pa, P), possibly empty): lane l
+            // evaluates slice l / Pa of slot l % Pa, and slice l / Pb of slot
+            // Pa + l % Pb
             const int Pa = chunk_first_part(P), Pb = P - Pa;
             const int Sa = 64 / Pa, Sb = Pb > 0 ? 64 / Pb : 0;
             const int sla = lane / Pa, slb = Pb > 0 ? lane / Pb : 64;
             const bool evla = sla < Sa, evlb = slb < Sb;
             Acc<F64> A, B;
             {
-                const double4 q = a.p4[b + c0 + lane % Pa];
-                const double4 r = a.p4[b + c0 + (Pb > 0 ? Pa + lane % Pb : lane % Pa)];
+                const double4 q = a.p4[b + c0 + __shfl(slot_pos, lane % Pa, 64)];
+                const double4 r =
+                    a.p4[b + c0 + __shfl(slot_pos, Pb > 0 ? Pa + lane % Pb : lane % Pa, 64)];
                 if constexpr (F64) {
                     A.px = q.x; A.py = q.y; A.pz = q.z;
                     B.px = r.x; B.py = r.y; B.pz = r.z;
@@ -388,10 +412,6 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                     if (evlb) B.flush_quad(s_qe, s_qa, s_qb, len, slb, Sb, eps2r);
                 }
             };
-            bool chunk_act = false;
-            if (lane < P) chunk_act = a.active ? a.active[a.idx[b + c0 + lane]] != 0 : true;
-            const int n_act = __popcll(__ballot(chunk_act));
-            if (n_act == 0) continue;
 
             int sn = 1, ln = 0, nq = 0;
             long long total_len = 0, total_nodes = 0;
@@ -590,7 +610,6 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 }
             }
             // reduce the slices of each particle in FP64, part A then part B
-            const bool act_b = __shfl(chunk_act, Pa + lane, 64);
             for (int part = 0; part < (Pb > 0 ? 2 : 1); ++part) {
                 const int Pp = part ? Pb : Pa, Sp = part ? Sb : Sa, p0 = part ? Pa : 0;
                 double3 v = make_double3(0, 0, 0);
@@ -599,15 +618,16 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 s_red[0][lane] = v.x;
                 s_red[1][lane] = v.y;
                 s_red[2][lane] = v.z;
+                const int ppos = __shfl(slot_pos, min(p0 + lane, P - 1), 64);
                 __syncthreads();
-                if (lane < Pp && (part ? act_b : chunk_act)) {
+                if (lane < Pp) {
                     double sx = 0, sy = 0, sz = 0;
                     for (int q = 0; q < Sp; ++q) {
                         sx += s_red[0][q * Pp + lane];
                         sy += s_red[1][q * Pp + lane];
                         sz += s_red[2][q * Pp + lane];
                     }
-                    const size_t o = 3 * size_t(a.idx[b + c0 + p0 + lane]);
+                    const size_t o = 3 * size_t(a.idx[b + c0 + ppos]);
                     a.acc[o] = sx;
                     a.acc[o + 1] = sy;
                     a.acc[o + 2] = sz;
