@@ -52,6 +52,10 @@ SIGNATURES = {
     "gf_walk_dev": (_ci, [_ci, _vp, _vp, _cd, _ci, _vp]),
     "gf_direct_dev": (_ci, [_ci, _vp, _vp, _ci, _vp, _vp, _cd, _vp]),
     "gf_potential_dev": (_ci, [_ci, _vp, _vp, _vp, _cd, _vp]),
+    "gf_combine_dev": (_ci, [_ci, _ci, _cd, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gf_step_drift_dev": (_ci, [_ci, _cd, _cll, _ci, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gf_step_kick_dev": (_ci, [_ci, _ci, _cll, _ci, _cd, _ci, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                               _cd, _vp]),
     "gf_interactions_dev": (_ci, [_vp]),
     "gf_node_interactions_dev": (_ci, [_vp]),
     "gf_sync": (_ci, [_vp]),

@@ -16,6 +16,7 @@
 #include "gf_capi.h"
 #include "gf_gravity.h"
 #include "gf_render.h"
+#include "gf_step.h"
 #include "gf_xfer.h"
 
 static char g_err[1024] = "";
@@ -358,6 +359,45 @@ int gf_potential_dev(int n, const double* pos, const double* mass, double* phi, 
     }
     if (STAGE(g_ctx.d_partial, sizeof(double) * gf_potential_partial_doubles(n))) return -1;
     return gf_potential(n, pos, mass, phi, eps, g_ctx.d_partial, gf_stream(stream));
+}
+
+int gf_combine_dev(int n, int ng, double eps, const double* acc_grav, const double* acc_sph,
+                   const double* dt_c, double* acc, double* dt_ind, double* dt_grav,
+                   void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (n < 0 || ng < 0 || ng > n || !dt_grav ||
+        (n > 0 && (!acc_grav || !acc || !dt_ind)) || (ng > 0 && (!acc_sph || !dt_c))) {
+        gf_set_error("gf_combine_dev: bad arguments");
+        return -1;
+    }
+    return gf_combine(n, ng, eps, acc_grav, acc_sph, dt_c, acc, dt_ind, dt_grav,
+                      gf_stream(stream));
+}
+
+int gf_step_drift_dev(int n, double dt, long long k, int rungs, double* pos, const double* vel,
+                      const int64_t* rung, uint8_t* active, int* count, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (n < 0 || k < 0 || rungs < 1 || rungs > 62 ||
+        (n > 0 && (!pos || !vel || !rung || !active || !count))) {
+        gf_set_error("gf_step_drift_dev: bad arguments");
+        return -1;
+    }
+    return gf_step_drift(n, dt, k, rungs, pos, vel, rung, active, count, gf_stream(stream));
+}
+
+int gf_step_kick_dev(int n, int ng, long long k, int rungs, double dt_max, int mode,
+                     const uint8_t* active, const double* acc, const double* dudt,
+                     const double* dt_ind, int64_t* rung, double* vel, double* u, double u_floor,
+                     void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (n < 0 || ng < 0 || ng > n || k < 0 || rungs < 1 || rungs > 62 || mode < 0 || mode > 2 ||
+        (n > 0 && (!active || !acc || !rung || !vel || (mode > 0 && !dt_ind))) ||
+        (ng > 0 && (!dudt || !u))) {
+        gf_set_error("gf_step_kick_dev: bad arguments");
+        return -1;
+    }
+    return gf_step_kick(n, ng, k, rungs, dt_max, mode, active, acc, dudt, dt_ind, rung, vel, u,
+                        u_floor, gf_stream(stream));
 }
 
 int gf_interactions_dev(long long* out) {

@@ -53,6 +53,7 @@ class ForceEvaluator:
         self.dudt, self.dt_c, self.divv = z(g), z(g), z(g)
         self.dk, self.cell_rho, self.cell_count, self.r0 = z(g), z(g), z(g), z(g)
         self.dt_ind = z(n)
+        self.dt_grav = torch.zeros((), dtype=torch.float64, device=self.dev)
         self.passes = 0
 
     @property
@@ -141,15 +142,12 @@ class ForceEvaluator:
         acc_grav + acc_sph (gas rows), dt_ind = min(0.3 sqrt(eps / max(|a|,
         1e-30)), dt_courant) with dt_courant only on the gas, and dt_grav =
         0.3 sqrt(eps / max(max |a|, 1e-30)) (a device scalar)."""
-        torch = self.torch
-        self.acc.copy_(self.acc_grav)
-        if self.ng:
-            self.acc[:self.ng] += self.acc_sph
-        amag = torch.sqrt(torch.sum(self.acc * self.acc, dim=1))
-        self.dt_ind = 0.3 * torch.sqrt(self.eps / torch.clamp(amag, min=1e-30))
-        self.dt_grav = 0.3 * torch.sqrt(self.eps / torch.clamp(amag.max(), min=1e-30))
-        if self.ng:
-            self.dt_ind[:self.ng] = torch.minimum(self.dt_ind[:self.ng], self.dt_c)
+        ng = self.ng
+        _lib.check(self.L.gf_combine_dev(self.n, ng, self.eps, self.acc_grav.data_ptr(),
+                                         self.acc_sph.data_ptr() if ng else None,
+                                         self.dt_c.data_ptr() if ng else None,
+                                         self.acc.data_ptr(), self.dt_ind.data_ptr(),
+                                         self.dt_grav.data_ptr(), self.sp))
 
 
 class Stepper:
@@ -270,7 +268,10 @@ class RungStepper:
         self.L = self.F.L
         self.phi = torch.zeros(p.n, dtype=torch.float64, device=dev)
         self.rung = torch.zeros(p.n, dtype=torch.int64, device=dev)
-        self.active_counts = []   # device scalars, read at the end
+        self.active = torch.zeros(p.n, dtype=torch.uint8, device=dev)   # this sub-step's
+        self._counts = torch.zeros(1024, dtype=torch.int32, device=dev)  # active per sub-step
+        self.active_counts = []   # device scalars (views of _counts), read at the end
+        self.k = 0
         self.passes = []
         self.L_sync = []
         self.t_hydro = 0.0        # time of the last hydro evaluation
@@ -279,9 +280,9 @@ class RungStepper:
     # -- forces ----------------------------------------------------------------
 
     def _forces(self, active, rebuild, t):
-        torch, F = self.torch, self.F
-        act8 = None if active is None else active.to(torch.uint8).contiguous()
-        F.gravity(self.pos, self.mass, active=act8, rebuild=rebuild)
+        """active: uint8 per particle, or None for all."""
+        F = self.F
+        F.gravity(self.pos, self.mass, active=active, rebuild=rebuild)
         kw = {}
         if self.first == "learned" and self.prev is not None:
             kw = dict(first="learned", dk_prev=self.prev[0], divv_prev=self.prev[1],
@@ -300,26 +301,21 @@ class RungStepper:
 
     def _ladder(self):
         """dt_min = the 1st percentile of dt_ind (linear interpolation, FP64),
-        dt_max = dt_min 2^(rungs-1), and every particle's rung."""
+        dt_max = dt_min 2^(rungs-1); the rungs follow in the kick of mode 2."""
         self.dt_min = float(self.torch.quantile(self.F.dt_ind, 0.01).item())
         self.dt_max = self.dt_min * 2 ** (self.R - 1)
-        self.rung = self._rung_of(self.F.dt_ind)
 
-    def _rung_of(self, dt):
-        torch = self.torch
-        r = torch.ceil(torch.log2(torch.clamp(self.dt_max / dt, min=1.0)))
-        return torch.clamp(r, 0, self.R - 1).to(torch.int64)
-
-    def _kick(self, sel, dt_r):
-        """Half kick of the selected particles by dt_r / 2 (per particle); no
-        host synchronisation (masks become zero weights)."""
-        torch, F = self.torch, self.F
-        h = torch.where(sel, 0.5 * dt_r, torch.zeros_like(dt_r))
-        self.vel.addcmul_(h[:, None], F.acc)
-        if self.ng:
-            gsel = sel[:self.ng]
-            ug = self.u[:self.ng]
-            ug.copy_(torch.where(gsel, torch.clamp(ug + h[:self.ng] * F.dudt, min=self.u_floor), ug))
+    def _kick(self, mode, active):
+        """gf_step_kick_dev: mode 0 half kicks of the active particles by
+        dt_max 2^-rung / 2 (v, and u of the gas with du/dt); mode 1 first
+        moves the active particles to the rung of their dt_ind, to a longer
+        step only where this sub-step is also its boundary; mode 2 first moves
+        every particle to it (a new ladder).  No host synchronisation."""
+        F, ng = self.F, self.ng
+        _lib.check(self.L.gf_step_kick_dev(
+            self.n, ng, self.k, self.R, self.dt_max, mode, active.data_ptr(), F.acc.data_ptr(),
+            F.dudt.data_ptr() if ng else None, F.dt_ind.data_ptr(), self.rung.data_ptr(),
+            self.vel.data_ptr(), self.u.data_ptr() if ng else None, self.u_floor, F.sp))
 
     # -- diagnostics -----------------------------------------------------------
 
@@ -351,38 +347,38 @@ class RungStepper:
         """Forces of every particle, rungs, opening half kicks."""
         torch = self.torch
         self.t = 0.0
+        self.k = 0
         self._forces(None, True, 0.0)
         self._ladder()
+        self._kick(2, torch.zeros_like(self.active))           # the rungs only
         self.E0 = self.energy()
-        all_ = torch.ones(self.n, dtype=torch.bool, device=self.dev)
-        self._kick(all_, self.dt_max / 2.0 ** self.rung.double())
-        self.k = 0
+        self._kick(0, torch.ones_like(self.active))
 
     def substep(self):
         torch = self.torch
         R = self.R
         self.k += 1
         self.t += self.dt_min
-        self.pos += self.dt_min * self.vel
-        period = 2 ** (R - 1 - self.rung)                      # sub-steps per step of each rung
-        active = (self.k % period) == 0
-        self.active_counts.append(active.sum())
+        if self.k >= len(self._counts):
+            grown = torch.zeros(2 * len(self._counts), dtype=torch.int32, device=self.dev)
+            grown[:len(self._counts)] = self._counts
+            self._counts = grown
+        self.active_counts.append(self._counts[self.k])
+        # every particle drifts; active: the particles at a boundary of their step
+        _lib.check(self.L.gf_step_drift_dev(self.n, self.dt_min, self.k, R, self.pos.data_ptr(),
+                                            self.vel.data_ptr(), self.rung.data_ptr(),
+                                            self.active.data_ptr(),
+                                            self._counts.data_ptr() + 4 * self.k, self.F.sp))
         long_step = self.k % 2 ** (R - 1) == 0
-        self._forces(active, long_step or not self.refit, self.t)
-        dt_old = self.dt_max / 2.0 ** self.rung.double()
-        self._kick(active, dt_old)                             # closing half kicks
+        self._forces(self.active, long_step or not self.refit, self.t)
+        self._kick(0, self.active)                             # closing half kicks
         if long_step:
             # every particle is at a boundary: a new ladder of steps
             self.L_sync.append(self.angular_momentum())
             self._ladder()
+            self._kick(2, self.active)                         # opening half kicks
         else:
-            new = self._rung_of(self.F.dt_ind)
-            # a longer step only where its own boundary is now
-            longer = new < self.rung
-            ok = (self.k % (2 ** (R - 1 - new))) == 0
-            new = torch.where(longer & ~ok, self.rung, new)
-            self.rung = torch.where(active, new, self.rung)
-        self._kick(active, self.dt_max / 2.0 ** self.rung.double())   # opening half kicks
+            self._kick(1, self.active)                         # new rungs, opening half kicks
 
     def synchronised_energy(self):
         """Energy and |L| at a long-step boundary (every particle just took its

@@ -153,6 +153,39 @@ int gf_direct_dev(int n, const double* pos, const double* mass, int nt, const in
  * particles by direct summation, for the energy of a run (device arrays). */
 int gf_potential_dev(int n, const double* pos, const double* mass, double* phi, double eps,
                      void* stream);
+/* --------------------------------------------- the caller's step, on the device
+ * No counterparts in the reference's C++: these are the NumPy lines of
+ * galaxy_formation/simulation.py around the force calls, for a caller that
+ * keeps its state on the device (device arrays, FP64, the operations in the
+ * order those lines write them). */
+
+/* simulation.py:68-71: acc = acc_grav + acc_sph on the gas rows 0 .. ng-1
+ * (acc_grav elsewhere); dt_ind = 0.3 sqrt(eps / max(|acc|, 1e-30)), on the
+ * gas rows the smaller of that and dt_c; *dt_grav (one device double) =
+ * 0.3 sqrt(eps / max(max |acc|, 1e-30)).  acc_sph and dt_c may be NULL when
+ * ng = 0. */
+int gf_combine_dev(int n, int ng, double eps, const double* acc_grav, const double* acc_sph,
+                   const double* dt_c, double* acc, double* dt_ind, double* dt_grav,
+                   void* stream);
+/* Sub-step k of a block leapfrog with `rungs` rungs (the drift of every
+ * particle, simulation.py:122, 138): pos += dt vel; active[i] = 1
+ * where k is a multiple of 2^(rungs-1-rung[i]) (the particles at a boundary
+ * of their step), else 0; *count (device int) += the number active. */
+int gf_step_drift_dev(int n, double dt, long long k, int rungs, double* pos, const double* vel,
+                      const int64_t* rung, uint8_t* active, int* count, void* stream);
+/* Half kicks of the active particles (simulation.py:121, 124, 131, 137, 142, 148)
+ * by h = dt_max 2^-rung / 2: vel += h acc, and on the gas rows
+ * u = max(u + h dudt, u_floor).  Before the kick, mode 1 moves each active
+ * particle to the rung of its dt_ind, to a longer step only where sub-step k
+ * is also a boundary of it, and mode 2 moves every particle to the rung of its
+ * dt_ind (a new ladder); mode 0 keeps the rungs.  The rung of dt is
+ * min(max(ceil(log2(max(dt_max / dt, 1))), 0), rungs - 1).  dudt and u may be
+ * NULL when ng = 0, dt_ind in mode 0. */
+int gf_step_kick_dev(int n, int ng, long long k, int rungs, double dt_max, int mode,
+                     const uint8_t* active, const double* acc, const double* dudt,
+                     const double* dt_ind, int64_t* rung, double* vel, double* u, double u_floor,
+                     void* stream);
+
 /* Interaction count of the last gravity call (synchronises). */
 int gf_interactions_dev(long long* out);
 /* The part of that count that are node (not particle) entries. */
