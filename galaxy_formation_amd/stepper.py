@@ -20,6 +20,8 @@ trains on.  (The reference's two-level sub-stepping, cooling and star
 formation are not part of this force path.)
 """
 
+import contextlib
+
 import numpy as np
 
 from . import _lib
@@ -239,6 +241,10 @@ class RungStepper:
     every sub-step; only the active particles use it.  Cooling, star formation
     and feedback are not part of this force path.
 
+    With overlap (the default), the gas pipeline runs on a second stream
+    beside the gravity tree build and walk: it waits for the sub-step's drift
+    and the forces are summed once both are done.
+
     Energy and angular momentum are measured where every particle is at a
     boundary (every 2^(rungs-1) sub-steps), on the synchronised velocities
     and energies (the opening half kicks taken back): kinetic + thermal +
@@ -246,7 +252,7 @@ class RungStepper:
 
     def __init__(self, particles, theta=0.5, k=64, precision="fp32", rungs=6, first="plain",
                  model=None, device="cuda:0", refit=False, quad_relative_tolerance=None,
-                 sph_precision=None):
+                 sph_precision=None, overlap=True):
         import torch
         self.torch = torch
         dev = torch.device(device)
@@ -276,12 +282,19 @@ class RungStepper:
         self.L_sync = []
         self.t_hydro = 0.0        # time of the last hydro evaluation
         self.prev = None          # (d_k, div v, dt_Courant) of that evaluation
+        # overlap: the gas on a second stream beside the gravity, forked after
+        # the drift and joined before the forces are summed
+        self.gas_stream = torch.cuda.Stream(device=dev) if overlap and self.ng else None
+        self._fork, self._join = torch.cuda.Event(), torch.cuda.Event()
 
     # -- forces ----------------------------------------------------------------
 
     def _forces(self, active, rebuild, t):
         """active: uint8 per particle, or None for all."""
-        F = self.F
+        torch, F, gs = self.torch, self.F, self.gas_stream
+        if gs is not None:
+            self._fork.record()
+            gs.wait_event(self._fork)
         F.gravity(self.pos, self.mass, active=active, rebuild=rebuild)
         kw = {}
         if self.first == "learned" and self.prev is not None:
@@ -292,11 +305,15 @@ class RungStepper:
         else:
             kw = dict(first="plain")
         if self.ng:
-            self.passes.append(F.hydro(self.pos[:self.ng], self.vel[:self.ng],
-                                       self.mass[:self.ng], self.u[:self.ng], **kw))
-            if self.ng > 40:
-                self.prev = (F.dk.clone(), F.divv.clone(), F.dt_c.clone())
-                self.t_hydro = t
+            with torch.cuda.stream(gs) if gs is not None else contextlib.nullcontext():
+                self.passes.append(F.hydro(self.pos[:self.ng], self.vel[:self.ng],
+                                           self.mass[:self.ng], self.u[:self.ng], **kw))
+                if self.ng > 40:
+                    self.prev = (F.dk.clone(), F.divv.clone(), F.dt_c.clone())
+                    self.t_hydro = t
+            if gs is not None:
+                self._join.record(gs)
+                torch.cuda.current_stream().wait_event(self._join)
         F.combine()
 
     def _ladder(self):
