@@ -40,6 +40,8 @@ SIGNATURES = {
     "gf_get_sph_precision": (_ci, []),
     "gf_set_knn_blocks": (_ci, [_ci]),
     "gf_get_knn_blocks": (_ci, []),
+    "gf_set_knn_merge": (_ci, [_ci]),
+    "gf_get_knn_merge": (_ci, []),
     "gf_set_walk_blocks": (_ci, [_ci]),
     "gf_get_walk_blocks": (_ci, []),
     "gf_get_precision": (_ci, []),

@@ -164,6 +164,17 @@ int gf_set_knn_blocks(int blocks) {
 
 int gf_get_knn_blocks(void) { return gf_sph_fast_blocks(g_ctx.sph); }
 
+int gf_set_knn_merge(int members) {
+    if (members < 0 || members > 64) {
+        gf_set_error("merged work items: members must be in [0, 64] (0 = default, 1 = none)");
+        return -1;
+    }
+    g_ctx.sph.merge_members = members;
+    return 0;
+}
+
+int gf_get_knn_merge(void) { return gf_sph_merge_members(g_ctx.sph); }
+
 int gf_set_walk_blocks(int blocks) {
     if (blocks < 0 || blocks > 65536) {
         gf_set_error("walk blocks must be in [0, 65536] (0 = default)");

@@ -22,18 +22,21 @@ struct GfSph {
     int *inv = nullptr;          // input index -> tree position
     int *leaf_of = nullptr;      // leaf node of each tree position
     int *anc = nullptr;          // per group: its node and ancestors, GF_ANC ints (-1 past the root)
+    int *items = nullptr;        // work items of a pass from a caller's radii: int4 (first group
+                                 // position, groups, first particle, members) per item
     int *nb = nullptr;           // [k][n] neighbours: tree positions (or input indices)
     int *state = nullptr;        // pass counters (layout in gf_sph.hip)
     int *ovf = nullptr, *ovf2 = nullptr;
     int *gstack = nullptr;
     int cap_groups = 0;
     int fast_blocks = 0;         // persistent blocks of the fast neighbour pass (0 = default)
+    int merge_members = 0;       // members of a merged work item at most (0 = default, 1 = none)
     int pair_f32 = 0;            // density and forces: FP32 pair arithmetic, FP64 sums
     int last_passes = 0;         // pass launches of the last solve
     long long stat_queued = 0;   // groups the fast pass handed on, summed over passes
     long long stat_queued2 = 0;  // groups handed on to the global-stack pass
     // the solve in flight (gf_sph_knn_begin .. gf_sph_knn_end)
-    int knn_kq = 0, knn_pos_out = 0, knn_sorted = 0, knn_narrow = 0;
+    int knn_kq = 0, knn_pos_out = 0, knn_sorted = 0, knn_narrow = 0, knn_items = 0;
     int knn_enqueued = 0;        // pass launches enqueued so far
     int knn_open = 0;            // begun, not yet ended
 };
@@ -44,6 +47,9 @@ void gf_sph_free(GfSph& S);
 // radius (S.fast_blocks, or the default of one per resident slot of the
 // narrow shape: 1280).  From the cell densities the default is 1024.
 int gf_sph_fast_blocks(const GfSph& S);
+// Members a work item of the first pass from a caller's radii holds at most
+// (S.merge_members, or the default); 1 = every group is an item of its own.
+int gf_sph_merge_members(const GfSph& S);
 
 // Builds the gas tree and gathers velocities and energies into tree order.
 // pos, vel (n, 3), mass, u (n) are device arrays in input order (vel and u

@@ -77,7 +77,7 @@ struct KnnArgs {
                     // queue 1 and 2 lengths summed over passes; [11] passes that
                     // had work; [12] errors summed over passes; [13] internal
                     // bounds errors of the fast pass (bit 0 candidate range,
-                    // bit 1 member-range stack)
+                    // bit 1 member-range stack); [14] merged work items
     int* ovf;       // queue 1: groups the fast pass hands to the heap pass
     int* ovf2;      // queue 2: groups the heap pass hands to its global-stack variant
     int* gstack;
@@ -86,6 +86,9 @@ struct KnnArgs {
     int* gmark;             // pass tag of the last retry request per group
     int pass_tag;
     int list_cap;           // candidate-list capacity (tests lower it)
+    const int4* items;      // first pass only (null = one group per item): merged work items
+                            // (first group position, groups, first particle, members); their
+                            // count is state[14]
 };
 
 // Nodes to pop from a LIFO of capacity cap holding sn entries: a batch of 64
@@ -375,6 +378,10 @@ constexpr int FSCAP = 1024;                  // stack of the gather walk
 #define KNN_WIDE_GRID 1024
 #endif
 constexpr int FAST_BLOCKS = KNN_FAST_GRID;
+// members of a merged work item at most (first pass from a caller's radii)
+#ifndef KNN_MERGE_MEMBERS
+#define KNN_MERGE_MEMBERS 40
+#endif
 constexpr int WIDE_BLOCKS = KNN_WIDE_GRID;
 constexpr uint32_t KEY_NONE = 0xFFFFFFFFu;
 constexpr double KEY_SCALE = 4294967294.0;   // keys d^2 / r^2 * KEY_SCALE < KEY_NONE
@@ -721,8 +728,7 @@ __device__ int select_member(const KnnArgs& a, int t, double xi, double yi, doub
         __builtin_amdgcn_wave_barrier();
         if (bcount <= 64 || shift == 0) break;
         shift -= 8;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) hist[4 * lane + q] = 0;
+        reinterpret_cast<int4*>(hist)[lane] = make_int4(0, 0, 0, 0);
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
@@ -830,7 +836,10 @@ __global__ __launch_bounds__(FT, Shape::BLOCKS_PER_CU) void k_knn_fast(KnnArgs a
     __shared__ unsigned long long rstack[16];
     __shared__ int rmode[16];                   // 0: bounding sphere, 1: union of member spheres
     const int lane = __lane_id(), w = threadIdx.x >> 6;
-    const int ngroups = a.retry_in ? a.state[8] : a.counters[1];
+    // the first pass serves merged work items when the solve built them; its
+    // retries, the later passes and the heap pass's queue are groups
+    const bool use_items = a.items != nullptr && a.retry_in == nullptr;
+    const int nwork = a.retry_in ? a.state[8] : (use_items ? a.state[14] : a.counters[1]);
     const int cap = min(CMAX, a.list_cap);
     if (threadIdx.x < 16) ctl[threadIdx.x] = 0;
     __syncthreads();
@@ -839,12 +848,23 @@ __global__ __launch_bounds__(FT, Shape::BLOCKS_PER_CU) void k_knn_fast(KnnArgs a
         if (threadIdx.x == 0) ctl[0] = atomicAdd(&a.state[1], 1);
         __syncthreads();
         const int gi = ctl[0];
-        if (gi >= ngroups) break;
-        const int gp = a.retry_in ? a.retry_in[gi] : gi;
-        const int G = a.groups[gp];
-        const int b0 = a.start[G], np = a.count[G];
+        if (gi >= nwork) break;
+        // the work's groups are positions [gp, gp + ngrp), its members the
+        // tree positions [b0, b0 + np)
+        int gp, ngrp, b0, np;
+        if (use_items) {
+            const int4 it = a.items[gi];
+            gp = it.x; ngrp = it.y; b0 = it.z; np = it.w;
+        } else {
+            gp = a.retry_in ? a.retry_in[gi] : gi;
+            ngrp = 1;
+            const int G = a.groups[gp];
+            b0 = a.start[G]; np = a.count[G];
+        }
         if (np > 64) {
-            if (threadIdx.x == 0) a.ovf[atomicAdd(&a.state[3], 1)] = gp;
+            // a depth-capped leaf of more than 64 (always an item of its own)
+            if (threadIdx.x == 0)
+                for (int g = 0; g < ngrp; ++g) a.ovf[atomicAdd(&a.state[3], 1)] = gp + g;
             __syncthreads();
             continue;
         }
@@ -993,8 +1013,12 @@ __global__ __launch_bounds__(FT, Shape::BLOCKS_PER_CU) void k_knn_fast(KnnArgs a
         if (lane == 0 && again) atomicOr(&ctl[3], 2);
         __syncthreads();
         if (threadIdx.x == 0) {
-            if (ctl[3] & 1) a.ovf[atomicAdd(&a.state[3], 1)] = gp;
-            if (ctl[3] & 2) request_retry(a, gp);
+            // the heap pass and the next pass take the work's groups (members
+            // already resolved are skipped there)
+            for (int g = 0; g < ngrp; ++g) {
+                if (ctl[3] & 1) a.ovf[atomicAdd(&a.state[3], 1)] = gp + g;
+                if (ctl[3] & 2) request_retry(a, gp + g);
+            }
             ctl[3] = 0;
         }
         __syncthreads();
@@ -1119,9 +1143,11 @@ __global__ __launch_bounds__(TB) void k_knn_init(int n, int kq, const double* __
                                                  double* __restrict__ r_hi,
                                                  int8_t* __restrict__ done,
                                                  int* __restrict__ passes,
-                                                 int* __restrict__ gmark) {
+                                                 int* __restrict__ gmark,
+                                                 int* __restrict__ state) {
     const int t = blockIdx.x * TB + threadIdx.x;
     if (t >= n) return;
+    if (t == 0) state[14] = 0;      // merged work items (k_knn_items counts them)
     gmark[t] = 0;                   // groups <= particles
     double rr;
     if (r_init) {
@@ -1660,7 +1686,8 @@ int sph_reserve(GfSph& S, int n, int k) {
     const int ng = gf_tree_node_bound(S.cap_n);
     if (ng > S.cap_groups) {
         if (dalloc(S.ovf, ng) || dalloc(S.ovf2, ng) || dalloc(S.retry, 2 * size_t(ng)) ||
-            dalloc(S.knode, 8 * size_t(ng)) || dalloc(S.anc, size_t(GF_ANC) * size_t(ng)))
+            dalloc(S.knode, 8 * size_t(ng)) || dalloc(S.anc, size_t(GF_ANC) * size_t(ng)) ||
+            dalloc(S.items, 4 * size_t(ng)))
             return -1;
         S.cap_groups = ng;
     }
@@ -1671,17 +1698,22 @@ int sph_reserve(GfSph& S, int n, int k) {
 
 int gf_sph_fast_blocks(const GfSph& S) { return S.fast_blocks > 0 ? S.fast_blocks : FAST_BLOCKS; }
 
+int gf_sph_merge_members(const GfSph& S) {
+    return S.merge_members > 0 ? S.merge_members : KNN_MERGE_MEMBERS;
+}
+
 static int wide_blocks(const GfSph& S) { return S.fast_blocks > 0 ? S.fast_blocks : WIDE_BLOCKS; }
 
 void gf_sph_free(GfSph& S) {
     gf_tree_free(S.tree);
     dfree(S.vx); dfree(S.vy); dfree(S.vz); dfree(S.u); dfree(S.r); dfree(S.dk); dfree(S.h);
     dfree(S.rho); dfree(S.done); dfree(S.passes); dfree(S.inv); dfree(S.leaf_of); dfree(S.r_lo); dfree(S.r_hi); dfree(S.rec); dfree(S.p4); dfree(S.gmark); dfree(S.retry); dfree(S.knode);
-    dfree(S.anc);
+    dfree(S.anc); dfree(S.items);
     dfree(S.nb); dfree(S.state); dfree(S.ovf); dfree(S.ovf2); dfree(S.gstack);
-    const int fb = S.fast_blocks;
+    const int fb = S.fast_blocks, mm = S.merge_members;
     S = GfSph();
     S.fast_blocks = fb;
+    S.merge_members = mm;
 }
 
 int gf_sph_setup(GfSph& S, int n, const double* pos, const double* vel, const double* mass,
@@ -1712,6 +1744,7 @@ static KnnArgs knn_args(GfSph& S) {
     a.r_lo = S.r_lo; a.r_hi = S.r_hi; a.dk = S.dk; a.done = S.done; a.passes = S.passes;
     a.nb = S.nb; a.n = T.n; a.kq = S.knn_kq; a.tree_pos_out = S.knn_pos_out;
     a.state = S.state; a.ovf = S.ovf; a.ovf2 = S.ovf2; a.gstack = S.gstack;
+    a.items = S.knn_items ? reinterpret_cast<const int4*>(S.items) : nullptr;
     return a;
 }
 
@@ -1769,7 +1802,15 @@ int gf_sph_knn_begin(GfSph& S, int kq, const double* r_init, bool tree_pos_out, 
                        s, T.counters, T.groups, T.parent, S.anc);
     hipLaunchKernelGGL(k_knn_init, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, kq, r_init, T.idx,
                        S.leaf_of, T.parent, T.count, T.box, S.r, S.r_lo, S.r_hi, S.done,
-                       S.passes, S.gmark);
+                       S.passes, S.gmark, S.state);
+    // from a caller's radii the first pass serves merged items (about k
+    // candidates a member: sharing a gather pays); the cell-density start
+    // overshoots, and its gathers stay per group
+    S.knn_items = r_init && gf_sph_merge_members(S) > 1 ? 1 : 0;
+    if (S.knn_items &&
+        gf_tree_merge_groups(T, std::min(64, gf_sph_merge_members(S)), S.state + 14,
+                             reinterpret_cast<int4*>(S.items), s))
+        return -1;
     hipLaunchKernelGGL(k_knode_pack, dim3(std::min(2048, gf_div_up(T.cap_nodes, TB))), dim3(TB), 0,
                        s, T.counters, T.box, T.leaf, T.start, T.count, T.child8,
                        reinterpret_cast<KNode*>(S.knode));

@@ -58,6 +58,12 @@ void gf_tree_free(GfTree& T);
 // with_com = false skips the centres of mass (a tree used for neighbour search).
 int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float theta,
                   hipStream_t s, bool with_com = true);
+// Work items of runs of sibling groups (children of one node, consecutive
+// in octant order) of at most cap <= 64 particles in all, every other group
+// an item of its own: items[i] = (first group position, groups, first tree
+// position, particles), appended in any order after *nitems (device; the
+// caller zeroes it first) by atomic increments.
+int gf_tree_merge_groups(const GfTree& T, int cap, int* nitems, int4* items, hipStream_t s);
 // Quadrupole moments of the held tree (computed once per build or refit, when
 // a walk asks for them).
 int gf_tree_quadrupoles(const GfTree& T, hipStream_t s);

@@ -829,6 +829,72 @@ void dfree(T*& p) {
 
 int node_grid(int cap_nodes) { return max(1, min(2048, gf_div_up(cap_nodes, TB))); }
 
+// Work items of several groups.  Groups (the largest nodes of at most 64
+// particles) average about 18 particles on the disk and cloud workloads, and
+// every group pays for a tree walk of its own; an item of several groups
+// shares it.  An item is a run of sibling
+// groups (children of one node, consecutive in octant order with only empty
+// octants between them) taken in order while the item holds at most `cap`
+// members; any other group is an item of its own.  The groups of an item
+// are consecutive group positions and its members consecutive tree
+// positions.  One thread per group; the first group of an item appends it
+// (the order of the items changes no result: work on an item depends only
+// on the item).
+__global__ __launch_bounds__(TB) void k_merge_groups(const int* __restrict__ counters,
+                                                  const int* __restrict__ groups,
+                                                  const int* __restrict__ parent,
+                                                  const int* __restrict__ child8,
+                                                  const int* __restrict__ start,
+                                                  const int* __restrict__ count, int cap,
+                                                  int* __restrict__ nitems,
+                                                  int4* __restrict__ items) {
+    const int ng = counters[1];
+    for (int gp = blockIdx.x * TB + threadIdx.x; gp < ng; gp += gridDim.x * TB) {
+        const int G = groups[gp];
+        const int np = count[G];
+        const int P = parent[G];
+        bool head = true;
+        int ig = 1, im = np;
+        if (np <= cap && P >= 0) {
+            // the parent holds more than a group (else it would be the
+            // group): its children of at most cap particles are groups, and
+            // a larger child ends a run (its subtree's groups lie between)
+            const int4* cs = reinterpret_cast<const int4*>(child8 + 8 * size_t(P));
+            const int4 lo = cs[0], hi = cs[1];
+            const int ch[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+            int cnt[8];
+#pragma unroll
+            for (int o = 0; o < 8; ++o) cnt[o] = ch[o] >= 0 ? count[ch[o]] : 0;
+            int cur = 0, first = -1, groups_in = 0;   // the open item
+            bool open = false;
+            for (int o = 0; o < 8; ++o) {
+                if (ch[o] < 0) continue;              // empty octant: no particles between
+                const bool fits = cnt[o] <= cap;
+                if (open && (!fits || cur + cnt[o] > cap)) {
+                    if (first == G) break;            // G's item ends before child o
+                    open = false;
+                }
+                if (!fits) continue;
+                if (!open) {
+                    open = true;
+                    cur = 0;
+                    first = ch[o];
+                    groups_in = 0;
+                }
+                cur += cnt[o];
+                ++groups_in;
+                if (ch[o] == G) head = first == G;
+            }
+            // G heads its item: cur and groups_in are that item's
+            if (head) {
+                ig = groups_in;
+                im = cur;
+            }
+        }
+        if (head) items[atomicAdd(nitems, 1)] = make_int4(gp, ig, start[G], im);
+    }
+}
+
 }  // namespace
 
 int gf_tree_node_bound(int n) {
@@ -920,6 +986,19 @@ static int com_pass(GfTree& T, hipStream_t s) {
     const bool fuse = T.want_quad != 0;
     if (moments_pass(T, fuse ? 1 : 0, s)) return -1;
     T.quad_ok = fuse ? 1 : 0;
+    return 0;
+}
+
+int gf_tree_merge_groups(const GfTree& T, int cap, int* nitems, int4* items, hipStream_t s) {
+    if (T.n == 0) return 0;
+    if (cap < 1 || cap > 64) {
+        gf_set_error("merged groups: at most %d members, must be in [1, 64]", cap);
+        return -1;
+    }
+    hipLaunchKernelGGL(k_merge_groups, dim3(std::min(2048, gf_div_up(T.cap_nodes, TB))), dim3(TB),
+                       0, s, T.counters, T.groups, T.parent, T.child8, T.start, T.count, cap,
+                       nitems, items);
+    GF_LAUNCH_CHECK();
     return 0;
 }
 

@@ -102,6 +102,13 @@ int gf_get_sph_precision(void);
  * do not depend on it.  blocks in [0, 65536]. */
 int gf_set_knn_blocks(int blocks);
 int gf_get_knn_blocks(void);
+/* Scheduling, no counterpart in the reference: the first pass of a solve
+ * from a caller's radii serves runs of sibling gas groups as one work item
+ * (one tree walk and one candidate gather) while the item holds at most
+ * `members` particles.  0 = the default (40), 1 = every group alone; at
+ * most 64.  The neighbours do not depend on it. */
+int gf_set_knn_merge(int members);
+int gf_get_knn_merge(void);
 /* Scheduling, no counterpart in the reference (nbody.cpp:323-411 walks with
  * OpenMP threads): the number of persistent one-wave blocks of the gravity
  * walk, which draws groups from a queue.  0 = the default, 4096 (one per

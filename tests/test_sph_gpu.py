@@ -354,6 +354,26 @@ def test_neighbours_independent_of_pass_blocks(blocks):
     np.testing.assert_array_equal(dk, od[:, -1])
 
 
+@pytest.mark.parametrize("members", [1, 16, 40, 64])
+def test_neighbours_independent_of_merged_items(members):
+    """The first pass from a caller's radii serves runs of sibling groups
+    of at most `members` particles as one work item (1: every group alone):
+    the neighbours and d_k are those of brute force whatever the items."""
+    from galaxy_formation_amd import _lib
+    L = _lib.load()
+    pos, *_ = disk_gas(20000, seed=19)
+    od, onb = O.knn(pos, 64)
+    _lib.check(L.gf_set_knn_merge(members))
+    try:
+        assert L.gf_get_knn_merge() == members
+        for first in (1.02, 1.3, 0.8):
+            nb, dk, _ = _device_solve(pos, 64, first * od[:, -1])
+            np.testing.assert_array_equal(np.sort(nb, axis=1), np.sort(onb, axis=1))
+            np.testing.assert_array_equal(dk, od[:, -1])
+    finally:
+        _lib.check(L.gf_set_knn_merge(0))
+
+
 @pytest.mark.parametrize("k,scale", [(32, None), (64, 1.0), (64, 0.4), (64, 3.0), (96, None)])
 def test_device_neighbour_sets_equal_bruteforce(k, scale):
     pos, *_ = disk_gas(20000, seed=11)
