@@ -82,10 +82,19 @@ def parse():
                    help="persistent blocks of the neighbour pass while the gravity walk runs "
                         "on the other stream (0 = the library default, every resident slot)")
     p.add_argument("--order", default="knn-first",
-                   choices=("gas-then-walk", "knn-then-walk", "knn-first", "walk-first"),
+                   choices=("gas-then-walk", "knn-then-walk", "knn-first", "gas-first",
+                            "walk-first"),
                    help="overlapped step: the gas tree and the neighbour solve first, the walk "
                         "after the solve (knn-then-walk); the gas passes enqueued before the "
-                        "walk, beside it (knn-first); or the walk enqueued first (walk-first)")
+                        "walk, beside it (knn-first); the same with the gas pipeline enqueued "
+                        "before the gravity tree build (gas-first); or the walk enqueued first "
+                        "(walk-first)")
+    p.add_argument("--gas-tree", default="gravity", choices=("gravity", "own"),
+                   help="the gas tree from the gravity tree's Morton order (gravity) or from a "
+                        "sort of its own (own); orders that enqueue the gas first use their own")
+    p.add_argument("--records", default="walk", choices=("walk", "build"),
+                   help="pack the walk's records with the walk (walk) or right after the tree "
+                        "build (build); best_schedule tries both")
     p.add_argument("--build-alone", type=int, default=None, metavar="PREC",
                    help="profiling aid: the workload's gravity tree (with the quadrupoles "
                         "for PREC 2) and gas tree, one warm-up and five builds each, "
@@ -236,6 +245,10 @@ class Bench:
         # the previous step's gravity (set by main): the quadrupole walk reads
         # it from its acc array on entry, as a simulation's acc_grav holds it
         self.acc_prev = None
+        # set inside a step once its gravity tree build is enqueued (the gas
+        # tree may then come from it)
+        self.grav_built = False
+        self.caller_radii = False
 
     def p(self, t):
         return None if t is None else t.data_ptr()
@@ -245,15 +258,23 @@ class Bench:
         self.hydro_begin(prec, pos, first, rec, stream)
         return self.hydro_end(prec, rec, stream)
 
-    def hydro_begin(self, prec, pos, first, rec=None, stream=None, after=None):
+    def hydro_begin(self, prec, pos, first, rec=None, stream=None, after=None, passes=True):
         """Gas tree, first radii, and the neighbour solve's first passes
         enqueued without waiting for the device (the passes after the event
-        `after`, when given)."""
+        `after`, when given; with passes=False they are left for
+        hydro_passes)."""
         stream = stream or self.stream
         L, chk, sp = self.L, self._lib.check, stream.cuda_stream
         rec(2) if rec else None
-        chk(L.gf_hydro_setup_dev(self.ng, self.p(pos), self.p(self.vel), self.p(self.mass),
-                                 self.p(self.u), sp))
+        if self.grav_built and pos is self.pos and self.a.gas_tree == "gravity":
+            # the gas tree from the gravity tree's Morton order (the gas is
+            # the first ng rows): no sort of its own; the stream waits for
+            # the gravity build's sort
+            chk(L.gf_hydro_setup_from_tree_dev(self.ng, self.p(pos), self.p(self.vel),
+                                               self.p(self.mass), self.p(self.u), sp))
+        else:
+            chk(L.gf_hydro_setup_dev(self.ng, self.p(pos), self.p(self.vel), self.p(self.mass),
+                                     self.p(self.u), sp))
         rec(3) if rec else None
         r_init = None
         if first is not None and first[0] == "radius":
@@ -270,10 +291,18 @@ class Bench:
         # but a few groups in one), five from the cell densities (the plain
         # start's usual count)
         chk(L.gf_hydro_knn_begin_dev(self.a.k, self.p(r_init), 0, sp))
+        self.caller_radii = r_init is not None
+        if passes:
+            self.hydro_passes(stream, after)
+
+    def hydro_passes(self, stream, after=None):
+        """The solve's first passes, after the event `after` when given; the
+        event knn_go marks their start."""
         if after is not None:
             stream.wait_event(after)
         self.knn_go.record(stream)
-        chk(L.gf_hydro_knn_more_dev(2 if r_init is not None else 5, sp))
+        self._lib.check(self.L.gf_hydro_knn_more_dev(2 if self.caller_radii else 5,
+                                                     stream.cuda_stream))
 
     def hydro_end(self, prec, rec=None, stream=None):
         """The neighbour solve's end (waits for it), then h, density, forces."""
@@ -311,6 +340,7 @@ class Bench:
         def rec(i, st):
             if ev is not None:
                 ev[i].record(st)
+        self.grav_built = False
         rec(0, self.stream)
         self.entry_acc(prec)          # inside the timed step
         self.fork.record(self.stream)
@@ -351,7 +381,37 @@ class Bench:
             self.join.record(gs)
             self.stream.wait_event(self.join)
             return passes
+        if overlap and self.a.order == "gas-first":
+            # the gas pipeline's chain of short launches is enqueued first,
+            # up to the neighbour solve's set-up, so it starts at once; then
+            # the gravity tree build; the neighbour passes wait for the build
+            # (their blocks would hold the CUs its kernels need) and the walk
+            # for the passes' start
+            gs.wait_event(self.fork)
+            with torch.cuda.stream(gs):
+                self.hydro_begin(prec, self.pos, first, lambda i: rec(i, gs), stream=gs,
+                                 passes=False)
+            chk(L.gf_tree_build_dev(self.n, self.p(self.pos), self.p(self.mass), self.a.theta, sp))
+            if self.a.records == "build":
+                chk(L.gf_walk_prepare_dev(prec, sp))
+            rec(1, self.stream)
+            self.tree_done.record(self.stream)
+            with torch.cuda.stream(gs):
+                self.hydro_passes(gs, after=self.tree_done)
+            self.stream.wait_event(self.knn_go)
+            chk(L.gf_walk_dev(self.n, None, self.p(self.acc_g[prec]), self.c.eps, prec, sp))
+            rec(6, self.stream)
+            with torch.cuda.stream(gs):
+                passes = self.hydro_end(prec, lambda i: rec(i, gs), stream=gs)
+            self.join.record(gs)
+            self.stream.wait_event(self.join)
+            return passes
         chk(L.gf_tree_build_dev(self.n, self.p(self.pos), self.p(self.mass), self.a.theta, sp))
+        self.grav_built = True
+        if self.a.records == "build":
+            # the walk's records are packed with the build, while the gas
+            # pipeline on the other stream is still short of its passes
+            chk(L.gf_walk_prepare_dev(prec, sp))
         rec(1, self.stream)
         if overlap and self.a.order == "knn-first":
             # the gas tree builds beside the gravity tree; the neighbour
@@ -443,27 +503,41 @@ class Bench:
 # them), 0 takes every resident slot (the quadrupole walk's 10 KB per wave
 # fills a CU, so the pass runs before it instead)
 KNN_BLOCKS_TRIED = (512, 0)
+# where the walk's records are packed: with the walk (after the neighbour
+# passes have started), or right after the tree build (beside the gas
+# pipeline); the plain path's walk runs beside its neighbour passes and
+# gains from the second, the fast path's waits for the passes' blocks
+RECORDS_TRIED = ("walk", "build")
+# the gas tree: from the gravity tree's Morton order (no sort of its own), or
+# built alone; the cell-density start of the plain path reads the cells of
+# the tree built alone better (its first radii overshoot less)
+GAS_TREES_TRIED = ("gravity", "own")
 
 
 def best_schedule(B, prec, first):
     """The overlapped step timed with each neighbour-pass grid of
-    KNN_BLOCKS_TRIED (the same steps, the same work); the faster one is the
-    path's time, and every one tried is reported under 'schedules'."""
+    KNN_BLOCKS_TRIED, each packing of the walk's records of RECORDS_TRIED and
+    each gas tree of GAS_TREES_TRIED (the same steps, the same work); the
+    fastest is the path's time, and every one tried is reported under
+    'schedules'."""
     a = B.a
-    keep = a.knn_blocks_beside_walk
+    keep = (a.knn_blocks_beside_walk, a.records, a.gas_tree)
     tried, best = {}, None
-    for kb in KNN_BLOCKS_TRIED:
-        a.knn_blocks_beside_walk = kb
-        r = B.timed(prec, first)
-        tried[f"{a.order}, knn blocks {kb or 'all'}"] = r["ms"]
-        if best is None or r["ms"] < best["ms"]:
-            best = dict(r, knn_blocks_beside_walk=kb)
-    a.knn_blocks_beside_walk = keep
+    for gt in GAS_TREES_TRIED:
+        for rec in RECORDS_TRIED:
+            for kb in KNN_BLOCKS_TRIED:
+                a.knn_blocks_beside_walk, a.records, a.gas_tree = kb, rec, gt
+                r = B.timed(prec, first)
+                tried[f"{a.order}, knn blocks {kb or 'all'}, records with the {rec}, "
+                      f"gas tree {gt}"] = r["ms"]
+                if best is None or r["ms"] < best["ms"]:
+                    best = dict(r, knn_blocks_beside_walk=kb, records=rec, gas_tree=gt)
     best["schedules"] = tried
     # the steps of the chosen schedule leave their outputs last
-    a.knn_blocks_beside_walk = best["knn_blocks_beside_walk"]
+    a.knn_blocks_beside_walk, a.records, a.gas_tree = (best["knn_blocks_beside_walk"],
+                                                        best["records"], best["gas_tree"])
     B.timed(prec, first, steps=1)
-    a.knn_blocks_beside_walk = keep
+    a.knn_blocks_beside_walk, a.records, a.gas_tree = keep
     return best
 
 
@@ -569,9 +643,14 @@ def main():
         res[name]["rho_max_rel_err_vs_plain"] = float(np.max(np.abs(rf - rho_plain) / rho_plain))
     # the same steps with gravity and gas one after the other: phase times
     # without sharing the GPU
-    serial = {"plain_fp64": B.timed(1, None, overlap=False),
-              "fast_fp32": B.timed(0, ("learned", model), overlap=False),
-              "fast_fp32_quad": B.timed(2, ("learned", model), overlap=False)}
+    # (each path with the gas tree and record packing of its chosen schedule)
+    serial = {}
+    keep = (a.records, a.gas_tree)
+    for name, prec, first in (("plain_fp64", 1, None), ("fast_fp32", 0, ("learned", model)),
+                              ("fast_fp32_quad", 2, ("learned", model))):
+        a.records, a.gas_tree = res[name]["records"], res[name]["gas_tree"]
+        serial[name] = B.timed(prec, first, overlap=False)
+    a.records, a.gas_tree = keep
     for name in serial:
         res[name]["serial"] = {k: serial[name][k] for k in ("ms",) + tuple(x + "_ms" for x in PHASES)}
     # the walk kernel alone, for its roofline
@@ -580,7 +659,8 @@ def main():
     node_inter = np.zeros(1, np.int64)
     B._lib.check(B.L.gf_node_interactions_dev(B._lib.ptr(node_inter)))
     # the neighbour solve from the other first radii (FP32 path, 3 steps each,
-    # serial so that the solve has the GPU to itself)
+    # serial so that the solve has the GPU to itself; the gas tree from the
+    # gravity tree, as the learned start's step uses it)
     ab = {"learned": {k: serial["fast_fp32"][k] for k in ("knn_ms", "passes",
                                                           "mean_passes_per_particle")}}
     import math

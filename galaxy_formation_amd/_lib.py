@@ -52,6 +52,7 @@ SIGNATURES = {
     "gf_gravity_dev": (_ci, [_ci, _vp, _vp, _vp, _cd, _cd, _vp, _ci, _ci, _vp]),
     "gf_tree_build_dev": (_ci, [_ci, _vp, _vp, _cd, _vp]),
     "gf_walk_dev": (_ci, [_ci, _vp, _vp, _cd, _ci, _vp]),
+    "gf_walk_prepare_dev": (_ci, [_ci, _vp]),
     "gf_direct_dev": (_ci, [_ci, _vp, _vp, _ci, _vp, _vp, _cd, _vp]),
     "gf_potential_dev": (_ci, [_ci, _vp, _vp, _vp, _cd, _vp]),
     "gf_combine_dev": (_ci, [_ci, _ci, _cd, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -74,6 +75,7 @@ SIGNATURES = {
     "gf_hydro_dev": (_ci, [_ci, _vp, _vp, _vp, _vp, _cd, _ci, _vp, _cd, _cd, _cd,
                            _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_hydro_setup_dev": (_ci, [_ci, _vp, _vp, _vp, _vp, _vp]),
+    "gf_hydro_setup_from_tree_dev": (_ci, [_ci, _vp, _vp, _vp, _vp, _vp]),
     "gf_hydro_knn_dev": (_ci, [_ci, _vp, _vp]),
     "gf_hydro_knn_begin_dev": (_ci, [_ci, _vp, _ci, _vp]),
     "gf_hydro_knn_more_dev": (_ci, [_ci, _vp]),

@@ -18,6 +18,10 @@ struct GfCtx {
     int precision = GF_FP32;
     unsigned long long last_inter = 0;
     GfTree tree;
+    // the last gf_tree_build_dev: its positions, and an event recorded once
+    // its sorted keys are kept (gas trees built from them wait for it)
+    const double* tree_pos = nullptr;
+    hipEvent_t tree_sorted = nullptr;
     GfWalkScratch walk;
     GfSph sph;
     int sph_k = 0;                  // k of the held neighbour solve

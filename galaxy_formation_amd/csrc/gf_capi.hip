@@ -333,7 +333,14 @@ int gf_tree_build_dev(int n, const double* pos, const double* mass, double theta
         gf_set_error("gf_tree_build_dev: n must be >= 0");
         return -1;
     }
-    return gf_tree_build(g_ctx.tree, n, pos, mass, float(theta), gf_stream(stream));
+    if (!g_ctx.tree_sorted) GF_HIP(hipEventCreateWithFlags(&g_ctx.tree_sorted, hipEventDisableTiming));
+    g_ctx.tree.keep_sorted = 1;
+    g_ctx.tree_pos = nullptr;
+    if (gf_tree_build(g_ctx.tree, n, pos, mass, float(theta), gf_stream(stream), true,
+                      g_ctx.tree_sorted))
+        return -1;
+    g_ctx.tree_pos = pos;
+    return 0;
 }
 
 int gf_walk_dev(int n, const uint8_t* active, double* acc, double eps, int precision,
@@ -352,6 +359,19 @@ int gf_walk_dev(int n, const uint8_t* active, double* acc, double eps, int preci
     if (gf_walk_reserve(g_ctx.walk, g_ctx.tree.cap_nodes)) return -1;
     GF_HIP(hipMemsetAsync(g_ctx.walk.inter, 0, 2 * sizeof(unsigned long long), s));
     return gf_walk(g_ctx.tree, g_ctx.walk, active, acc, eps, precision, s);
+}
+
+int gf_walk_prepare_dev(int precision, void* stream) {
+    if (gf_ctx_init()) return -1;
+    if (g_ctx.tree.n > 0 && !g_ctx.tree.built) {
+        gf_set_error("gf_walk_prepare_dev: no tree built");
+        return -1;
+    }
+    if (precision != GF_FP32 && precision != GF_FP64 && precision != GF_FP32_QUAD) {
+        gf_set_error("gf_walk_prepare_dev: bad precision");
+        return -1;
+    }
+    return gf_walk_prepare(g_ctx.tree, g_ctx.walk, precision, gf_stream(stream));
 }
 
 int gf_direct_dev(int n, const double* pos, const double* mass, int nt, const int64_t* tgt,
@@ -718,6 +738,25 @@ int gf_hydro_setup_dev(int n, const double* pos, const double* vel, const double
     return gf_sph_setup(g_ctx.sph, n, pos, vel, mass, u, gf_stream(stream));
 }
 
+int gf_hydro_setup_from_tree_dev(int n, const double* pos, const double* vel, const double* mass,
+                                 const double* u, void* stream) {
+    if (gf_ctx_init()) return -1;
+    g_ctx.sph_k = 0;
+    if (n < 0 || n > g_ctx.tree.n) {
+        gf_set_error("gf_hydro_setup_from_tree_dev: n = %d must be in [0, %d] (the particles of "
+                     "the last gf_tree_build_dev)", n, g_ctx.tree.n);
+        return -1;
+    }
+    if (n > 0 && (!g_ctx.tree.built || pos != g_ctx.tree_pos || !g_ctx.tree_sorted)) {
+        gf_set_error("gf_hydro_setup_from_tree_dev: pos is not the array of the last "
+                     "gf_tree_build_dev");
+        return -1;
+    }
+    hipStream_t s = gf_stream(stream);
+    if (n > 0) GF_HIP(hipStreamWaitEvent(s, g_ctx.tree_sorted, 0));
+    return gf_sph_setup_from_tree(g_ctx.sph, g_ctx.tree, n, pos, vel, mass, u, s);
+}
+
 int gf_hydro_knn_dev(int k, const double* r_init, void* stream) {
     if (gf_ctx_init()) return -1;
     g_ctx.sph_k = 0;
@@ -821,8 +860,10 @@ int gf_hydro_stats(long long* out, int len) {
         nn = c[0];
         ng = c[1];
     }
-    const long long v[6] = {S.last_passes, S.stat_queued, S.stat_queued2, nn, ng, S.tree.n};
-    for (int i = 0; i < len && i < 6; ++i) out[i] = v[i];
+    int items = 0;
+    if (S.state && S.knn_items) GF_HIP(hipMemcpy(&items, S.state + 14, sizeof(int), hipMemcpyDeviceToHost));
+    const long long v[7] = {S.last_passes, S.stat_queued, S.stat_queued2, nn, ng, S.tree.n, items};
+    for (int i = 0; i < len && i < 7; ++i) out[i] = v[i];
     return 0;
 }
 

@@ -32,12 +32,23 @@ struct GfWalkScratch {
     float quad_rel_tol = 0.0f;
     // persistent one-wave blocks of the walk (0 = the default, gf_walk_default_blocks)
     int walk_blocks = 0;
+    // the packed records were made from this tree version, with (1) or
+    // without (0) quadrupole estimates (-1: none made)
+    int prep_version = -1, prep_quad = -1;
+    const GfTree* prep_tree = nullptr;
 };
 
 int gf_walk_reserve(GfWalkScratch& W, int max_groups);
 // The walk's persistent grid when walk_blocks is 0.
 int gf_walk_default_blocks();
 void gf_walk_free(GfWalkScratch& W);
+
+// Packs the walk's records of the tree T for a walk at `precision` (node
+// records level by level, particles (x, y, z, m), and for precision 2 the
+// quadrupoles and their estimates), unless the records of this tree
+// version and kind are already held.  gf_walk calls it; a caller may call
+// it earlier on the walk's stream, beside other work.
+int gf_walk_prepare(const GfTree& T, GfWalkScratch& W, int precision, hipStream_t s);
 
 // Accelerations of the active particles (active: input order, may be null =
 // all) from the tree T, written to acc (n, 3) in input order; inactive rows

@@ -735,13 +735,15 @@ __global__ __launch_bounds__(256) void k_gp4_pack(int n, const double* __restric
     if (t < n) p4[t] = make_double4(tx[t], ty[t], tz[t], tm[t]);
 }
 
-__global__ void k_walk_reset(int* work, int* ovf_count) {
+// The walk's cursors and overflow counts; with records, also the record
+// layout error count that the packing of the records adds to.
+__global__ void k_walk_reset(int* work, int* ovf_count, int records) {
     if (threadIdx.x == 0) {
         work[0] = 0;
         work[1] = 0;
         ovf_count[0] = 0;
         ovf_count[1] = 0;
-        ovf_count[2] = 0;
+        if (records) ovf_count[2] = 0;
     }
 }
 
@@ -916,17 +918,18 @@ void gf_walk_free(GfWalkScratch& W) {
 
 int gf_walk_default_blocks() { return WALK_GRID; }
 
-int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* acc, double eps,
-            int precision, hipStream_t s) {
+int gf_walk_prepare(const GfTree& T, GfWalkScratch& W, int precision, hipStream_t s) {
     if (T.n == 0) return 0;
     if (!T.built) {
         gf_set_error("gravity walk without a built tree");
         return -1;
     }
+    const int quad = precision == GF_FP32_QUAD ? 1 : 0;
+    if (W.prep_tree == &T && W.prep_version == T.version && W.prep_quad == quad) return 0;
     if (gf_walk_reserve(W, T.cap_nodes) || walk_pack_reserve(W, T.cap_nodes, T.n)) return -1;
-    T.want_quad = precision == GF_FP32_QUAD;
-    if (precision == GF_FP32_QUAD && gf_tree_quadrupoles(T, s)) return -1;
-    hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, W.work, W.ovf_count);
+    T.want_quad = quad;
+    if (quad && gf_tree_quadrupoles(T, s)) return -1;
+    hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, W.work, W.ovf_count, 1);
     // the records' order: nodes by depth, each depth in tree order
     const int cn = T.cap_nodes, gb = std::min(2048, gf_div_up(cn, 256));
     uint64_t* rkey = static_cast<uint64_t*>(W.rkey);
@@ -946,6 +949,18 @@ int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* ac
                        W.ovf_count + 2);
     hipLaunchKernelGGL(k_gp4_pack, dim3(gf_div_up(T.n, 256)), dim3(256), 0, s, T.n, T.tx, T.ty,
                        T.tz, T.tm, reinterpret_cast<double4*>(W.p4));
+    GF_LAUNCH_CHECK();
+    W.prep_tree = &T;
+    W.prep_version = T.version;
+    W.prep_quad = quad;
+    return 0;
+}
+
+int gf_walk(const GfTree& T, GfWalkScratch& W, const uint8_t* active, double* acc, double eps,
+            int precision, hipStream_t s) {
+    if (T.n == 0) return 0;
+    if (gf_walk_prepare(T, W, precision, s)) return -1;
+    hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, W.work, W.ovf_count, 0);
     WalkArgs a;
     a.gn = static_cast<const GNode*>(W.gnode);
     a.gq = static_cast<const GQuad*>(W.gquad);

@@ -56,6 +56,11 @@ int gf_sph_merge_members(const GfSph& S);
 // may be null when only neighbours are wanted).
 int gf_sph_setup(GfSph& S, int n, const double* pos, const double* vel, const double* mass,
                  const double* u, hipStream_t s);
+// The same for the first n particles of the arrays the tree T was last
+// built from: the gas tree is T's sorted order restricted to them
+// (gf_tree_build_subset), without a sort of its own.
+int gf_sph_setup_from_tree(GfSph& S, const GfTree& T, int n, const double* pos, const double* vel,
+                           const double* mass, const double* u, hipStream_t s);
 
 // Exact kq nearest neighbours of every gas particle (the particle itself
 // included) by radius iteration: a pass gathers, per particle, every gas

@@ -1727,6 +1727,17 @@ int gf_sph_setup(GfSph& S, int n, const double* pos, const double* vel, const do
     return 0;
 }
 
+int gf_sph_setup_from_tree(GfSph& S, const GfTree& T, int n, const double* pos, const double* vel,
+                           const double* mass, const double* u, hipStream_t s) {
+    if (sph_reserve(S, n, 0)) return -1;
+    if (gf_tree_build_subset(S.tree, T, n, pos, mass, s)) return -1;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_sph_gather, dim3(gf_div_up(n, TB)), dim3(TB), 0, s, n, S.tree.idx, vel, u,
+                       S.vx, S.vy, S.vz, S.u, S.inv);
+    GF_LAUNCH_CHECK();
+    return 0;
+}
+
 static KnnArgs knn_args(GfSph& S) {
     GfTree& T = S.tree;
     KnnArgs a;

@@ -10,6 +10,7 @@ struct GfTree {
     int cap_n = 0;      // particle capacity of the buffers
     int cap_nodes = 0;  // node capacity of the buffers
     int built = 0;      // a tree is held (refit allowed)
+    int version = 0;    // counts builds and refits (the walk's packed records follow it)
     float theta = 0.7f; // opening angle of the held tree (cast to float, nbody.cpp:327)
 
     // per particle
@@ -45,6 +46,15 @@ struct GfTree {
     double *root = nullptr;                        // root centre (3) and half-width
     double *bbox_part = nullptr;
     int *scan_tmp = nullptr, *sort_tmp = nullptr;
+    // keep_sorted: every build also keeps its Morton-sorted (key, input
+    // index), before particles within leaves are put in input order, for
+    // subset trees built from it (gf_tree_build_subset)
+    int keep_sorted = 0;
+    uint64_t* skey = nullptr;
+    uint32_t* sidx = nullptr;
+    // a subset tree's scratch: flags and offsets over the source tree's n
+    int *sub_off = nullptr, *sub_tmp = nullptr;
+    int cap_sub = 0;
 };
 
 // Node capacity that always suffices for n particles: at most n leaves and,
@@ -56,8 +66,18 @@ void gf_tree_free(GfTree& T);
 
 // Builds the tree of pos (n, 3) / mass (n) (device, input order).
 // with_com = false skips the centres of mass (a tree used for neighbour search).
+// sorted_event (may be null) is recorded on s once the sorted keys are kept.
 int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float theta,
-                  hipStream_t s, bool with_com = true);
+                  hipStream_t s, bool with_com = true, hipEvent_t sorted_event = nullptr);
+// Builds S as the tree of the first m particles of the arrays the tree T
+// was last built from (T.keep_sorted set): T's sorted keys restricted to
+// them (a stable compaction, no sort of their own; S's root box is T's),
+// then S's nodes, leaf order, gathered positions and groups, without
+// centres of mass.  pos, mass: the arrays T was built from.  T's build
+// must be complete on the device before S's kernels run (the caller orders
+// the streams).
+int gf_tree_build_subset(GfTree& S, const GfTree& T, int m, const double* pos, const double* mass,
+                         hipStream_t s);
 // Work items of runs of sibling groups (children of one node, consecutive
 // in octant order) of at most cap <= 64 particles in all, every other group
 // an item of its own: items[i] = (first group position, groups, first tree

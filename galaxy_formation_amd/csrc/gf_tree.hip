@@ -895,6 +895,36 @@ __global__ __launch_bounds__(TB) void k_merge_groups(const int* __restrict__ cou
     }
 }
 
+// Subset trees: the positions of the source tree's sorted order that hold
+// one of the first m particles.
+__global__ __launch_bounds__(TB) void k_subset_flag(const uint32_t* __restrict__ sidx, int n, int m,
+                                                    int* __restrict__ flag) {
+    const int p = blockIdx.x * TB + threadIdx.x;
+    if (p < n) flag[p] = int(sidx[p]) < m ? 1 : 0;
+}
+
+// Stable compaction of the subset's (key, input index), by the exclusive
+// prefix `off` of the flags; the root box is the source tree's.
+__global__ __launch_bounds__(TB) void k_subset_scatter(const uint64_t* __restrict__ skey,
+                                                       const uint32_t* __restrict__ sidx,
+                                                       const int* __restrict__ off, int n, int m,
+                                                       const double* __restrict__ root_src,
+                                                       uint64_t* __restrict__ key,
+                                                       uint32_t* __restrict__ idx,
+                                                       double* __restrict__ root) {
+    const int p = blockIdx.x * TB + threadIdx.x;
+    if (p < 4) root[p] = root_src[p];
+    if (p >= n) return;
+    const uint32_t i = sidx[p];
+    if (int(i) < m) {
+        const int d = off[p];
+        if (d < m) {
+            key[d] = skey[p];
+            idx[d] = i;
+        }
+    }
+}
+
 }  // namespace
 
 int gf_tree_node_bound(int n) {
@@ -919,7 +949,13 @@ int gf_tree_reserve(GfTree& T, int n) {
             return -1;
         T.cap_n = cn;
         T.built = 0;
+        if (T.skey) {
+            dfree(T.skey);
+            dfree(T.sidx);
+        }
     }
+    if (T.keep_sorted && !T.skey && (dalloc(T.skey, T.cap_n) || dalloc(T.sidx, T.cap_n)))
+        return -1;
     const int nb = gf_tree_node_bound(T.cap_n);
     if (nb > T.cap_nodes) {
         if (dalloc(T.start, nb) || dalloc(T.count, nb) || dalloc(T.next, nb) ||
@@ -942,7 +978,10 @@ void gf_tree_free(GfTree& T) {
     dfree(T.depth); dfree(T.child8); dfree(T.dlist); dfree(T.dmeta); dfree(T.com); dfree(T.open_r);
     dfree(T.box); dfree(T.ndisp); dfree(T.quad); dfree(T.gflag); dfree(T.groups); dfree(T.counters);
     dfree(T.root); dfree(T.bbox_part); dfree(T.scan_tmp); dfree(T.sort_tmp);
+    dfree(T.skey); dfree(T.sidx); dfree(T.sub_off); dfree(T.sub_tmp);
+    const int keep = T.keep_sorted;
     T = GfTree();
+    T.keep_sorted = keep;
 }
 
 static MomentArgs moment_args(const GfTree& T) {
@@ -1009,8 +1048,11 @@ int gf_tree_quadrupoles(const GfTree& T, hipStream_t s) {
     return 0;
 }
 
+static int build_sorted(GfTree& T, int n, const double* pos, const double* mass, hipStream_t s,
+                        bool with_com);
+
 int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float theta,
-                  hipStream_t s, bool with_com) {
+                  hipStream_t s, bool with_com, hipEvent_t sorted_event) {
     if (gf_tree_reserve(T, n)) return -1;
     T.n = n;
     T.theta = theta;
@@ -1031,6 +1073,19 @@ int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float
         std::swap(T.key, T.key_alt);
         std::swap(T.idx, T.idx_alt);
     }
+    if (T.keep_sorted) {
+        GF_HIP(hipMemcpyAsync(T.skey, T.key, sizeof(uint64_t) * size_t(n), hipMemcpyDeviceToDevice, s));
+        GF_HIP(hipMemcpyAsync(T.sidx, T.idx, sizeof(uint32_t) * size_t(n), hipMemcpyDeviceToDevice, s));
+        if (sorted_event) GF_HIP(hipEventRecord(sorted_event, s));
+    }
+    return build_sorted(T, n, pos, mass, s, with_com);
+}
+
+// The build from the Morton-sorted (key, input index) on: nodes, leaf order,
+// gathered particles, centres of mass (with_com) and groups.
+static int build_sorted(GfTree& T, int n, const double* pos, const double* mass, hipStream_t s,
+                        bool with_com) {
+    const int g = gf_div_up(n, TB);
     hipLaunchKernelGGL(k_lcp, dim3(g), dim3(TB), 0, s, T.key, n, T.lcp);
     hipLaunchKernelGGL(k_chain, dim3(g), dim3(TB), 0, s, T.lcp, n, T.chain, T.d0);
     GF_LAUNCH_CHECK();
@@ -1059,7 +1114,40 @@ int gf_tree_build(GfTree& T, int n, const double* pos, const double* mass, float
                        T.leaf, T.gflag, T.groups);
     GF_LAUNCH_CHECK();
     T.built = 1;
+    ++T.version;
     return 0;
+}
+
+int gf_tree_build_subset(GfTree& S, const GfTree& T, int m, const double* pos, const double* mass,
+                         hipStream_t s) {
+    if (m < 0 || m > T.n) {
+        gf_set_error("subset tree: m = %d must be in [0, %d]", m, T.n);
+        return -1;
+    }
+    if (!T.keep_sorted || !T.skey || !T.built) {
+        gf_set_error("subset tree: the source tree keeps no sorted keys");
+        return -1;
+    }
+    if (gf_tree_reserve(S, m)) return -1;
+    S.n = m;
+    S.theta = T.theta;
+    S.built = 0;
+    S.quad_ok = 0;
+    if (m == 0) return 0;
+    const int n = T.n;
+    if (n + 1 > S.cap_sub) {
+        if (dalloc(S.sub_off, size_t(n) + 1) || dalloc(S.sub_tmp, gf_scan_temp_ints(n + 1)))
+            return -1;
+        S.cap_sub = n + 1;
+    }
+    const int g = gf_div_up(n, TB);
+    hipLaunchKernelGGL(k_subset_flag, dim3(g), dim3(TB), 0, s, T.sidx, n, m, S.sub_off);
+    GF_LAUNCH_CHECK();
+    if (gf_scan_excl(S.sub_off, S.sub_off, n, S.sub_tmp, s)) return -1;
+    hipLaunchKernelGGL(k_subset_scatter, dim3(std::max(g, 1)), dim3(TB), 0, s, T.skey, T.sidx,
+                       S.sub_off, n, m, T.root, S.key, S.idx, S.root);
+    GF_LAUNCH_CHECK();
+    return build_sorted(S, m, pos, mass, s, false);
 }
 
 int gf_tree_refit(GfTree& T, const double* pos, const double* mass, hipStream_t s) {
@@ -1075,5 +1163,6 @@ int gf_tree_refit(GfTree& T, const double* pos, const double* mass, hipStream_t 
                        T.tz, T.tm, T.tdisp);
     GF_LAUNCH_CHECK();
     (void)ng;
+    ++T.version;
     return com_pass(T, s);
 }

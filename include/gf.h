@@ -152,6 +152,13 @@ int gf_gravity_dev(int n, const double* pos, const double* mass, double* acc, do
 int gf_tree_build_dev(int n, const double* pos, const double* mass, double theta, void* stream);
 int gf_walk_dev(int n, const uint8_t* active, double* acc, double eps, int precision,
                 void* stream);
+/* Scheduling, no counterpart in the reference: packs the held tree's walk
+ * records for a walk at `precision` (node records level by level, packed
+ * particles, and for precision 2 the quadrupoles), which gf_walk_dev
+ * otherwise does first.  Called on the walk's stream right after the
+ * build, it moves that work beside other work; the walk then reuses the
+ * records of the same tree build. */
+int gf_walk_prepare_dev(int precision, void* stream);
 /* Device-pointer FP64 direct sum for nt targets. */
 int gf_direct_dev(int n, const double* pos, const double* mass, int nt, const int64_t* tgt,
                   double* acc, double eps, void* stream);
@@ -266,6 +273,16 @@ int gf_hydro_dev(int n, const double* pos, const double* vel, const double* mass
 /* The three stages of gf_hydro_dev, so callers can time them apart. */
 int gf_hydro_setup_dev(int n, const double* pos, const double* vel, const double* mass,
                        const double* u, void* stream);
+/* No counterpart in the reference (sph.py:166 builds its own cKDTree): the
+ * same set-up for a gas that is the first n rows of pos, the array of the
+ * last gf_tree_build_dev, whose Morton order restricted to the gas becomes
+ * the gas tree (no bounding box, keys or sort of its own; the gas tree's
+ * cells are aligned with the gravity tree's).  The stream waits for that
+ * build's sort, which may run on another stream; the next gf_tree_build_dev
+ * must come after this call's kernels.  The neighbours do not depend on the
+ * gas tree. */
+int gf_hydro_setup_from_tree_dev(int n, const double* pos, const double* vel, const double* mass,
+                                 const double* u, void* stream);
 int gf_hydro_knn_dev(int k, const double* r_init, void* stream);
 /* gf_hydro_knn_dev in parts, so that a caller can enqueue other work (the
  * gravity walk on another stream) while the solve runs: _begin enqueues the
@@ -294,7 +311,8 @@ int gf_hydro_neighbours(int64_t* nb, double* dk);
 /* Statistics of the last neighbour solve: out[0] pass launches, [1] groups
  * the fast pass handed to the heap pass (summed over passes), [2] groups
  * handed on to the global-stack pass, [3] gas-tree nodes, [4] gas groups,
- * [5] gas particles (len entries are written, at most 6). */
+ * [5] gas particles, [6] work items of the first pass (merged groups; 0
+ * without a caller's radii) (len entries are written, at most 7). */
 int gf_hydro_stats(long long* out, int len);
 /* Passes each particle of the last solve took (host array, input order; may
  * be NULL) and their total.  Returns the number of pass launches. */

@@ -548,3 +548,38 @@ def test_walk_grid_does_not_change_results(precision):
             assert e.last_interactions() == i0
     finally:
         _lib.check(L.gf_set_walk_blocks(0))
+
+
+def test_walk_records_packed_ahead_equal_packed_by_the_walk():
+    """gf_walk_prepare_dev packs the walk's records right after the build;
+    the walk then reuses them.  Accelerations are bit-identical to a walk
+    that packs its own records, for every precision, for a second walk on
+    the same build, and when the records were packed for another precision
+    (the walk packs them again)."""
+    import torch
+
+    from galaxy_formation_amd import _lib
+    L = _lib.load()
+    dev = torch.device("cuda:0")
+    c = ics.config("cloud10k")
+    pos = torch.from_numpy(np.ascontiguousarray(c.pos)).to(dev)
+    mass = torch.from_numpy(np.ascontiguousarray(c.mass)).to(dev)
+    n = len(c.pos)
+
+    def walk(prec, prepare=None, twice=False):
+        acc = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+        _lib.check(L.gf_tree_build_dev(n, pos.data_ptr(), mass.data_ptr(), 0.5, None))
+        if prepare is not None:
+            _lib.check(L.gf_walk_prepare_dev(prepare, None))
+        _lib.check(L.gf_walk_dev(n, None, acc.data_ptr(), c.eps, prec, None))
+        if twice:
+            _lib.check(L.gf_walk_dev(n, None, acc.data_ptr(), c.eps, prec, None))
+        torch.cuda.synchronize()
+        return acc.cpu().numpy()
+
+    for prec in (0, 1, 2):
+        base = walk(prec)
+        np.testing.assert_array_equal(walk(prec, prepare=prec), base)
+        np.testing.assert_array_equal(walk(prec, prepare=prec, twice=True), base)
+        other = 2 if prec != 2 else 0
+        np.testing.assert_array_equal(walk(prec, prepare=other), base)

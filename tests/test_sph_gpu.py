@@ -420,3 +420,71 @@ def test_neighbours_with_small_candidate_lists(k, cap, first):
     np.testing.assert_array_equal(dk, od[:, -1])
     if cap < 100:
         assert st[1] > 0          # groups handed to the heap pass
+
+
+@pytest.mark.parametrize("first", [None, 1.05])
+def test_gas_tree_from_the_gravity_tree_equals_oracle(first):
+    """The gas tree as the gravity tree's Morton order restricted to the gas
+    (gf_hydro_setup_from_tree_dev: the gas is the first rows of the gravity
+    tree's particles, no sort of its own): neighbours, d_k and h equal brute
+    force, and density and forces the oracle's, from the cell-density start
+    and from a caller's radii."""
+    import torch
+
+    from galaxy_formation_amd import _lib
+    L = _lib.load()
+    c = ics.make_disk(15000, 15000, seed=23)
+    g = c.gas
+    assert (g == np.arange(len(g))).all()
+    n, ng, k = len(c.pos), len(g), 64
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    pos, vel, mass, u = T(c.pos), T(c.vel), T(c.mass), T(c.u)
+    gp = np.ascontiguousarray(c.pos[g])
+    od, onb = O.knn(gp, k)
+    r0 = None if first is None else T(first * od[:, -1])
+    torch.cuda.synchronize()
+    _lib.check(L.gf_tree_build_dev(n, pos.data_ptr(), mass.data_ptr(), 0.5, None))
+    _lib.check(L.gf_hydro_setup_from_tree_dev(ng, pos.data_ptr(), vel.data_ptr(), mass.data_ptr(),
+                                              u.data_ptr(), None))
+    _lib.check(L.gf_hydro_knn_dev(k, None if r0 is None else r0.data_ptr(), None))
+    nb = np.empty((ng, k), np.int64)
+    dk = np.empty(ng)
+    _lib.check(L.gf_hydro_neighbours(nb, _lib.ptr(dk)))
+    np.testing.assert_array_equal(np.sort(nb, axis=1), np.sort(onb, axis=1))
+    np.testing.assert_array_equal(dk, od[:, -1])
+    out = {name: torch.zeros(shape, dtype=torch.float64, device=dev) for name, shape in
+           (("h", ng), ("rho", ng), ("acc", (ng, 3)), ("dudt", ng), ("dt", ng), ("divv", ng))}
+    _lib.check(L.gf_set_sph_precision(0))
+    _lib.check(L.gf_hydro_forces_dev(c.eps, GAMMA, sph.ALPHA, sph.BETA,
+                                     *[out[x].data_ptr() for x in ("h", "rho", "acc", "dudt",
+                                                                   "dt", "divv")], None))
+    _lib.check(L.gf_sync(None))
+    h = O.smoothing_length(od[:, -1], c.eps)
+    orho, oacc, _, _, _ = O.sph_forces(gp, np.ascontiguousarray(c.vel[g]),
+                                       np.ascontiguousarray(c.mass[g]), onb, h,
+                                       np.ascontiguousarray(c.u[g]), GAMMA, sph.ALPHA, sph.BETA)
+    np.testing.assert_array_equal(out["h"].cpu().numpy(), h)
+    np.testing.assert_allclose(out["rho"].cpu().numpy(), orho, rtol=1e-13)
+    np.testing.assert_allclose(out["acc"].cpu().numpy(), oacc, rtol=1e-10,
+                               atol=1e-12 * np.abs(oacc).max())
+
+
+def test_gas_tree_from_the_gravity_tree_refuses_other_positions():
+    """The set-up needs the positions of the last gravity build."""
+    import torch
+
+    from galaxy_formation_amd import _lib
+    L = _lib.load()
+    c = ics.make_cloud(2000, 2000, seed=5)
+    dev = torch.device("cuda:0")
+    pos = torch.from_numpy(np.ascontiguousarray(c.pos)).to(dev)
+    other = pos.clone()
+    mass = torch.from_numpy(np.ascontiguousarray(c.mass)).to(dev)
+    _lib.check(L.gf_tree_build_dev(len(c.pos), pos.data_ptr(), mass.data_ptr(), 0.5, None))
+    with pytest.raises(_lib.GFError):
+        _lib.check(L.gf_hydro_setup_from_tree_dev(2000, other.data_ptr(), None, mass.data_ptr(),
+                                                  None, None))
+    with pytest.raises(_lib.GFError):
+        _lib.check(L.gf_hydro_setup_from_tree_dev(len(c.pos) + 1, pos.data_ptr(), None,
+                                                  mass.data_ptr(), None, None))
