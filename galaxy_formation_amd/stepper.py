@@ -32,7 +32,8 @@ GAMMA, ALPHA, BETA = 5.0 / 3.0, 1.0, 2.0
 
 class ForceEvaluator:
     def __init__(self, n, n_gas, eps, theta=0.5, k=64, precision="fp32", device="cuda:0",
-                 stream=None, quad_relative_tolerance=None, sph_precision=None):
+                 stream=None, quad_relative_tolerance=None, sph_precision=None,
+                 gas_tree="gravity"):
         import torch
         self.torch = torch
         self.L = _lib.load()
@@ -47,6 +48,13 @@ class ForceEvaluator:
         # density and forces: 0 FP64 pair terms, 1 FP32 pair terms with FP64
         # sums (None leaves the library's setting)
         self.sph_prec = sph_precision
+        # the gas tree: "gravity" takes the gravity tree's Morton order
+        # restricted to the gas (the first n_gas rows) when this evaluation
+        # built that tree from the same positions; "own" sorts the gas alone
+        if gas_tree not in ("gravity", "own"):
+            raise ValueError(f"unknown gas tree {gas_tree!r}")
+        self.gas_tree = gas_tree
+        self._built_pos = None      # positions of this evaluation's gravity tree build
         z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.dev)  # noqa: E731
         self.acc_grav = z(n, 3)
         self.acc = z(n, 3)
@@ -70,6 +78,7 @@ class ForceEvaluator:
             chk(L.gf_set_quad_relative_tolerance(float(self.quad_rel)))
         if ev is not None:
             ev[0].record()
+        self._built_pos = pos.data_ptr() if rebuild else None
         chk(L.gf_tree_build_dev(self.n, pos.data_ptr(), mass.data_ptr(), self.theta, self.sp)
             if rebuild else
             L.gf_gravity_dev(self.n, pos.data_ptr(), mass.data_ptr(), self.acc_grav.data_ptr(),
@@ -95,8 +104,14 @@ class ForceEvaluator:
             self.dt_c.fill_(float("inf"))
             self.passes = 0
             return 0
-        chk(L.gf_hydro_setup_dev(ng, pos.data_ptr(), vel.data_ptr(), mass.data_ptr(),
-                                 u.data_ptr(), self.sp))
+        if self.gas_tree == "gravity" and self._built_pos == pos.data_ptr():
+            # the gas tree from the gravity tree built from these positions
+            # (the stream waits for that build's sort)
+            chk(L.gf_hydro_setup_from_tree_dev(ng, pos.data_ptr(), vel.data_ptr(),
+                                               mass.data_ptr(), u.data_ptr(), self.sp))
+        else:
+            chk(L.gf_hydro_setup_dev(ng, pos.data_ptr(), vel.data_ptr(), mass.data_ptr(),
+                                     u.data_ptr(), self.sp))
         if ev is not None:
             ev[3].record()
         r0 = None
@@ -252,7 +267,7 @@ class RungStepper:
 
     def __init__(self, particles, theta=0.5, k=64, precision="fp32", rungs=6, first="plain",
                  model=None, device="cuda:0", refit=False, quad_relative_tolerance=None,
-                 sph_precision=None, overlap=True):
+                 sph_precision=None, overlap=True, gas_tree="gravity"):
         import torch
         self.torch = torch
         dev = torch.device(device)
@@ -270,7 +285,7 @@ class RungStepper:
         self.first, self.model = first, model
         self.F = ForceEvaluator(p.n, self.ng, p.eps, theta, k, precision, device,
                                 quad_relative_tolerance=quad_relative_tolerance,
-                                sph_precision=sph_precision)
+                                sph_precision=sph_precision, gas_tree=gas_tree)
         self.L = self.F.L
         self.phi = torch.zeros(p.n, dtype=torch.float64, device=dev)
         self.rung = torch.zeros(p.n, dtype=torch.int64, device=dev)
