@@ -36,12 +36,14 @@ namespace {
 #ifndef WALK_WAVES
 #define WALK_WAVES 4
 #endif
-// with quadrupoles: a 192-entry particle list and 64 quadrupole entries of
-// 36 B (a wave holds 9.25 KB of LDS with its 4 KB stack); a 112-entry list
-// (8 KB, the monopole walk's footprint) measured 1-2 % slower alone and no
-// faster beside the neighbour pass, which the walk's registers limit anyway
+// with quadrupoles: a 112-entry particle list and 64 quadrupole entries of
+// 36 B, so that a wave holds 8 KB of LDS with its 4 KB stack, and (with the
+// flushes unrolled less, below) at most 104 VGPRs: sixteen waves of the walk
+// then leave a CU room for one block of the neighbour pass, which the
+// overlapped step runs beside it (alone, the walk is 4 % slower than with a
+// 192-entry list and 114 VGPRs; in the overlapped step 3 % faster)
 #ifndef WALK_LCAP_QUAD
-#define WALK_LCAP_QUAD 192
+#define WALK_LCAP_QUAD 112
 #endif
 #ifndef WALK_QCAP
 #define WALK_QCAP 64
@@ -141,15 +143,19 @@ template <>
 struct Acc<false> {
     double sx = 0, sy = 0, sz = 0;      // FP64 sum of the flushes' FP32 partial sums
     float px, py, pz;
+    // U entries a step: 4 in the monopole walk, 2 in the quadrupole walk,
+    // whose register budget is smaller (the entry order of a lane's sum is
+    // the same)
+    template <int U = 4>
     __device__ void flush(const float4* L, int ln, int sl, int S, float eps2) {
         float ax = 0, ay = 0, az = 0;
         int j = sl;
-        for (; j + 3 * S < ln; j += 4 * S) {
-            float4 e[4];
+        for (; j + (U - 1) * S < ln; j += U * S) {
+            float4 e[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) e[u] = L[j + u * S];
+            for (int u = 0; u < U; ++u) e[u] = L[j + u * S];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 const float dx = e[u].x - px, dy = e[u].y - py, dz = e[u].z - pz;
                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
                 const float inv = __builtin_amdgcn_rsqf(r2);
@@ -200,15 +206,7 @@ struct Acc<false> {
     __device__ void flush_quad(const float4* E, const float4* QA, const float* QB, int ln, int sl,
                                int S, float eps2) {
         float ax = 0, ay = 0, az = 0;
-        int j = sl;
-        for (; j + S < ln; j += 2 * S) {
-            const float4 e0 = E[j], qa0 = QA[j];
-            const float4 e1 = E[j + S], qa1 = QA[j + S];
-            const float qb0 = QB[j], qb1 = QB[j + S];
-            quad_term(e0, qa0, qb0, px, py, pz, eps2, ax, ay, az);
-            quad_term(e1, qa1, qb1, px, py, pz, eps2, ax, ay, az);
-        }
-        if (j < ln) quad_term(E[j], QA[j], QB[j], px, py, pz, eps2, ax, ay, az);
+        for (int j = sl; j < ln; j += S) quad_term(E[j], QA[j], QB[j], px, py, pz, eps2, ax, ay, az);
         sx += double(ax);
         sy += double(ay);
         sz += double(az);
@@ -403,8 +401,14 @@ __global__ __launch_bounds__(64, WALK_WAVES) void k_walk(WalkArgs a) {
                 }
             }
             auto eval_list = [&](int len) {
-                if (evla) A.flush(s_list, len, sla, Sa, eps2r);
-                if (evlb) B.flush(s_list, len, slb, Sb, eps2r);
+                if constexpr (F64) {
+                    if (evla) A.flush(s_list, len, sla, Sa, eps2r);
+                    if (evlb) B.flush(s_list, len, slb, Sb, eps2r);
+                } else {
+                    constexpr int U = QUAD ? 2 : 4;
+                    if (evla) A.template flush<U>(s_list, len, sla, Sa, eps2r);
+                    if (evlb) B.template flush<U>(s_list, len, slb, Sb, eps2r);
+                }
             };
             auto eval_quad = [&](int len) {
                 if constexpr (QUAD) {
