@@ -80,6 +80,9 @@ def parse():
                    help="profiling aid: build the workload's tree, then one warm-up and "
                         "three walks of precision PREC (0 FP32, 1 FP64, 2 FP32 quadrupoles), "
                         "nothing else; prints their mean duration")
+    p.add_argument("--walk-blocks-beside-knn", type=int, default=0,
+                   help="persistent blocks of the gravity walk in the overlapped step (0 = the "
+                        "library's default, one per resident slot)")
     p.add_argument("--knn-blocks-beside-walk", type=int, default=512,
                    help="persistent blocks of the neighbour pass while the gravity walk runs "
                         "on the other stream (0 = the library default, every resident slot)")
@@ -338,6 +341,8 @@ class Bench:
         # beside the walk the neighbour pass takes fewer persistent blocks, so
         # that the walk keeps more of each CU; alone it takes them all
         chk(L.gf_set_knn_blocks(self.a.knn_blocks_beside_walk if overlap else 0))
+        # and the walk's persistent grid beside the solve (0 = the library's)
+        chk(L.gf_set_walk_blocks(self.a.walk_blocks_beside_knn if overlap else 0))
 
         def rec(i, st):
             if ev is not None:
@@ -514,33 +519,44 @@ RECORDS_TRIED = ("walk", "build")
 # built alone; the cell-density start of the plain path reads the cells of
 # the tree built alone better (its first radii overshoot less)
 GAS_TREES_TRIED = ("gravity", "own")
+# the walk's persistent grid beside the solve: one block per resident slot
+# (0, the library's 4096), or three quarters of them, which leaves each CU
+# room for a second neighbour-pass block (the quadrupole walk's 8 KB of LDS
+# and 104 VGPRs a wave: 12 waves and two pass blocks fit a CU)
+WALK_BLOCKS_TRIED = (0, 3072)
 
 
 def best_schedule(B, prec, first):
     """The overlapped step timed with each neighbour-pass grid of
-    KNN_BLOCKS_TRIED, each packing of the walk's records of RECORDS_TRIED and
-    each gas tree of GAS_TREES_TRIED (the same steps, the same work); the
-    fastest is the path's time, and every one tried is reported under
-    'schedules'."""
+    KNN_BLOCKS_TRIED, walk grid of WALK_BLOCKS_TRIED, packing of the walk's
+    records of RECORDS_TRIED and gas tree of GAS_TREES_TRIED (the same steps,
+    the same work); every one tried is reported under 'schedules', and the
+    fastest is timed again (its own steps, so that the reported time is not
+    the best of several noisy ones) for the path's time."""
     a = B.a
-    keep = (a.knn_blocks_beside_walk, a.records, a.gas_tree)
+    fields = ("knn_blocks_beside_walk", "walk_blocks_beside_knn", "records", "gas_tree")
+    keep = tuple(getattr(a, f) for f in fields)
     tried, best = {}, None
     for gt in GAS_TREES_TRIED:
         for rec in RECORDS_TRIED:
-            for kb in KNN_BLOCKS_TRIED:
-                a.knn_blocks_beside_walk, a.records, a.gas_tree = kb, rec, gt
-                r = B.timed(prec, first)
-                tried[f"{a.order}, knn blocks {kb or 'all'}, records with the {rec}, "
-                      f"gas tree {gt}"] = r["ms"]
-                if best is None or r["ms"] < best["ms"]:
-                    best = dict(r, knn_blocks_beside_walk=kb, records=rec, gas_tree=gt)
-    best["schedules"] = tried
-    # the steps of the chosen schedule leave their outputs last
-    a.knn_blocks_beside_walk, a.records, a.gas_tree = (best["knn_blocks_beside_walk"],
-                                                        best["records"], best["gas_tree"])
-    B.timed(prec, first, steps=1)
-    a.knn_blocks_beside_walk, a.records, a.gas_tree = keep
-    return best
+            for wb in WALK_BLOCKS_TRIED:
+                for kb in KNN_BLOCKS_TRIED:
+                    cfg = (kb, wb, rec, gt)
+                    for f, v in zip(fields, cfg):
+                        setattr(a, f, v)
+                    r = B.timed(prec, first)
+                    tried[f"{a.order}, knn blocks {kb or 'all'}, walk blocks {wb or 'all'}, "
+                          f"records with the {rec}, gas tree {gt}"] = r["ms"]
+                    if best is None or r["ms"] < best[1]:
+                        best = (cfg, r["ms"])
+    for f, v in zip(fields, best[0]):
+        setattr(a, f, v)
+    # the chosen schedule timed again; its steps leave their outputs last
+    r = B.timed(prec, first)
+    r.update(dict(zip(fields, best[0])), schedules=tried, best_of_schedules_ms=best[1])
+    for f, v in zip(fields, keep):
+        setattr(a, f, v)
+    return r
 
 
 def steploop(a):
