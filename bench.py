@@ -44,7 +44,7 @@ PEAK_FP64 = 78.6e12
 # 0 monopole, 2 quadrupole): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from
 # rocprofv3 --pmc (profiles/); counters cannot be read inside a run, so the
 # measured figures are carried here
-WALK_TRAFFIC_BYTES = {0: 1659382464, 2: 2309621760}
+WALK_TRAFFIC_BYTES = {0: 1655899819, 2: 2263004592}
 # the same for one evaluation's FP32-pair SPH sums on the 1M disk's gas
 # (k_pack32 + k_density32 + k_force32)
 SPH_TRAFFIC_BYTES = 641058784
@@ -194,18 +194,22 @@ def build_alone_only(a):
     chk(L.gf_tree_build_dev(B.n, B.p(B.pos), B.p(B.mass), a.theta, B.sp))
     if a.build_alone == 2:
         chk(L.gf_walk_dev(B.n, None, B.p(B.acc_g[2]), B.c.eps, 2, B.sp))
-    ms = {"gravity tree": [], "gas tree": []}
+    ms = {"gravity tree": [], "gas tree (own sort)": [], "gas tree (from the gravity tree)": []}
     for it in range(6):
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
         e[0].record(B.stream)
         chk(L.gf_tree_build_dev(B.n, B.p(B.pos), B.p(B.mass), a.theta, B.sp))
         e[1].record(B.stream)
         chk(L.gf_hydro_setup_dev(B.ng, B.p(B.pos), B.p(B.vel), B.p(B.mass), B.p(B.u), B.sp))
         e[2].record(B.stream)
+        chk(L.gf_hydro_setup_from_tree_dev(B.ng, B.p(B.pos), B.p(B.vel), B.p(B.mass), B.p(B.u),
+                                           B.sp))
+        e[3].record(B.stream)
         torch.cuda.synchronize()
         if it > 0:
             ms["gravity tree"].append(e[0].elapsed_time(e[1]))
-            ms["gas tree"].append(e[1].elapsed_time(e[2]))
+            ms["gas tree (own sort)"].append(e[1].elapsed_time(e[2]))
+            ms["gas tree (from the gravity tree)"].append(e[2].elapsed_time(e[3]))
     print(f"builds alone, precision {a.build_alone}, {a.workload}: " +
           ", ".join(f"{k} {np.mean(v):.3f} ms" for k, v in ms.items()) +
           " (mean of 5 after one warm-up)", flush=True)
