@@ -509,11 +509,12 @@ class Bench:
 
 
 # persistent blocks of the neighbour pass beside the walk that each path is
-# timed with: 512 leaves room on each CU for the walk's waves (the monopole
-# walk's 8 KB of LDS per wave then lets a pass block stay beside 16 of
-# them), 0 takes every resident slot (the quadrupole walk's 10 KB per wave
-# fills a CU, so the pass runs before it instead)
+# timed with: 512 leaves room on each CU for the walk's waves (the walks'
+# 8 KB of LDS and at most 98 VGPRs per wave then let a pass block stay
+# beside 16 of them), 0 takes every resident slot
 KNN_BLOCKS_TRIED = (512, 0)
+# rounds of the paired learned / fixed-margin comparison of the solve
+AB_ROUNDS = 8
 # where the walk's records are packed: with the walk (after the neighbour
 # passes have started), or right after the tree build (beside the gas
 # pipeline); the plain path's walk runs beside its neighbour passes and
@@ -526,7 +527,7 @@ GAS_TREES_TRIED = ("gravity", "own")
 # the walk's persistent grid beside the solve: one block per resident slot
 # (0, the library's 4096), or three quarters of them, which leaves each CU
 # room for a second neighbour-pass block (the quadrupole walk's 8 KB of LDS
-# and 104 VGPRs a wave: 12 waves and two pass blocks fit a CU)
+# and 98 VGPRs a wave: 12 waves and two pass blocks fit a CU)
 WALK_BLOCKS_TRIED = (0, 3072)
 
 
@@ -701,6 +702,20 @@ def main():
         r = B.timed(0, first, steps=3, overlap=False)
         ab[name] = {k: r[k] for k in ("knn_ms", "passes", "mean_passes_per_particle")}
         ab[name]["h_equal_plain"] = bool(np.array_equal(B.out[0]["h"].cpu().numpy(), h_plain))
+    # the learned start against the fixed margin, alternated step by step
+    # (serial steps, one timed after the warm-up each), so that a drift of
+    # the box's speed falls on both alike: the mean and standard error of
+    # the paired differences of the solve's time
+    fixed_r = (B.dk_prev * math.exp(model.margin_fixed)).contiguous()
+    diffs = []
+    for _ in range(AB_ROUNDS):
+        t_learned = B.timed(0, ("learned", model), steps=1, overlap=False)["knn_ms"]
+        t_fixed = B.timed(0, ("radius", fixed_r), steps=1, overlap=False)["knn_ms"]
+        diffs.append(t_fixed - t_learned)
+    ab["paired_fixed_minus_learned_ms"] = {
+        "rounds": AB_ROUNDS, "mean": float(np.mean(diffs)),
+        "stderr": float(np.std(diffs, ddof=1) / math.sqrt(len(diffs))),
+        "min": float(np.min(diffs)), "max": float(np.max(diffs))}
 
     # accuracy: gravity on a seeded 2,000-particle subset vs the FP64 direct sum
     sub = np.sort(np.random.default_rng(0).choice(n, min(n, 2000), replace=False))
