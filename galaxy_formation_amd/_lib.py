@@ -48,6 +48,7 @@ SIGNATURES = {
     "gf_last_interactions": (_cll, []),
     "gf_accel_direct_subset": (_ci, [_ci, _dbl, _dbl, _ci, _vp, _dbl, _cd]),
     "gf_tree_info": (_ci, [_vp, _vp, _vp]),
+    "gf_lookback_timeouts": (_ci, []),
     "gf_tree_export": (_ci, [_ci] + [_vp] * 10),
     "gf_gravity_dev": (_ci, [_ci, _vp, _vp, _vp, _cd, _cd, _vp, _ci, _ci, _vp]),
     "gf_tree_build_dev": (_ci, [_ci, _vp, _vp, _cd, _vp]),

@@ -15,6 +15,7 @@
 
 #include "gf_capi.h"
 #include "gf_gravity.h"
+#include "gf_prim.h"
 #include "gf_render.h"
 #include "gf_step.h"
 #include "gf_xfer.h"
@@ -280,6 +281,16 @@ int gf_tree_info(int* nodes, int* groups, int* deep_cells) {
     if (groups) *groups = c[1];
     if (deep_cells) *deep_cells = c[2];
     return 0;
+}
+
+int gf_lookback_timeouts(void) {
+    if (gf_ctx_init()) return -1;
+    const unsigned int v = gf_prim_lookback_timeouts();
+    if (v == ~0u) {
+        gf_set_error("gf_lookback_timeouts: the device counter could not be read");
+        return -1;
+    }
+    return int(v);
 }
 
 int gf_tree_export(int cap, int* start, int* count, int* next, int* leaf, int* depth,

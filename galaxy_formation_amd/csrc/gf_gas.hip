@@ -407,7 +407,11 @@ int ensure_state(GfGas& G) {
 // Scan scratch for n flags: flag (n + 1) and the scan partials.
 int ensure_flags(GfGas& G, int n) {
     if (grow(G.flag, G.cap_flag, size_t(n) + 1)) return -1;
-    return grow(G.scan_tmp, G.cap_scan, gf_scan_temp_ints(n + 1));
+    const size_t need = gf_scan_temp_ints(n + 1);
+    if (need <= G.cap_scan && G.scan_tmp) return 0;
+    // a new scan scratch starts with its look-back state cleared
+    if (grow(G.scan_tmp, G.cap_scan, need)) return -1;
+    return gf_scan_temp_clear(G.scan_tmp, n + 1);
 }
 
 // Reads and clears the error counter at state[2].

@@ -2,14 +2,22 @@
 #pragma once
 #include "gf_common.h"
 
-// Ints of temporary storage gf_scan_excl needs for n inputs.
+// Ints of temporary storage gf_scan_excl needs for n inputs.  The storage
+// holds the single-pass scan's state across calls: clear it once after
+// allocating it (gf_scan_temp_clear, with the n it was sized for), and let
+// only one stream at a time scan with it.
 size_t gf_scan_temp_ints(int n);
+int gf_scan_temp_clear(int* temp, int n);
 // out[i] = in[0] + ... + in[i-1] for i in [0, n]; out has n + 1 entries
-// (out[n] is the total).  in == out is allowed.  With len (device) given,
-// only the first m = min(n, *len) inputs are scanned: out[0..m] is written,
-// out past m is not.
+// (out[n] is the total).  in == out is allowed; both 16-byte aligned.  With
+// len (device) given, only the first m = min(n, *len) inputs are scanned:
+// out[0..m] is written, out past m is not.  One launch: one block up to
+// 65,536 inputs, else a single pass with decoupled look-back.
 int gf_scan_excl(const int* in, int* out, int n, int* temp, hipStream_t s,
                  const int* len = nullptr);
+// Look-back waits abandoned since the library was loaded (0 unless a
+// single-pass scan went wrong).
+unsigned int gf_prim_lookback_timeouts();
 
 // Ints of temporary storage gf_radix_sort_pairs needs for n pairs.
 size_t gf_sort_temp_ints(int n);

@@ -945,6 +945,7 @@ int gf_tree_reserve(GfTree& T, int n) {
             dalloc(T.tx, cn) || dalloc(T.ty, cn) || dalloc(T.tz, cn) || dalloc(T.tm, cn) ||
             dalloc(T.tdisp, cn) || dalloc(T.xb, 3 * size_t(cn)) ||
             dalloc(T.scan_tmp, gf_scan_temp_ints(std::max(cn, gf_tree_node_bound(cn)) + 1)) ||
+            gf_scan_temp_clear(T.scan_tmp, std::max(cn, gf_tree_node_bound(cn)) + 1) ||
             dalloc(T.sort_tmp, gf_sort_temp_ints(cn)))
             return -1;
         T.cap_n = cn;
@@ -1136,7 +1137,8 @@ int gf_tree_build_subset(GfTree& S, const GfTree& T, int m, const double* pos, c
     if (m == 0) return 0;
     const int n = T.n;
     if (n + 1 > S.cap_sub) {
-        if (dalloc(S.sub_off, size_t(n) + 1) || dalloc(S.sub_tmp, gf_scan_temp_ints(n + 1)))
+        if (dalloc(S.sub_off, size_t(n) + 1) || dalloc(S.sub_tmp, gf_scan_temp_ints(n + 1)) ||
+            gf_scan_temp_clear(S.sub_tmp, n + 1))
             return -1;
         S.cap_sub = n + 1;
     }

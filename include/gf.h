@@ -135,6 +135,10 @@ int gf_accel_direct_subset(int n, const double* pos, const double* mass, int nt,
 
 /* Node, group and key-depth-cut cell counts of the held tree. */
 int gf_tree_info(int* nodes, int* groups, int* deep_cells);
+/* Diagnostics, no counterpart in the reference: the number of look-back
+ * waits of the single-pass scan (tree builds, gas compactions) that gave up
+ * since the library was loaded; 0 unless a scan went wrong. */
+int gf_lookback_timeouts(void);
 /* The held tree in the layout of cpp/nbody.cpp:38-45 (DFS order): returns the
  * node count (cap = room in the node arrays); any pointer may be NULL.
  * com4 = (cx, cy, cz, m) FP64, box4 = (gx, gy, gz, half) FP64, idx = input

@@ -86,6 +86,27 @@ def test_tree_topology_hash_equals_reference(eng, golden, name):
     assert ics.sha256(t["groups"]) == str(g["groups_sha"])
 
 
+def test_single_pass_scans_repeat_and_never_give_up(eng):
+    """The chain and group scans of a 200k build exceed one block's 65,536
+    entries and run as one launch with decoupled look-back, whose state
+    carries over from launch to launch (an epoch per launch, never cleared).
+    Builds of other sizes in between, then the same build again, give the
+    same tree, and no look-back wait was abandoned."""
+    from galaxy_formation_amd import _lib
+    L = _lib.load()
+    c = ics.config("cloud200k")
+    run(eng, c.pos, c.mass, 0.5, c.eps, "fp64")
+    first = eng.tree_export()
+    for n in (70000, 150000, 120001):
+        run(eng, np.ascontiguousarray(c.pos[:n]), np.ascontiguousarray(c.mass[:n]), 0.5, c.eps,
+            "fp64")
+    run(eng, c.pos, c.mass, 0.5, c.eps, "fp64")
+    again = eng.tree_export()
+    for k in ("start", "count", "next", "leaf", "idx", "groups"):
+        np.testing.assert_array_equal(again[k], first[k], err_msg=k)
+    assert L.gf_lookback_timeouts() == 0
+
+
 def test_tree_fp64_centres_equal_oracle(eng):
     """FP64 centres of mass and opening radii are the oracle's, bit for bit
     (same summation order, no contraction)."""
