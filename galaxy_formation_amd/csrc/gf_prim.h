@@ -16,8 +16,10 @@ int gf_scan_temp_clear(int* temp, int n);
 int gf_scan_excl(const int* in, int* out, int n, int* temp, hipStream_t s,
                  const int* len = nullptr);
 // Look-back waits abandoned since the library was loaded (0 unless a
-// single-pass scan went wrong).
+// single-pass launch went wrong), and the device address of that counter on
+// the current device, which single-pass launches (gf_lookback.h) take.
 unsigned int gf_prim_lookback_timeouts();
+unsigned int* gf_prim_timeout_counter();
 
 // Ints of temporary storage gf_radix_sort_pairs needs for n pairs.
 size_t gf_sort_temp_ints(int n);

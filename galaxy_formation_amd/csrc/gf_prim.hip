@@ -2,9 +2,11 @@
 // of (uint64 key, uint32 value) pairs.  Both are written for wave64: ranks
 // inside a wave come from ballots, block-level offsets from LDS.
 #include "gf_prim.h"
+#include "gf_lookback.h"
 
-// look-back waits that gave up (a diagnostic: stays 0 unless a tile's
-// predecessor never published, which the ticket order rules out)
+// look-back waits that gave up, in every single-pass launch of the library
+// (a diagnostic: stays 0 unless a tile's predecessor never published, which
+// the ticket order rules out)
 __device__ unsigned int g_lookback_timeouts = 0;
 
 namespace {
@@ -83,152 +85,27 @@ __global__ __launch_bounds__(1024) void k_scan_one(const int* in, int n,
 }
 
 // ------------------------------------------------- single-pass scan
-// A scan of more than SCAN_ONE_MAX inputs in one launch, by decoupled
-// look-back (Merrill and Garland, "Single-pass parallel prefix scan with
-// decoupled look-back", 2016), written for this part:
-//  * a block takes its tile from a ticket (an atomic counter), so it only
-//    ever waits for tiles whose blocks already run, whatever order the
-//    blocks are dispatched in;
-//  * it publishes its tile's total at once (an aggregate), then one wave
-//    looks back over the status words of the 64 tiles before it at a time:
-//    the nearest inclusive prefix, plus the aggregates after it, is the
-//    tile's exclusive prefix, which it then publishes as an inclusive one;
-//  * a status word is (epoch << 34 | state << 32 | value), state 1 an
-//    aggregate, 2 an inclusive prefix; a word of another epoch reads as not
-//    yet published, so the words are never cleared;
-//  * the last block to finish starts the next launch's epoch at ticket 0.
-// Status words are read and written with device-scope atomics, which reach
-// past the L2 of one XCD.  The state starts at zero (gf_scan_temp_clear).
-struct LbCtl {
-    unsigned long long ticket;     // epoch << 32 | next tile
-    unsigned long long finished;   // blocks done with the state
-};
-constexpr int LB_STATE_INTS = 4;   // sizeof(LbCtl) / sizeof(int)
-constexpr int LB_SPIN_MAX = 1 << 20;
-
-__device__ __forceinline__ unsigned long long lb_load(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void lb_store(unsigned long long* p, unsigned long long v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// The sum of the values of the tiles before `tile` (tag: this launch's
-// epoch << 34).  Called by a whole wave, every lane returns it.  Lane l reads
-// the word of tile j0 - l; lanes past tile 0 read as a prefix of 0.  The
-// window is used when every tile from the nearest one with a prefix (lane
-// p) up to this tile has published: the values of lanes 0..p are summed,
-// and without a prefix in the window the walk moves 64 tiles back.  Every
-// tile publishes its aggregate before it looks back, and tile 0 publishes
-// its prefix at once, so the waits end; a wait of LB_SPIN_MAX polls is
-// counted in g_lookback_timeouts and abandoned.
-__device__ int lb_exclusive(const unsigned long long* status, int tile, unsigned long long tag) {
-    const int lane = __lane_id();
-    int sum = 0;
-    int j0 = tile - 1;
-    int spin = 0;
-    while (j0 >= 0) {
-        const int j = j0 - lane;
-        int st = 2, val = 0;
-        if (j >= 0) {
-            const unsigned long long v = lb_load(status + j);
-            st = (v >> 34) == (tag >> 34) ? int((v >> 32) & 3u) : 0;
-            val = int(uint32_t(v));
-        }
-        const uint64_t pre = __ballot(st == 2);
-        const uint64_t ready = __ballot(st != 0);
-        const int p = pre ? __ffsll((unsigned long long)pre) - 1 : 63;
-        const uint64_t need = p == 63 ? ~0ull : (2ull << p) - 1;
-        if ((ready & need) != need) {
-            if (++spin > LB_SPIN_MAX) {
-                if (lane == 0) atomicAdd(&g_lookback_timeouts, 1u);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-            continue;
-        }
-        sum += gf_wave_sum_i(lane <= p ? val : 0);
-        if (pre) break;
-        j0 -= 64;
+// A scan past SCAN_ONE_MAX inputs is one launch of gf_k_lookback: each
+// thread loads its 4 inputs as one 16-byte load and stores its 4 prefixes
+// as one.  in and out may be the same array: a thread reads its items
+// before it writes them, and no other block reads them.
+struct ScanOp {
+    const int* in;
+    int* out;
+    __device__ void values(int i, int n, int (&v)[4]) const {
+        const int4 x = scan_one_load(in, n, i);
+        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
     }
-    return sum;
-}
-
-// Ends a block's use of the look-back state: the last of the launch's
-// `blocks` blocks that take part starts the next epoch (ticket 0).  Called
-// by one thread, after the block's look-back.
-__device__ __forceinline__ void lb_finish(LbCtl* ctl, uint32_t epoch, int blocks) {
-    if (atomicAdd(&ctl->finished, 1ull) == (unsigned long long)(blocks - 1)) {
-        __hip_atomic_store(&ctl->finished, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        atomicExch(&ctl->ticket, (unsigned long long)(epoch + 1u) << 32);
-    }
-}
-
-// One tile of 4096 inputs per block of 1024 threads (4 a thread, 16-byte
-// loads); in and out 16-byte aligned, and may be the same array (a thread
-// reads its items before it writes them, and no other block reads them).
-constexpr int LS_THREADS = 1024;
-constexpr int LS_TILE = 4 * LS_THREADS;
-
-__global__ __launch_bounds__(LS_THREADS) void k_scan_lookback(const int* in, int n,
-                                                              const int* __restrict__ len,
-                                                              int* out, LbCtl* ctl,
-                                                              unsigned long long* status) {
-    __shared__ unsigned long long s_ticket;
-    __shared__ int s_wpre[LS_THREADS / 64];
-    __shared__ int s_base;
-    const int t = threadIdx.x, lane = __lane_id(), w = t >> 6;
-    n = scan_len(n, len);
-    // the grid covers the largest length; only the first ntiles blocks take
-    // part (each takes one of the tickets 0 .. ntiles - 1), the rest leave
-    const int ntiles = (n + LS_TILE - 1) / LS_TILE;
-    if (ntiles == 0) {
-        if (blockIdx.x == 0 && t == 0) out[0] = 0;   // nothing to scan: the total
-        return;
-    }
-    if (int(blockIdx.x) >= ntiles) return;
-    if (t == 0) s_ticket = atomicAdd(&ctl->ticket, 1ull);
-    __syncthreads();
-    const uint32_t epoch = uint32_t(s_ticket >> 32);
-    const int tile = int(uint32_t(s_ticket));
-    {
-        const int i = tile * LS_TILE + 4 * t;
-        const int4 v = scan_one_load(in, n, i);
-        const int sum = v.x + v.y + v.z + v.w;
-        const int inc = gf_wave_incl_scan(sum);
-        if (lane == 63) s_wpre[w] = inc;
-        __syncthreads();
-        if (w == 0) {
-            const int wt = lane < LS_THREADS / 64 ? s_wpre[lane] : 0;
-            const int winc = gf_wave_incl_scan(wt);
-            const int total = __builtin_amdgcn_readlane(winc, LS_THREADS / 64 - 1);
-            if (lane < LS_THREADS / 64) s_wpre[lane] = winc - wt;
-            const unsigned long long tag = (unsigned long long)(epoch & 0x3fffffffu) << 34;
-            if (lane == 0)
-                lb_store(status + tile, tag | ((tile == 0 ? 2ull : 1ull) << 32) | uint32_t(total));
-            int excl = 0;
-            if (tile > 0) {
-                excl = lb_exclusive(status, tile, tag);
-                if (lane == 0) lb_store(status + tile, tag | (2ull << 32) | uint32_t(excl + total));
-            }
-            if (lane == 0) {
-                s_base = excl;
-                if (tile == ntiles - 1) out[n] = excl + total;
-            }
-        }
-        __syncthreads();
-        const int b = s_base + s_wpre[w] + inc - sum;
-        const int4 o = make_int4(b, b + v.x, b + v.x + v.y, b + v.x + v.y + v.z);
+    __device__ void put(int i, int n, const int (&)[4], const int (&o)[4]) const {
         if (i + 3 < n) {
-            *reinterpret_cast<int4*>(out + i) = o;
+            *reinterpret_cast<int4*>(out + i) = make_int4(o[0], o[1], o[2], o[3]);
         } else {
-            if (i < n) out[i] = o.x;
-            if (i + 1 < n) out[i + 1] = o.y;
-            if (i + 2 < n) out[i + 2] = o.z;
+            for (int q = 0; q < 3; ++q)
+                if (i + q < n) out[i + q] = o[q];
         }
     }
-    if (t == 0) lb_finish(ctl, epoch, ntiles);
-}
+    __device__ void total(int n, int s) const { out[n] = s; }
+};
 
 // ------------------------------------------------------------------ radix sort
 // A block sorts a tile of 4096 keys per pass with 16 waves of 4 keys per lane
@@ -389,12 +266,26 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
 
 // Scan scratch: the look-back state (control words, then one status word
 // per tile).
-size_t gf_scan_temp_ints(int n) { return LB_STATE_INTS + 2 * size_t(gf_div_up(n, LS_TILE)); }
+size_t gf_scan_temp_ints(int n) { return gf_lb_state_ints(n); }
 
 int gf_scan_temp_clear(int* temp, int n) {
     GF_HIP(hipMemsetAsync(temp, 0, sizeof(int) * gf_scan_temp_ints(n), nullptr));
     GF_HIP(hipStreamSynchronize(nullptr));
     return 0;
+}
+
+unsigned int* gf_prim_timeout_counter() {
+    static int dev_cached = -1;
+    static unsigned int* ptr = nullptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    if (dev != dev_cached) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(g_lookback_timeouts)) != hipSuccess) return nullptr;
+        ptr = static_cast<unsigned int*>(p);
+        dev_cached = dev;
+    }
+    return ptr;
 }
 
 unsigned int gf_prim_lookback_timeouts() {
@@ -417,9 +308,14 @@ int gf_scan_excl(const int* in, int* out, int n, int* temp, hipStream_t s, const
     if (n <= SCAN_ONE_MAX) {
         hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, s, in, n, len, out);
     } else {
-        hipLaunchKernelGGL(k_scan_lookback, dim3(gf_div_up(n, LS_TILE)), dim3(LS_THREADS), 0, s,
-                           in, n, len, out, reinterpret_cast<LbCtl*>(temp),
-                           reinterpret_cast<unsigned long long*>(temp + LB_STATE_INTS));
+        unsigned int* timeouts = gf_prim_timeout_counter();
+        if (!timeouts) {
+            gf_set_error("gf_scan_excl: no device address for the look-back counter");
+            return -1;
+        }
+        hipLaunchKernelGGL(gf_k_lookback<ScanOp>, dim3(gf_lb_blocks(n)), dim3(GF_LB_THREADS), 0,
+                           s, ScanOp{in, out}, n, len, reinterpret_cast<GfLbCtl*>(temp),
+                           reinterpret_cast<unsigned long long*>(temp + GF_LB_CTL_INTS), timeouts);
     }
     GF_LAUNCH_CHECK();
     return 0;

@@ -37,7 +37,6 @@ struct GfTree {
     mutable int quad_ok = 0;                       // quad matches the held centres of mass
     mutable int want_quad = 0;                     // the last walk used quadrupoles: the next
                                                    // centre-of-mass pass computes them too
-    int *gflag = nullptr;                          // nodes + 1, then group offsets
     int *groups = nullptr;                         // group nodes, DFS order
 
     // scalars on the device: [0] nodes, [1] groups, [2] cells cut at the key
@@ -52,8 +51,9 @@ struct GfTree {
     int keep_sorted = 0;
     uint64_t* skey = nullptr;
     uint32_t* sidx = nullptr;
-    // a subset tree's scratch: flags and offsets over the source tree's n
-    int *sub_off = nullptr, *sub_tmp = nullptr;
+    // a subset tree's scratch: the compaction's look-back state over the
+    // source tree's n
+    int* sub_tmp = nullptr;
     int cap_sub = 0;
 };
 

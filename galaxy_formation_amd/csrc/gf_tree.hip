@@ -19,6 +19,7 @@
 // Centres of mass are summed bottom-up in FP64, children in octant order,
 // in two launches: subtrees of at most 256 nodes, each finished by one block
 // in LDS, then the nodes above them.
+#include "gf_lookback.h"
 #include "gf_prim.h"
 #include "gf_tree.h"
 
@@ -115,53 +116,102 @@ __global__ __launch_bounds__(TB) void k_lcp(const uint64_t* __restrict__ key, in
     lcp[j] = j == 0 ? int8_t(-1) : lcp_levels(key[j - 1], key[j]);
 }
 
-// Number of nodes that start at sorted position j (a chain of nested cells of
-// depths d0 .. d_last, the last one a leaf), and d0.
-__global__ __launch_bounds__(TB) void k_chain(const int8_t* __restrict__ lcp, int n,
-                                              int* __restrict__ chain, int8_t* __restrict__ d0o) {
-    const int j = blockIdx.x * TB + threadIdx.x;
-    if (j >= n) return;
-    const int L = lcp[j];
+// Number of nodes that start at sorted position j = i + Q < n (a chain of
+// nested cells of depths d0 .. d_last, the last one a leaf; d0 = lcp[j] + 1),
+// from w[k] = lcp[i - 8 + k], k in [0, 20): the entries this reads all lie
+// in [0, n).
+template <int Q>
+__device__ __forceinline__ int chain_count(const int (&w)[20], int n, int j) {
+    const int L = w[8 + Q];
     const int d0 = L + 1;
     int c = 0;
     bool node = (j == 0);
     if (j > 0 && L < GF_KEY_LEVELS) {
-        // the parent cell (depth L) starts at ip, the last position before j
-        // with lcp < L; it is internal iff it holds more than 8 particles
-        int ip = -1;
-        for (int t = 1; t <= GF_LEAF_SIZE; ++t) {
-            const int i = j - t;
-            if (i < 0) break;
-            if (lcp[i] < L) { ip = i; break; }
-        }
-        if (ip < 0) {
+        // the parent cell (depth L) starts at ip = j + ipo, the last position
+        // before j with lcp < L; it is internal iff it holds more than 8
+        int ipo = 0;
+#pragma unroll
+        for (int t = 1; t <= GF_LEAF_SIZE; ++t)
+            if (ipo == 0 && j - t >= 0 && w[8 + Q - t] < L) ipo = -t;
+        if (ipo == 0) {
             node = true;                          // parent spans >= 10 positions
         } else {
-            node = ip + GF_LEAF_SIZE <= n - 1;
-            for (int k = j + 1; node && k <= ip + GF_LEAF_SIZE; ++k) node = lcp[k] >= L;
+            node = j + ipo + GF_LEAF_SIZE <= n - 1;
+#pragma unroll
+            for (int u = 1; u < GF_LEAF_SIZE; ++u)
+                if (node && u <= ipo + GF_LEAF_SIZE) node = w[8 + Q + u] >= L;
         }
     }
     if (node) {
         int m8 = -1;                              // cells (d, j) with d <= m8 hold > 8
         if (j + GF_LEAF_SIZE <= n - 1) {
             m8 = GF_KEY_LEVELS;
-            for (int k = j + 1; k <= j + GF_LEAF_SIZE; ++k) m8 = min(m8, int(lcp[k]));
+#pragma unroll
+            for (int u = 1; u <= GF_LEAF_SIZE; ++u) m8 = min(m8, w[8 + Q + u]);
         }
         int dl = m8 >= d0 ? m8 + 1 : d0;
         dl = min(dl, GF_KEY_LEVELS);
         c = dl - d0 + 1;
     }
-    chain[j] = c;
-    d0o[j] = int8_t(d0);
+    return c;
 }
 
-__global__ void k_copy_count(const int* __restrict__ base, int n, int* __restrict__ counters) {
-    if (threadIdx.x == 0) {
-        counters[0] = base[n];
+// Node bases in one single-pass launch: chain[j] = the nodes that start
+// before sorted position j (chain[n] = the node count, also counters[0];
+// counters[1] and [2] restart at 0).
+struct ChainOp {
+    const int8_t* lcp;
+    int* chain;
+    int8_t* d0o;
+    int* counters;
+    // a thread's 4 positions read lcp[i - 8, i + 12): five aligned 4-byte
+    // loads inside the array, else byte loads of the positions in [0, n)
+    __device__ void values(int i, int n, int (&v)[4]) const {
+        int w[20];
+        if (i >= 8 && i + 12 <= n) {
+            const int* src = reinterpret_cast<const int*>(lcp + i - 8);
+#pragma unroll
+            for (int d = 0; d < 5; ++d) {
+                const int x = src[d];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) w[4 * d + b] = int(int8_t(uint32_t(x) >> (8 * b)));
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 20; ++k) {
+                const int p = i - 8 + k;
+                w[k] = p >= 0 && p < n ? int(lcp[p]) : 0;
+            }
+        }
+        v[0] = i < n ? chain_count<0>(w, n, i) : 0;
+        v[1] = i + 1 < n ? chain_count<1>(w, n, i + 1) : 0;
+        v[2] = i + 2 < n ? chain_count<2>(w, n, i + 2) : 0;
+        v[3] = i + 3 < n ? chain_count<3>(w, n, i + 3) : 0;
+        if (i + 3 < n) {
+            const uint32_t d = (uint32_t(uint8_t(w[8] + 1))) | (uint32_t(uint8_t(w[9] + 1)) << 8) |
+                               (uint32_t(uint8_t(w[10] + 1)) << 16) |
+                               (uint32_t(uint8_t(w[11] + 1)) << 24);
+            *reinterpret_cast<uint32_t*>(d0o + i) = d;
+        } else {
+            for (int q = 0; q < 3; ++q)
+                if (i + q < n) d0o[i + q] = int8_t(w[8 + q] + 1);
+        }
+    }
+    __device__ void put(int i, int n, const int (&)[4], const int (&o)[4]) const {
+        if (i + 3 < n) {
+            *reinterpret_cast<int4*>(chain + i) = make_int4(o[0], o[1], o[2], o[3]);
+        } else {
+            for (int q = 0; q < 3; ++q)
+                if (i + q < n) chain[i + q] = o[q];
+        }
+    }
+    __device__ void total(int n, int s) const {
+        chain[n] = s;
+        counters[0] = s;
         counters[1] = 0;
         counters[2] = 0;
     }
-}
+};
 
 __global__ __launch_bounds__(TB) void k_node_init(const int* __restrict__ counters,
                                                   int* __restrict__ child8,
@@ -783,35 +833,34 @@ __global__ __launch_bounds__(MOMENT_TOP_TB) void k_moments_top(MomentArgs a,
 
 // Groups (nbody.cpp:234-243): the outermost nodes holding <= 64 particles, or
 // leaves, i.e. those whose parent holds more than 64.
-// flag has `len` entries (>= node count); entries past the node count are 0.
-__global__ __launch_bounds__(TB) void k_group_flag(const int* __restrict__ counters,
-                                                   const int* __restrict__ count,
-                                                   const int* __restrict__ parent,
-                                                   const int8_t* __restrict__ leaf, int len,
-                                                   int* __restrict__ flag) {
-    const int nn = counters[0];
-    for (int id = blockIdx.x * TB + threadIdx.x; id < len; id += gridDim.x * TB) {
-        bool g = false;
-        if (id < nn) {
-            const int p = parent[id];
-            g = (count[id] <= GF_GROUP_SIZE || leaf[id]) && (p < 0 || count[p] > GF_GROUP_SIZE);
+// in one single-pass launch over the node ids: the group nodes in DFS
+// order to groups[], their number to counters[1]
+struct GroupOp {
+    const int* count;
+    const int* parent;
+    const int8_t* leaf;
+    int* groups;
+    int* counters;
+    __device__ void values(int i, int n, int (&v)[4]) const {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int id = i + q;
+            bool g = false;
+            if (id < n) {
+                const int p = parent[id];
+                g = (count[id] <= GF_GROUP_SIZE || leaf[id]) &&
+                    (p < 0 || count[p] > GF_GROUP_SIZE);
+            }
+            v[q] = g;
         }
-        flag[id] = g;
     }
-}
-
-__global__ __launch_bounds__(TB) void k_group_compact(int* __restrict__ counters,
-                                                      const int* __restrict__ count,
-                                                      const int* __restrict__ parent,
-                                                      const int8_t* __restrict__ leaf,
-                                                      const int* __restrict__ off,
-                                                      int* __restrict__ groups) {
-    const int nn = counters[0];
-    for (int id = blockIdx.x * TB + threadIdx.x; id < nn; id += gridDim.x * TB) {
-        if (off[id + 1] != off[id]) groups[off[id]] = id;
+    __device__ void put(int i, int n, const int (&v)[4], const int (&o)[4]) const {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (i + q < n && v[q]) groups[o[q]] = i + q;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[1] = off[nn];
-}
+    __device__ void total(int, int s) const { counters[1] = s; }
+};
 
 template <typename T>
 int dalloc(T*& p, size_t count) {
@@ -828,6 +877,22 @@ void dfree(T*& p) {
 }
 
 int node_grid(int cap_nodes) { return max(1, min(2048, gf_div_up(cap_nodes, TB))); }
+
+// One single-pass launch of op over at most n_max items (len: the device
+// count that bounds them, or null) with the look-back state in temp.
+template <class Op>
+int lookback_launch(const Op& op, int n_max, const int* len, int* temp, hipStream_t s) {
+    unsigned int* timeouts = gf_prim_timeout_counter();
+    if (!timeouts) {
+        gf_set_error("tree build: no device address for the look-back counter");
+        return -1;
+    }
+    hipLaunchKernelGGL(gf_k_lookback<Op>, dim3(gf_lb_blocks(n_max)), dim3(GF_LB_THREADS), 0, s,
+                       op, n_max, len, reinterpret_cast<GfLbCtl*>(temp),
+                       reinterpret_cast<unsigned long long*>(temp + GF_LB_CTL_INTS), timeouts);
+    GF_LAUNCH_CHECK();
+    return 0;
+}
 
 // Work items of several groups.  Groups (the largest nodes of at most 64
 // particles) average about 18 particles on the disk and cloud workloads, and
@@ -895,35 +960,34 @@ __global__ __launch_bounds__(TB) void k_merge_groups(const int* __restrict__ cou
     }
 }
 
-// Subset trees: the positions of the source tree's sorted order that hold
-// one of the first m particles.
-__global__ __launch_bounds__(TB) void k_subset_flag(const uint32_t* __restrict__ sidx, int n, int m,
-                                                    int* __restrict__ flag) {
-    const int p = blockIdx.x * TB + threadIdx.x;
-    if (p < n) flag[p] = int(sidx[p]) < m ? 1 : 0;
-}
-
-// Stable compaction of the subset's (key, input index), by the exclusive
-// prefix `off` of the flags; the root box is the source tree's.
-__global__ __launch_bounds__(TB) void k_subset_scatter(const uint64_t* __restrict__ skey,
-                                                       const uint32_t* __restrict__ sidx,
-                                                       const int* __restrict__ off, int n, int m,
-                                                       const double* __restrict__ root_src,
-                                                       uint64_t* __restrict__ key,
-                                                       uint32_t* __restrict__ idx,
-                                                       double* __restrict__ root) {
-    const int p = blockIdx.x * TB + threadIdx.x;
-    if (p < 4) root[p] = root_src[p];
-    if (p >= n) return;
-    const uint32_t i = sidx[p];
-    if (int(i) < m) {
-        const int d = off[p];
-        if (d < m) {
-            key[d] = skey[p];
-            idx[d] = i;
+// Subset trees: a stable compaction, in one single-pass launch, of the
+// positions of the source tree's sorted order that hold one of the first m
+// particles (their key and input index); the root box is the source tree's.
+struct SubsetOp {
+    const uint64_t* skey;
+    const uint32_t* sidx;
+    int m;
+    const double* root_src;
+    uint64_t* key;
+    uint32_t* idx;
+    double* root;
+    __device__ void values(int i, int n, int (&v)[4]) const {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = i + q < n && int(sidx[i + q]) < m;
+    }
+    __device__ void put(int i, int n, const int (&v)[4], const int (&o)[4]) const {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (i + q < n && v[q] && o[q] < m) {
+                key[o[q]] = skey[i + q];
+                idx[o[q]] = sidx[i + q];
+            }
         }
     }
-}
+    __device__ void total(int, int) const {
+        for (int c = 0; c < 4; ++c) root[c] = root_src[c];
+    }
+};
 
 }  // namespace
 
@@ -963,8 +1027,7 @@ int gf_tree_reserve(GfTree& T, int n) {
             dalloc(T.parent, nb) || dalloc(T.leaf, nb) || dalloc(T.depth, nb) ||
             dalloc(T.child8, 8 * size_t(nb)) || dalloc(T.dlist, nb) || dalloc(T.com, nb) ||
             dalloc(T.open_r, nb) || dalloc(T.box, nb) || dalloc(T.ndisp, nb) ||
-            dalloc(T.quad, 6 * size_t(nb)) ||
-            dalloc(T.gflag, size_t(nb) + 1) || dalloc(T.groups, nb))
+            dalloc(T.quad, 6 * size_t(nb)) || dalloc(T.groups, nb))
             return -1;
         T.cap_nodes = nb;
         T.built = 0;
@@ -977,9 +1040,9 @@ void gf_tree_free(GfTree& T) {
     dfree(T.chain); dfree(T.tx); dfree(T.ty); dfree(T.tz); dfree(T.tm); dfree(T.tdisp);
     dfree(T.xb); dfree(T.start); dfree(T.count); dfree(T.next); dfree(T.parent); dfree(T.leaf);
     dfree(T.depth); dfree(T.child8); dfree(T.dlist); dfree(T.dmeta); dfree(T.com); dfree(T.open_r);
-    dfree(T.box); dfree(T.ndisp); dfree(T.quad); dfree(T.gflag); dfree(T.groups); dfree(T.counters);
+    dfree(T.box); dfree(T.ndisp); dfree(T.quad); dfree(T.groups); dfree(T.counters);
     dfree(T.root); dfree(T.bbox_part); dfree(T.scan_tmp); dfree(T.sort_tmp);
-    dfree(T.skey); dfree(T.sidx); dfree(T.sub_off); dfree(T.sub_tmp);
+    dfree(T.skey); dfree(T.sidx); dfree(T.sub_tmp);
     const int keep = T.keep_sorted;
     T = GfTree();
     T.keep_sorted = keep;
@@ -1088,11 +1151,10 @@ static int build_sorted(GfTree& T, int n, const double* pos, const double* mass,
                         bool with_com) {
     const int g = gf_div_up(n, TB);
     hipLaunchKernelGGL(k_lcp, dim3(g), dim3(TB), 0, s, T.key, n, T.lcp);
-    hipLaunchKernelGGL(k_chain, dim3(g), dim3(TB), 0, s, T.lcp, n, T.chain, T.d0);
     GF_LAUNCH_CHECK();
-    if (gf_scan_excl(T.chain, T.chain, n, T.scan_tmp, s)) return -1;
+    if (lookback_launch(ChainOp{T.lcp, T.chain, T.d0, T.counters}, n, nullptr, T.scan_tmp, s))
+        return -1;
     const int ng = node_grid(T.cap_nodes);
-    hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, s, T.chain, n, T.counters);
     hipLaunchKernelGGL(k_node_init, dim3(ng), dim3(TB), 0, s, T.counters, T.child8, T.dmeta);
     hipLaunchKernelGGL(k_emit, dim3(g), dim3(TB), 0, s, T.key, T.chain, T.d0, n, T.root, T.start,
                        T.count, T.next, T.parent, T.leaf, T.depth, T.box, T.child8, T.counters);
@@ -1103,17 +1165,12 @@ static int build_sorted(GfTree& T, int n, const double* pos, const double* mass,
                        T.ty, T.tz, T.tm, T.tdisp);
     GF_LAUNCH_CHECK();
     if (with_com && com_pass(T, s)) return -1;
-    // group offsets: the node count lives on the device, so the scan runs over
-    // the node bound of this n, with zero flags past the node count
+    // groups: the node count lives on the device (counters[0]), so the launch
+    // covers the node bound of this n and only the node count's tiles take part
     const int len = gf_tree_node_bound(n);
-    hipLaunchKernelGGL(k_group_flag, dim3(ng), dim3(TB), 0, s, T.counters, T.count, T.parent,
-                       T.leaf, len, T.gflag);
-    GF_LAUNCH_CHECK();
-    // only the node count's flags matter: counters[0] bounds the scan
-    if (gf_scan_excl(T.gflag, T.gflag, len, T.scan_tmp, s, T.counters)) return -1;
-    hipLaunchKernelGGL(k_group_compact, dim3(ng), dim3(TB), 0, s, T.counters, T.count, T.parent,
-                       T.leaf, T.gflag, T.groups);
-    GF_LAUNCH_CHECK();
+    if (lookback_launch(GroupOp{T.count, T.parent, T.leaf, T.groups, T.counters}, len,
+                        T.counters, T.scan_tmp, s))
+        return -1;
     T.built = 1;
     ++T.version;
     return 0;
@@ -1137,18 +1194,13 @@ int gf_tree_build_subset(GfTree& S, const GfTree& T, int m, const double* pos, c
     if (m == 0) return 0;
     const int n = T.n;
     if (n + 1 > S.cap_sub) {
-        if (dalloc(S.sub_off, size_t(n) + 1) || dalloc(S.sub_tmp, gf_scan_temp_ints(n + 1)) ||
-            gf_scan_temp_clear(S.sub_tmp, n + 1))
+        if (dalloc(S.sub_tmp, gf_scan_temp_ints(n + 1)) || gf_scan_temp_clear(S.sub_tmp, n + 1))
             return -1;
         S.cap_sub = n + 1;
     }
-    const int g = gf_div_up(n, TB);
-    hipLaunchKernelGGL(k_subset_flag, dim3(g), dim3(TB), 0, s, T.sidx, n, m, S.sub_off);
-    GF_LAUNCH_CHECK();
-    if (gf_scan_excl(S.sub_off, S.sub_off, n, S.sub_tmp, s)) return -1;
-    hipLaunchKernelGGL(k_subset_scatter, dim3(std::max(g, 1)), dim3(TB), 0, s, T.skey, T.sidx,
-                       S.sub_off, n, m, T.root, S.key, S.idx, S.root);
-    GF_LAUNCH_CHECK();
+    if (lookback_launch(SubsetOp{T.skey, T.sidx, m, T.root, S.key, S.idx, S.root}, n, nullptr,
+                        S.sub_tmp, s))
+        return -1;
     return build_sorted(S, m, pos, mass, s, false);
 }
 
