@@ -9,8 +9,9 @@ struct GfTree {
     int n = 0;          // particles the held tree was built for
     int cap_n = 0;      // particle capacity of the buffers
     int cap_nodes = 0;  // node capacity of the buffers
-    int built = 0;      // a tree is held (refit allowed)
+    int built = 0;      // a tree is held
     int version = 0;    // counts builds and refits (the walk's packed records follow it)
+    int refit_ok = 0;   // the held tree was built with centres of mass (refit allowed)
     float theta = 0.7f; // opening angle of the held tree (cast to float, nbody.cpp:327)
 
     // per particle

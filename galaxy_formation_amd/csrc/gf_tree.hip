@@ -338,6 +338,7 @@ __global__ __launch_bounds__(TB) void k_gather(const double* __restrict__ pos,
     const double x = pos[3 * j], y = pos[3 * j + 1], z = pos[3 * j + 2];
     const double m = mass ? mass[j] : 0.0;      // no masses: a neighbour-search tree
     tx[i] = x; ty[i] = y; tz[i] = z; tm[i] = m;
+    if (!tdisp) return;                         // a tree without centres of mass
     double d = 0.0;
     if (xb) {
         const double dx = x - xb[3 * j], dy = y - xb[3 * j + 1], dz = z - xb[3 * j + 2];
@@ -1160,9 +1161,12 @@ static int build_sorted(GfTree& T, int n, const double* pos, const double* mass,
                        T.count, T.next, T.parent, T.leaf, T.depth, T.box, T.child8, T.counters);
     hipLaunchKernelGGL(k_leaf_sort, dim3(ng), dim3(TB), 0, s, T.counters, T.start, T.count,
                        T.leaf, T.idx, T.key);
-    GF_HIP(hipMemcpyAsync(T.xb, pos, sizeof(double) * 3 * size_t(n), hipMemcpyDeviceToDevice, s));
+    // the positions at the build and the displacements since then serve the
+    // centres of mass and refits only
+    if (with_com)
+        GF_HIP(hipMemcpyAsync(T.xb, pos, sizeof(double) * 3 * size_t(n), hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_gather, dim3(g), dim3(TB), 0, s, pos, mass, T.idx, n, nullptr, T.tx,
-                       T.ty, T.tz, T.tm, T.tdisp);
+                       T.ty, T.tz, T.tm, with_com ? T.tdisp : nullptr);
     GF_LAUNCH_CHECK();
     if (with_com && com_pass(T, s)) return -1;
     // groups: the node count lives on the device (counters[0]), so the launch
@@ -1172,6 +1176,7 @@ static int build_sorted(GfTree& T, int n, const double* pos, const double* mass,
                         T.counters, T.scan_tmp, s))
         return -1;
     T.built = 1;
+    T.refit_ok = with_com;
     ++T.version;
     return 0;
 }
@@ -1208,6 +1213,10 @@ int gf_tree_refit(GfTree& T, const double* pos, const double* mass, hipStream_t 
     const int n = T.n;
     if (!T.built) {
         gf_set_error("refit without a built tree");
+        return -1;
+    }
+    if (!T.refit_ok) {
+        gf_set_error("refit of a tree built without centres of mass");
         return -1;
     }
     if (n == 0) return 0;
