@@ -1,6 +1,8 @@
 // Device-wide primitives: exclusive scan of int32 and a stable LSD radix sort
 // of (uint64 key, uint32 value) pairs.  Both are written for wave64: ranks
 // inside a wave come from ballots, block-level offsets from LDS.
+#include <atomic>
+
 #include "gf_prim.h"
 #include "gf_lookback.h"
 
@@ -274,18 +276,21 @@ int gf_scan_temp_clear(int* temp, int n) {
     return 0;
 }
 
+// The counter's device address, looked up once per device (each device holds
+// its own copy of the variable) and kept.
 unsigned int* gf_prim_timeout_counter() {
-    static int dev_cached = -1;
-    static unsigned int* ptr = nullptr;
+    constexpr int kMaxDevices = 64;
+    static std::atomic<unsigned int*> ptr[kMaxDevices];
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-    if (dev != dev_cached) {
-        void* p = nullptr;
-        if (hipGetSymbolAddress(&p, HIP_SYMBOL(g_lookback_timeouts)) != hipSuccess) return nullptr;
-        ptr = static_cast<unsigned int*>(p);
-        dev_cached = dev;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    unsigned int* p = ptr[dev].load(std::memory_order_acquire);
+    if (!p) {
+        void* q = nullptr;
+        if (hipGetSymbolAddress(&q, HIP_SYMBOL(g_lookback_timeouts)) != hipSuccess) return nullptr;
+        p = static_cast<unsigned int*>(q);
+        ptr[dev].store(p, std::memory_order_release);
     }
-    return ptr;
+    return p;
 }
 
 unsigned int gf_prim_lookback_timeouts() {
