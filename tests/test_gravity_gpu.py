@@ -107,6 +107,27 @@ def test_single_pass_scans_repeat_and_never_give_up(eng):
     assert L.gf_lookback_timeouts() == 0
 
 
+EDGE = 65536 + 4096 * 4
+
+
+@pytest.mark.parametrize("n", [EDGE, EDGE + 1, EDGE + 3, EDGE + 12, EDGE + 4095])
+def test_single_pass_build_steps_at_tile_edges_equal_oracle(eng, n):
+    """Node bases, tree order and groups from the single-pass launches
+    (tiles of 4096 items, 4 a thread) equal the oracle's for particle counts
+    at and just past a tile boundary, where a thread's items and its window
+    of LCPs run past the end of the arrays."""
+    rng = np.random.default_rng(n)
+    pos = np.ascontiguousarray(rng.normal(size=(n, 3)))
+    mass = np.full(n, 1.0 / n)
+    active = np.zeros(n, np.uint8)              # the build alone: no group is walked
+    run(eng, pos, mass, 0.5, 0.01, "fp64", active=active)
+    t = eng.tree_export()
+    O.accel(pos, mass, 0.5, 0.01, active=active, mode=1)
+    o = O.tree_export()
+    for k in ("start", "count", "next", "leaf", "idx", "groups"):
+        np.testing.assert_array_equal(t[k], o[k], err_msg=k)
+
+
 def test_tree_fp64_centres_equal_oracle(eng):
     """FP64 centres of mass and opening radii are the oracle's, bit for bit
     (same summation order, no contraction)."""
