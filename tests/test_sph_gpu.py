@@ -422,18 +422,20 @@ def test_neighbours_with_small_candidate_lists(k, cap, first):
         assert st[1] > 0          # groups handed to the heap pass
 
 
+@pytest.mark.parametrize("sizes", [(15000, 15000), (20000, 65536 + 4096 * 4 + 3 - 20000)])
 @pytest.mark.parametrize("first", [None, 1.05])
-def test_gas_tree_from_the_gravity_tree_equals_oracle(first):
+def test_gas_tree_from_the_gravity_tree_equals_oracle(first, sizes):
     """The gas tree as the gravity tree's Morton order restricted to the gas
     (gf_hydro_setup_from_tree_dev: the gas is the first rows of the gravity
     tree's particles, no sort of its own): neighbours, d_k and h equal brute
     force, and density and forces the oracle's, from the cell-density start
-    and from a caller's radii."""
+    and from a caller's radii.  The second size puts the gravity tree's
+    particle count just past a 4096-item tile of the compaction."""
     import torch
 
     from galaxy_formation_amd import _lib
     L = _lib.load()
-    c = ics.make_disk(15000, 15000, seed=23)
+    c = ics.make_disk(sizes[0], sizes[1], seed=23)
     g = c.gas
     assert (g == np.arange(len(g))).all()
     n, ng, k = len(c.pos), len(g), 64
