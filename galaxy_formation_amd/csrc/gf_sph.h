@@ -26,6 +26,7 @@ struct GfSph {
                                  // position, groups, first particle, members) per item
     int *nb = nullptr;           // [k][n] neighbours: tree positions (or input indices)
     int *state = nullptr;        // pass counters (layout in gf_sph.hip)
+    int *state_host = nullptr;   // their copy at the end of a solve (pinned host memory)
     int *ovf = nullptr, *ovf2 = nullptr;
     int *gstack = nullptr;
     int cap_groups = 0;

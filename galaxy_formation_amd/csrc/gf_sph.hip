@@ -1710,6 +1710,7 @@ void gf_sph_free(GfSph& S) {
     dfree(S.rho); dfree(S.done); dfree(S.passes); dfree(S.inv); dfree(S.leaf_of); dfree(S.r_lo); dfree(S.r_hi); dfree(S.rec); dfree(S.p4); dfree(S.gmark); dfree(S.retry); dfree(S.knode);
     dfree(S.anc); dfree(S.items);
     dfree(S.nb); dfree(S.state); dfree(S.ovf); dfree(S.ovf2); dfree(S.gstack);
+    if (S.state_host) (void)hipHostFree(S.state_host);
     const int fb = S.fast_blocks, mm = S.merge_members;
     S = GfSph();
     S.fast_blocks = fb;
@@ -1856,9 +1857,12 @@ int gf_sph_knn_end(GfSph& S, int max_passes, hipStream_t s) {
         return -1;
     }
     S.knn_open = 0;
-    int st[16];
+    // the counters come back through pinned memory: one short DMA after the
+    // passes, where a pageable copy took tens of microseconds
+    if (!S.state_host) GF_HIP(hipHostMalloc(&S.state_host, 16 * sizeof(int), hipHostMallocDefault));
+    int* st = S.state_host;
     while (true) {
-        GF_HIP(hipMemcpyAsync(st, S.state, sizeof(st), hipMemcpyDeviceToHost, s));
+        GF_HIP(hipMemcpyAsync(st, S.state, 16 * sizeof(int), hipMemcpyDeviceToHost, s));
         GF_HIP(hipStreamSynchronize(s));
         if (st[4] != 0 || st[12] != 0) {
             gf_set_error("kNN: %d groups overflowed the global stack", st[4] + st[12]);
