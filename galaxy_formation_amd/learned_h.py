@@ -12,7 +12,7 @@ scales so that one network serves every resolution:
     x3 = log(dt / dt_Courant(prev))
 
 to a correction y of log d_k: the proposal is r = d_k(prev) exp(y + margin).
-The plain radius iteration (gf_hydro_knn_dev) stays the judge: a proposal only
+The plain radius iteration (gf_hydro_knn_dev) decides the result: a proposal only
 sets where the iteration starts, so neighbour sets, d_k and densities are the
 ones of the plain solve whatever the network says.
 

@@ -326,8 +326,8 @@ int gf_hydro_passes(int* per_particle, long long* total);
  * No counterpart in the reference (new component): a 4 -> 32 -> 32 -> 1 SiLU
  * MLP (GF_MLP_PARAMS = 1249 FP32 parameters: W1[32][4], b1, W2[32][32], b2,
  * W3[32], b3) on 4 BF16 features per gas particle that proposes the first
- * search radius of the neighbour solve; the plain radius iteration stays the
- * judge.  All pointers are device arrays unless marked host. */
+ * search radius of the neighbour solve; the plain radius iteration still
+ * decides the result.  All pointers are device arrays unless marked host. */
 
 /* Number of MLP parameters. */
 int gf_mlp_params(void);
